@@ -1,0 +1,219 @@
+/*
+ * shaw_amd.h -- C ABI of the MI355X-native SHAW time-stepping library.
+ *
+ * The reference (Pressio/SHAW) runs its hot path as C++ template calls inside
+ * one executable; these entry points replace those calls one for one:
+ *
+ *   shaw_app_*        ShWavePP operator assembly + mesh files
+ *                     (src/kokkos/shwavepp.hpp:100-148, 246-466;
+ *                      src/shared/mesh_helpers/)
+ *   shaw_map_*        source / receiver mapping
+ *                     (src/shared/nominal_to_grid_mappers/)
+ *   shaw_fom_*        runFom + updateVelocity/updateStress + forcing +
+ *                     seismogram/snapshot collection
+ *                     (src/kokkos/fom_run.hpp:57-161,
+ *                      src/kokkos/fom_update_kernels.hpp:60-181,
+ *                      src/kokkos/forcing_rank_{one,two}.hpp,
+ *                      src/kokkos/seismogram.hpp:192-214,
+ *                      src/kokkos/state_observer.hpp:174-203)
+ *   shaw_rom_*        Galerkin projection and reduced leapfrog
+ *                     (src/kokkos/rom_compute_jacobians.hpp:58-109,
+ *                      src/kokkos/rom_run_rank_one.hpp:83-162,
+ *                      src/kokkos/torevise/run_rom_rank_two_forcing.hpp:149-188,
+ *                      src/tools/main_reconstruct_seismo.cc:13-41)
+ *
+ * Conventions: every function returns SHAW_OK (0) or an error code and the
+ * message is available from shaw_last_error() (thread-local). No C++
+ * exception crosses this boundary. Handles own their device buffers; host
+ * buffers passed in are caller-owned and only read/written during the call.
+ * Each FOM/ROM handle owns one HIP stream; calls on one handle are not
+ * re-entrant. Arrays are C order unless stated; "LayoutLeft [a x b x c]"
+ * means the reference's Kokkos::LayoutLeft order (first index fastest),
+ * which equals C order [c][b][a].
+ */
+#ifndef SHAW_AMD_H_
+#define SHAW_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHAW_OK 0
+#define SHAW_ERR_ARG 1     /* invalid argument or input file content */
+#define SHAW_ERR_HIP 2     /* HIP runtime failure (no device, launch error) */
+#define SHAW_ERR_MEMORY 3  /* allocation failed or footprint over budget */
+#define SHAW_ERR_STATE 4   /* call out of order (e.g. advance before forcing) */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* shaw_last_error(void);
+/* Library version string. */
+const char* shaw_version(void);
+
+/* ------------------------------------------------------------------ app --
+ * Mesh + material -> matrix-free operators (the ShWavePP replacement).   */
+typedef struct shaw_app* shaw_app_t;
+
+/* Material kinds and their `params` flattening:
+ *   0 unilayer : rho c0,c1,c2, vs c0,c1,c2  (polynomials in depth [m])
+ *   1 bilayer  : depth2 [km], layer1 rho c0..c2, vs c0..c2, layer2 rho c0..c2, vs c0..c2
+ *   2 prem     : (none)
+ *   3 layered  : n, radius_0..radius_{n-1} [km, ascending], rho_0.. [kg/m^3], vs_0.. [m/s]
+ *                (piecewise linear in radius between nodes, constant outside)
+ * material_model_unilayer.hpp:72-83, material_model_bilayer.hpp:74-101,
+ * material_model_prem.hpp:85-176.                                          */
+typedef struct {
+  int kind;
+  int nparams;
+  const double* params;
+} shaw_material;
+
+typedef struct {
+  uint64_t nr, nth, nvp, nsp;
+  double th_left_deg, th_right_deg, r_cmb_m, r_surf_m;
+  double dth, drr;               /* [rad], [m] */
+  double min_vs, max_vs;         /* over stress points, as ShWavePP */
+} shaw_app_info;
+
+/* Read mesh_info.dat / graph_vp.dat / coeff_vp.dat / graph_sp.dat from
+ * mesh_dir (ShWavePP ctor, shwavepp.hpp:100-148). Fails if the graph is not
+ * the structured full mesh meshing/ writes.                                */
+int shaw_app_create_from_mesh(const char* mesh_dir, const shaw_material* mat, shaw_app_t* out);
+/* Same operators for a mesh generated in memory (angles deg, radii km). */
+int shaw_app_create_structured(uint64_t nr, uint64_t nth, double th_left_deg, double th_right_deg,
+                               double r_cmb_km, double r_surf_km, const shaw_material* mat,
+                               shaw_app_t* out);
+/* Write the four mesh files exactly as meshing/create_single_mesh.py does. */
+int shaw_mesh_write(const char* dir, uint64_t nr, uint64_t nth, double th_left_deg,
+                    double th_right_deg, double r_cmb_km, double r_surf_km);
+int shaw_app_info_get(shaw_app_t app, shaw_app_info* info);
+/* coef_vp [nvp][4] (slots N,S,W,E), coef_sp [nsp][2], rho_inv [nvp], shear [nsp];
+ * any pointer may be NULL.                                                */
+int shaw_app_operators(shaw_app_t app, double* coef_vp, double* coef_sp, double* rho_inv,
+                       double* shear_mod);
+/* coords [n][2] = (theta [rad], r [m]) for dof 0 = vp, 1 = sp. */
+int shaw_app_coords(shaw_app_t app, int dof, double* coords);
+/* Nearest vp gid (ties -> last gid); sources on the theta edges move one
+ * cell inward (map_point_source_to_velocity_grid_point.hpp:61-133,
+ * map_nominal_location_to_velocity_grid_point.hpp:57-117).                */
+int shaw_map_source(shaw_app_t app, double depth_km, double angle_deg, uint64_t* vp_gid);
+int shaw_map_receiver(shaw_app_t app, double angle_deg, uint64_t* vp_gid);
+void shaw_app_destroy(shaw_app_t app);
+
+/* Signal kinds (signal.hpp:86-128): 0 ricker, 1 sinusoid, 2 gaussDer.
+ * table[(s-1)*m + k] = f_k((s-1) dt), s = 1..nsteps, on the host with libm. */
+int shaw_forcing_table(int m, const int* kinds, const double* delays, const double* periods,
+                       double dt, uint64_t nsteps, double* table);
+
+/* ------------------------------------------------------------------ FOM --
+ * Full-order leapfrog on one GPU: x_v += dt J_vp x_s + dt rho^-1 f, then
+ * x_s += dt J_sp x_v, for m forcing columns (m = 1 is the rank-1 state).   */
+typedef struct shaw_fom* shaw_fom_t;
+
+#define SHAW_FP64 0
+#define SHAW_FP32 1
+#define SHAW_KERNEL_AUTO 0   /* fused one-sweep step (ping-pong states) */
+#define SHAW_KERNEL_SPLIT 1  /* velocity and stress half-step kernels, in place */
+#define SHAW_KERNEL_FUSED 2
+
+typedef struct {
+  int precision;        /* SHAW_FP64 | SHAW_FP32 (state, coefficients, arithmetic) */
+  int kernel;           /* SHAW_KERNEL_* */
+  int rank1_forcing;    /* 1: add (dt*rho_inv)*f as KokkosBlas::mult (rank-1 form);
+                           0: add (rho_inv*f)*dt as AddForcingRank2 (rank-2 form) */
+  int cols_per_thread;  /* 0 = default; else 1, 2, 4 or 8 (must divide the column block) */
+  int strip;            /* 0 = default; fused kernel: vp points per block row (incl. halo) */
+  int segment;          /* 0 = default; fused kernel: radial rows per block */
+} shaw_fom_options;
+
+/* app: operators; m: forcing columns; dt: time step. opts may be NULL. */
+int shaw_fom_create(shaw_app_t app, uint64_t m, double dt, const shaw_fom_options* opts,
+                    shaw_fom_t* out);
+/* vp gid of each column's point source (RankTwoForcing / RankOneForcing). */
+int shaw_fom_set_sources(shaw_fom_t h, const uint64_t* vp_gid);
+/* Forcing values [nsteps][m] for steps 1..nsteps (see shaw_forcing_table);
+ * uploaded once, read by the velocity kernels.                             */
+int shaw_fom_set_forcing(shaw_fom_t h, uint64_t nsteps, const double* table);
+/* Record x_v at r receivers every `every` steps (step % every == 0), up to
+ * `max_samples` samples; a device buffer LayoutLeft [r x T x m].          */
+int shaw_fom_set_receivers(shaw_fom_t h, uint64_t r, const uint64_t* vp_gid, uint64_t every,
+                           uint64_t max_samples);
+/* Snapshot x_v (dof 0) or x_s (dof 1) every `every` steps for the listed
+ * forcing columns into a device buffer LayoutLeft [n x T x ncols].        */
+int shaw_fom_set_snapshots(shaw_fom_t h, int dof, uint64_t every, uint64_t ncols,
+                           const uint64_t* cols, uint64_t max_samples);
+/* Zero the state and the step/sample counters (fom_run.hpp:76-77). */
+int shaw_fom_reset(shaw_fom_t h);
+/* Enqueue nsteps leapfrog steps on the handle's stream. */
+int shaw_fom_advance(shaw_fom_t h, uint64_t nsteps);
+/* Wait for the handle's stream. */
+int shaw_fom_sync(shaw_fom_t h);
+/* Steps taken since reset. */
+int shaw_fom_step(shaw_fom_t h, uint64_t* step);
+/* Copy out the seismogram LayoutLeft [r x T x m] (T = samples so far). */
+int shaw_fom_get_seismogram(shaw_fom_t h, double* out, uint64_t* samples);
+int shaw_fom_get_snapshots(shaw_fom_t h, int dof, double* out, uint64_t* samples);
+/* Current state, C order [nvp][m] and [nsp][m] (either may be NULL). */
+int shaw_fom_get_state(shaw_fom_t h, double* vp, double* sp);
+int shaw_fom_set_state(shaw_fom_t h, const double* vp, const double* sp);
+/* Time `nsteps` steps with HIP events on the handle's stream: total ms and
+ * the average duration of one leapfrog step (all its kernels).            */
+int shaw_fom_time(shaw_fom_t h, uint64_t nsteps, double* total_ms, double* ms_per_step);
+/* Algorithmic HBM bytes of one leapfrog step of the selected kernel, and
+ * the reference model s*m*3*(nvp+nsp) + 8*(4*nvp + 2*nsp).               */
+int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel_bytes, double* model_bytes);
+/* Kernel configuration actually used: kernel, cols_per_thread, strip, segment. */
+int shaw_fom_config(shaw_fom_t h, int* kernel, int* cols_per_thread, int* strip, int* segment);
+void shaw_fom_destroy(shaw_fom_t h);
+
+/* ------------------------------------------------------------------ ROM --
+ * Galerkin ROM. Projection in FP64 on the GPU; the reduced leapfrog for m
+ * forcing columns in FP64 or BF16x3 split precision with FP32 accumulation
+ * on MFMA, FP64 state accumulation.                                       */
+typedef struct shaw_rom* shaw_rom_t;
+
+#define SHAW_ROM_FP64 0
+#define SHAW_ROM_BF16X3 1
+
+/* Ar_v = Phi_v^T (J_vp Phi_s) [kv x ks], Ar_s = Phi_s^T (J_sp Phi_v) [ks x kv],
+ * row-major outputs; phi_v LayoutLeft [nvp x kv], phi_s LayoutLeft [nsp x ks]
+ * (the lsv_* file layout).                                                */
+int shaw_rom_project(shaw_app_t app, const double* phi_v, uint64_t kv, const double* phi_s,
+                     uint64_t ks, double* ar_v, double* ar_s);
+
+typedef struct {
+  uint64_t kv, ks;
+  double dt;
+  int precision;         /* SHAW_ROM_FP64 | SHAW_ROM_BF16X3 */
+  const double* ar_v;    /* [kv][ks] */
+  const double* ar_s;    /* [ks][kv] */
+  uint64_t r;            /* receivers */
+  const double* c;       /* [r][kv] = rows of Phi_v at the receiver gids */
+  uint64_t max_cols;     /* resident column slots */
+  uint64_t max_samples;  /* seismogram samples per column */
+} shaw_rom_desc;
+
+int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out);
+/* Admit n columns into free slots: b [kv][n] (rho_inv[g] Phi_v[g,:] per
+ * column), forcing values [nsteps][n], horizons (steps per column), sample
+ * period `every`. slots_out[n] receives the slot ids. Fails with
+ * SHAW_ERR_MEMORY when not enough slots are free.                          */
+int shaw_rom_admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const double* ftab,
+                   const uint64_t* horizons, uint64_t every, uint64_t* slots_out);
+/* Advance every active column by up to nsteps (columns stop at their horizon). */
+int shaw_rom_advance(shaw_rom_t h, uint64_t nsteps);
+int shaw_rom_sync(shaw_rom_t h);
+/* Copy a column's seismogram LayoutLeft [r x T] and final reduced states
+ * (any output may be NULL), then free the slot.                            */
+int shaw_rom_retire(shaw_rom_t h, uint64_t slot, double* seis, double* qv, double* qs,
+                    uint64_t* samples);
+int shaw_rom_free_slots(shaw_rom_t h, uint64_t* n);
+int shaw_rom_time(shaw_rom_t h, uint64_t nsteps, double* total_ms);
+void shaw_rom_destroy(shaw_rom_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHAW_AMD_H_ */

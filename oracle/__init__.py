@@ -41,6 +41,9 @@ def _load():
     lib.orc_last_error.restype = C.c_char_p
     lib.orc_app_create.restype = C.c_void_p
     lib.orc_app_create.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_int]
+    lib.orc_app_create_structured.restype = C.c_void_p
+    lib.orc_app_create_structured.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                              C.c_double, C.c_int, C.c_void_p, C.c_int]
     lib.orc_app_destroy.argtypes = [C.c_void_p]
     lib.orc_app_info.argtypes = [C.c_void_p, _up, _dp]
     lib.orc_app_csr.argtypes = [C.c_void_p, C.c_int, _up, _up, _dp]
@@ -95,11 +98,18 @@ def material_params(mat: dict) -> tuple[int, np.ndarray]:
 class App:
     """Operators assembled by the oracle from a mesh directory (CSR, as ShWavePP)."""
 
-    def __init__(self, mesh_dir: str, material: dict):
+    def __init__(self, mesh_dir: str, material: dict, structured=None):
+        """structured = (nr, nth, thL_deg, thR_deg, rCmb_km, rSurf_km) builds the
+        mesh in memory (restated generator) instead of reading mesh_dir."""
         lib = _load()
         kind, params = material_params(material)
         self._params = np.ascontiguousarray(params)
-        h = lib.orc_app_create(mesh_dir.encode(), kind, _ptr(self._params), len(self._params))
+        if structured is None:
+            h = lib.orc_app_create(mesh_dir.encode(), kind, _ptr(self._params), len(self._params))
+        else:
+            nr, nth, thl, thr, rc, rs = structured
+            h = lib.orc_app_create_structured(nr, nth, thl, thr, rc, rs, kind, _ptr(self._params),
+                                              len(self._params))
         if not h:
             raise RuntimeError(lib.orc_last_error().decode())
         self._h = h

@@ -369,6 +369,94 @@ void* orc_app_create(const char* meshDir, int matKind, const double* params, int
   } catch (const std::exception& e) { setErr(e); return nullptr; }
 }
 
+// The same App for a mesh generated in memory by restating the Python mesh
+// generator cell by cell: meshing/helpers/full_mesh_utils.py:8-89 (cells in
+// natural order, dof gids vp/srp/stp), unit_cell.py:58-66 (gid counters),
+// build_graph_impl.py:7-167 (neighbours and +-1/0 multipliers),
+// write_files.py (values printed with %.20f and parsed back).
+void* orc_app_create_structured(uint64_t nr, uint64_t nth, double thLDeg, double thRDeg, double rCmbKm,
+                                double rSurfKm, int matKind, const double* params, int nParams) {
+  try {
+    App* a = new App();
+    auto rt = [](double x) {  // %.20f print + strtod parse of the mesh files
+      // 20 decimals carry >= 17 significant digits for |x| >= 1e-3: exact round trip
+      if (x == 0.0 || std::abs(x) >= 1e-3) return x;
+      char buf[64];
+      std::snprintf(buf, sizeof buf, "%.20f", x);
+      return std::strtod(buf, nullptr);
+    };
+    const double thL = thLDeg * M_PI / 180., thR = thRDeg * M_PI / 180.;
+    const double dth = (thR - thL) / (double)(nth - 1), dr = (rSurfKm - rCmbKm) / (double)(nr - 1);
+    a->bounds[0] = thLDeg; a->bounds[1] = thRDeg;
+    a->bounds[2] = rCmbKm * 1000.0; a->bounds[3] = rSurfKm * 1000.0;
+    a->dth = rt(dth); a->dthInv = 1.0 / a->dth;
+    a->drr = rt(dr) * 1000.0; a->drrInv = 1.0 / a->drr;
+    a->nr = nr; a->nth = nth;
+    // pass 1: gids of every cell's dofs (vp, then srp if j<nr-1, then stp if i<nth-1)
+    const size_t ncell = nr * nth;
+    std::vector<long> gvp(ncell), gsrp(ncell, -1), gstp(ncell, -1);
+    long cv = -1, cs = -1;
+    for (size_t j = 0; j < nr; ++j)
+      for (size_t i = 0; i < nth; ++i) {
+        const size_t k = j * nth + i;
+        gvp[k] = ++cv;
+        if (j < nr - 1) gsrp[k] = ++cs;
+        if (i < nth - 1) gstp[k] = ++cs;
+      }
+    a->nVp = cv + 1; a->nSp = cs + 1;
+    a->gVp.assign(a->nVp * 5, 0); a->cVp.assign(a->nVp * 2, 0); a->cotVp.assign(a->nVp, 0);
+    a->coeffVp.assign(a->nVp * 4, 0);
+    a->gSp.assign(a->nSp * 3, 0); a->cSp.assign(a->nSp * 2, 0); a->cotSp.assign(a->nSp, 0);
+    a->labels.assign(a->nSp, 0);
+    auto setPt = [&](bool sp, size_t gid, double th, double rKm, bool onSym) {
+      auto& c = sp ? a->cSp : a->cVp;
+      auto& cot = sp ? a->cotSp : a->cotVp;
+      c[gid * 2] = rt(th);
+      c[gid * 2 + 1] = 1.0 / (rt(rKm) * 1000.0);
+      cot[gid] = onSym ? 0.0 : computeCotangent(c[gid * 2]);
+    };
+    for (size_t j = 0; j < nr; ++j)
+      for (size_t i = 0; i < nth; ++i) {
+        const size_t k = j * nth + i;
+        const double th = thL + i * dth, r = rCmbKm + j * dr;
+        const bool sym = (i == 0 || i == nth - 1);
+        // vp: W, N, E, S with multipliers (build_graph_impl.py:57-120)
+        long W, N, E, S;
+        double c[4];
+        if (i >= 1) { W = gstp[k - 1]; c[0] = 1.; } else { W = gstp[k]; c[0] = 0.; }
+        if (i < nth - 1) { E = gstp[k]; c[2] = 1.; } else { E = gstp[k - 1]; c[2] = 0.; }
+        if (j < nr - 1) { N = gsrp[k]; c[1] = 1.; } else { N = gsrp[k - nth]; c[1] = -1.; }
+        if (j >= 1) { S = gsrp[k - nth]; c[3] = 1.; } else { S = gsrp[k]; c[3] = -1.; }
+        if (sym) c[0] = c[1] = c[2] = c[3] = 0.;
+        const size_t g = gvp[k];
+        size_t* gr = &a->gVp[g * 5];
+        gr[0] = g; gr[1] = W; gr[2] = N; gr[3] = E; gr[4] = S;
+        for (int q = 0; q < 4; ++q) a->coeffVp[g * 4 + q] = c[q];
+        setPt(false, g, th, r, sym);
+        if (j < nr - 1) {  // srp: north vp, south vp
+          const size_t q = gsrp[k];
+          a->gSp[q * 3] = q; a->gSp[q * 3 + 1] = gvp[k + nth]; a->gSp[q * 3 + 2] = gvp[k];
+          a->labels[q] = 1;
+          setPt(true, q, th, r + dr * 0.5, sym);
+        }
+        if (i < nth - 1) {  // stp: west vp, east vp
+          const size_t q = gstp[k];
+          a->gSp[q * 3] = q; a->gSp[q * 3 + 1] = gvp[k]; a->gSp[q * 3 + 2] = gvp[k + 1];
+          a->labels[q] = 2;
+          setPt(true, q, th + dth * 0.5, r, false);
+        }
+      }
+    Material m;
+    m.kind = matKind;
+    m.p.assign(params, params + nParams);
+    m.rSurfM = a->bounds[3];
+    setMaterial(*a, m);
+    fillVp(*a);
+    fillSp(*a);
+    return a;
+  } catch (const std::exception& e) { setErr(e); return nullptr; }
+}
+
 void orc_app_destroy(void* h) { delete (App*)h; }
 
 // info: [nVp, nSp, nth, nr, nnzVp, nnzSp]; dinfo: [thL, thR, rCmb, rSurf, dth, drr, minVs, maxVs]

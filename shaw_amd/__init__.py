@@ -1,0 +1,388 @@
+"""MI355X-native SHAW time-stepping path: Python front end over the C ABI.
+
+The library (``shaw_amd/libshaw_amd.so``: gfx950 HIP kernels + C++ host
+layer, declared in ``include/shaw_amd.h``) is loaded with ctypes. There is
+no fallback: if the library is missing, or no GPU is present when a GPU
+entry point is called, the call raises.
+
+Mirrors of the reference's in-process interface (src/kokkos/*):
+  App            ShWavePP (operator assembly from mesh files, shwavepp.hpp)
+  App.map_*      source / receiver mappers (nominal_to_grid_mappers/)
+  forcing_table  RankOne/RankTwoForcing signal evaluation (signal.hpp)
+  Fom            runFom + updateVelocity/updateStress (fom_run.hpp,
+                 fom_update_kernels.hpp), seismogram.hpp, state_observer.hpp
+  Rom            rom_compute_jacobians.hpp + the reduced leapfrog
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libshaw_amd.so")
+EXE_PATH = os.path.join(HERE, "shawExe")
+
+FP64, FP32 = 0, 1
+KERNEL_AUTO, KERNEL_SPLIT, KERNEL_FUSED = 0, 1, 2
+ROM_FP64, ROM_BF16X3 = 0, 1
+SIGNAL_KINDS = {"ricker": 0, "sinusoid": 1, "gaussDer": 2}
+
+
+class ShawError(RuntimeError):
+    pass
+
+
+class _Material(C.Structure):
+    _fields_ = [("kind", C.c_int), ("nparams", C.c_int), ("params", C.POINTER(C.c_double))]
+
+
+class _AppInfo(C.Structure):
+    _fields_ = [("nr", C.c_uint64), ("nth", C.c_uint64), ("nvp", C.c_uint64), ("nsp", C.c_uint64),
+                ("th_left_deg", C.c_double), ("th_right_deg", C.c_double), ("r_cmb_m", C.c_double),
+                ("r_surf_m", C.c_double), ("dth", C.c_double), ("drr", C.c_double),
+                ("min_vs", C.c_double), ("max_vs", C.c_double)]
+
+
+class _FomOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("kernel", C.c_int), ("rank1_forcing", C.c_int),
+                ("cols_per_thread", C.c_int), ("strip", C.c_int), ("segment", C.c_int)]
+
+
+class _RomDesc(C.Structure):
+    _fields_ = [("kv", C.c_uint64), ("ks", C.c_uint64), ("dt", C.c_double), ("precision", C.c_int),
+                ("ar_v", C.c_void_p), ("ar_s", C.c_void_p), ("r", C.c_uint64), ("c", C.c_void_p),
+                ("max_cols", C.c_uint64), ("max_samples", C.c_uint64)]
+
+
+_lib = None
+
+
+def build() -> None:
+    """Compile libshaw_amd.so and shawExe for gfx950 (hipcc; no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", HERE, "-j8"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ShawError(f"{LIB_PATH} is not built: run `make -C shaw_amd` (or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, u64, dbl, i32 = C.c_void_p, C.c_uint64, C.c_double, C.c_int
+    P = C.POINTER
+    sig = {
+        "shaw_last_error": (C.c_char_p, []),
+        "shaw_version": (C.c_char_p, []),
+        "shaw_app_create_from_mesh": (i32, [C.c_char_p, P(_Material), P(vp)]),
+        "shaw_app_create_structured": (i32, [u64, u64, dbl, dbl, dbl, dbl, P(_Material), P(vp)]),
+        "shaw_mesh_write": (i32, [C.c_char_p, u64, u64, dbl, dbl, dbl, dbl]),
+        "shaw_app_info_get": (i32, [vp, P(_AppInfo)]),
+        "shaw_app_operators": (i32, [vp, vp, vp, vp, vp]),
+        "shaw_app_coords": (i32, [vp, i32, vp]),
+        "shaw_map_source": (i32, [vp, dbl, dbl, P(u64)]),
+        "shaw_map_receiver": (i32, [vp, dbl, P(u64)]),
+        "shaw_app_destroy": (None, [vp]),
+        "shaw_forcing_table": (i32, [i32, vp, vp, vp, dbl, u64, vp]),
+        "shaw_fom_create": (i32, [vp, u64, dbl, P(_FomOptions), P(vp)]),
+        "shaw_fom_set_sources": (i32, [vp, vp]),
+        "shaw_fom_set_forcing": (i32, [vp, u64, vp]),
+        "shaw_fom_set_receivers": (i32, [vp, u64, vp, u64, u64]),
+        "shaw_fom_set_snapshots": (i32, [vp, i32, u64, u64, vp, u64]),
+        "shaw_fom_reset": (i32, [vp]),
+        "shaw_fom_advance": (i32, [vp, u64]),
+        "shaw_fom_sync": (i32, [vp]),
+        "shaw_fom_step": (i32, [vp, P(u64)]),
+        "shaw_fom_get_seismogram": (i32, [vp, vp, P(u64)]),
+        "shaw_fom_get_snapshots": (i32, [vp, i32, vp, P(u64)]),
+        "shaw_fom_get_state": (i32, [vp, vp, vp]),
+        "shaw_fom_set_state": (i32, [vp, vp, vp]),
+        "shaw_fom_time": (i32, [vp, u64, P(dbl), P(dbl)]),
+        "shaw_fom_bytes_per_step": (i32, [vp, P(dbl), P(dbl)]),
+        "shaw_fom_config": (i32, [vp, P(i32), P(i32), P(i32), P(i32)]),
+        "shaw_fom_destroy": (None, [vp]),
+    }
+    optional = {
+        "shaw_rom_project": (i32, [vp, vp, u64, vp, u64, vp, vp]),
+        "shaw_rom_create": (i32, [P(_RomDesc), P(vp)]),
+        "shaw_rom_admit": (i32, [vp, u64, vp, u64, vp, vp, u64, vp]),
+        "shaw_rom_advance": (i32, [vp, u64]),
+        "shaw_rom_sync": (i32, [vp]),
+        "shaw_rom_retire": (i32, [vp, u64, vp, vp, vp, P(u64)]),
+        "shaw_rom_free_slots": (i32, [vp, P(u64)]),
+        "shaw_rom_time": (i32, [vp, u64, P(dbl)]),
+        "shaw_rom_destroy": (None, [vp]),
+    }
+    for name, (res, args) in list(sig.items()) + list(optional.items()):
+        if not hasattr(L, name):
+            if name in sig:
+                raise ShawError(f"{LIB_PATH} does not export {name}")
+            continue
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise ShawError(f"[{rc}] {lib().shaw_last_error().decode()}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u64(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+# ---------------------------------------------------------------- material
+def material_params(mat: dict) -> tuple[int, np.ndarray]:
+    """Flatten a ``material`` input section (parser_material_model.hpp) into
+    the C ABI's (kind, params)."""
+    kind = str(mat["kind"])
+    pad = lambda v: [float(x) for x in v] + [0.0] * (3 - len(v))
+    k = kind.lower()
+    if k == "unilayer":
+        lay = mat["layer"]
+        return 0, np.array(pad(lay["density"]) + pad(lay["velocity"]))
+    if k == "bilayer":
+        l1, l2 = mat["layer1"], mat["layer2"]
+        return 1, np.array([float(l2["depth"])] + pad(l1["density"]) + pad(l1["velocity"])
+                           + pad(l2["density"]) + pad(l2["velocity"]))
+    if k == "prem":
+        return 2, np.zeros(0)
+    if k == "layered":
+        r = [float(x) for x in mat["radius"]]
+        return 3, np.array([float(len(r))] + r + [float(x) for x in mat["density"]]
+                           + [float(x) for x in mat["velocity"]])
+    raise ShawError(f"Unknown material model kind '{kind}'")
+
+
+def _material_struct(mat: dict):
+    kind, params = material_params(mat)
+    params = _f64(params)
+    m = _Material(kind, len(params), params.ctypes.data_as(C.POINTER(C.c_double)))
+    return m, params  # keep params alive
+
+
+def mesh_write(directory: str, nr: int, nth: int, th_left=0.0, th_right=180.0, r_cmb=3480.0, r_surf=6371.0):
+    """Write mesh_info.dat/graph_vp.dat/coeff_vp.dat/graph_sp.dat as meshing/ does."""
+    os.makedirs(directory, exist_ok=True)
+    _check(lib().shaw_mesh_write(directory.encode(), nr, nth, th_left, th_right, r_cmb, r_surf))
+
+
+def forcing_table(kinds, delays, periods, dt: float, nsteps: int) -> np.ndarray:
+    """[nsteps][m] forcing values f_m((s-1) dt) for steps s = 1..nsteps."""
+    kinds = np.ascontiguousarray([SIGNAL_KINDS[k] if isinstance(k, str) else int(k) for k in kinds], dtype=np.int32)
+    delays, periods = _f64(delays), _f64(periods)
+    m = len(kinds)
+    out = np.zeros((nsteps, m))
+    _check(lib().shaw_forcing_table(m, _p(kinds), _p(delays), _p(periods), dt, nsteps, _p(out)))
+    return out
+
+
+class App:
+    """Operators of one mesh + material (the ShWavePP replacement)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        info = _AppInfo()
+        _check(lib().shaw_app_info_get(self._h, C.byref(info)))
+        self.nr, self.nth, self.nVp, self.nSp = int(info.nr), int(info.nth), int(info.nvp), int(info.nsp)
+        self.bounds = np.array([info.th_left_deg, info.th_right_deg, info.r_cmb_m, info.r_surf_m])
+        self.dth, self.drr = info.dth, info.drr
+        self.minVs, self.maxVs = info.min_vs, info.max_vs
+
+    @classmethod
+    def from_mesh(cls, mesh_dir: str, material: dict) -> "App":
+        m, keep = _material_struct(material)
+        h = C.c_void_p()
+        _check(lib().shaw_app_create_from_mesh(mesh_dir.encode(), C.byref(m), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def structured(cls, nr: int, nth: int, material: dict, th_left=0.0, th_right=180.0,
+                   r_cmb=3480.0, r_surf=6371.0) -> "App":
+        m, keep = _material_struct(material)
+        h = C.c_void_p()
+        _check(lib().shaw_app_create_structured(nr, nth, th_left, th_right, r_cmb, r_surf,
+                                                C.byref(m), C.byref(h)))
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.shaw_app_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def operators(self):
+        cv = np.zeros((self.nVp, 4))
+        cs = np.zeros((self.nSp, 2))
+        ri = np.zeros(self.nVp)
+        g = np.zeros(self.nSp)
+        _check(lib().shaw_app_operators(self._h, _p(cv), _p(cs), _p(ri), _p(g)))
+        return cv, cs, ri, g
+
+    def coords(self, dof: str) -> np.ndarray:
+        n = self.nVp if dof == "vp" else self.nSp
+        c = np.zeros((n, 2))
+        _check(lib().shaw_app_coords(self._h, 0 if dof == "vp" else 1, _p(c)))
+        return c
+
+    def map_source(self, depth_km: float, angle_deg: float) -> int:
+        g = C.c_uint64()
+        _check(lib().shaw_map_source(self._h, depth_km, angle_deg, C.byref(g)))
+        return int(g.value)
+
+    def map_receiver(self, angle_deg: float) -> int:
+        g = C.c_uint64()
+        _check(lib().shaw_map_receiver(self._h, angle_deg, C.byref(g)))
+        return int(g.value)
+
+
+class Fom:
+    """Full-order leapfrog for m forcing columns on the GPU."""
+
+    def __init__(self, app: App, m: int, dt: float, precision=FP64, kernel=KERNEL_AUTO,
+                 rank1_forcing=None, cols_per_thread=0, strip=0, segment=0):
+        if rank1_forcing is None:
+            rank1_forcing = m == 1
+        o = _FomOptions(precision, kernel, int(bool(rank1_forcing)), cols_per_thread, strip, segment)
+        h = C.c_void_p()
+        _check(lib().shaw_fom_create(app.handle, m, dt, C.byref(o), C.byref(h)))
+        self._h, self.app, self.m, self.dt = h, app, m, dt
+        self.R = 0
+        self._snap_cols = {0: 0, 1: 0}
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.shaw_fom_destroy(self._h)
+            self._h = None
+
+    def close(self):
+        self.__del__()
+
+    def set_sources(self, gids):
+        g = _u64(gids)
+        assert len(g) == self.m
+        _check(lib().shaw_fom_set_sources(self._h, _p(g)))
+
+    def set_forcing(self, table):
+        t = _f64(table)
+        assert t.ndim == 2 and t.shape[1] == self.m
+        _check(lib().shaw_fom_set_forcing(self._h, t.shape[0], _p(t)))
+
+    def set_receivers(self, gids, every: int, max_samples: int):
+        g = _u64(gids)
+        self.R = len(g)
+        _check(lib().shaw_fom_set_receivers(self._h, len(g), _p(g), every, max_samples))
+
+    def set_snapshots(self, dof: str, every: int, cols, max_samples: int):
+        c = _u64(cols)
+        d = 0 if dof == "vp" else 1
+        self._snap_cols[d] = len(c)
+        _check(lib().shaw_fom_set_snapshots(self._h, d, every, len(c), _p(c), max_samples))
+
+    def reset(self):
+        _check(lib().shaw_fom_reset(self._h))
+
+    def advance(self, nsteps: int):
+        _check(lib().shaw_fom_advance(self._h, nsteps))
+
+    def sync(self):
+        _check(lib().shaw_fom_sync(self._h))
+
+    @property
+    def step(self) -> int:
+        s = C.c_uint64()
+        _check(lib().shaw_fom_step(self._h, C.byref(s)))
+        return int(s.value)
+
+    def seismogram(self) -> np.ndarray:
+        """[m][T][R] (= LayoutLeft [R x T x m])."""
+        every_max = max(1, self.step)
+        out = np.zeros(max(1, self.R * every_max * self.m))
+        t = C.c_uint64()
+        _check(lib().shaw_fom_get_seismogram(self._h, _p(out), C.byref(t)))
+        T = int(t.value)
+        return out[: self.R * T * self.m].reshape(self.m, T, self.R)
+
+    def snapshots(self, dof: str) -> np.ndarray:
+        """[ncols][T][n] (= LayoutLeft [n x T x ncols])."""
+        d = 0 if dof == "vp" else 1
+        n = self.app.nVp if d == 0 else self.app.nSp
+        nc = self._snap_cols[d]
+        out = np.zeros(max(1, n * max(1, self.step) * nc))
+        t = C.c_uint64()
+        _check(lib().shaw_fom_get_snapshots(self._h, d, _p(out), C.byref(t)))
+        T = int(t.value)
+        return out[: n * T * nc].reshape(nc, T, n)
+
+    def state(self):
+        vp = np.zeros((self.app.nVp, self.m))
+        sp = np.zeros((self.app.nSp, self.m))
+        _check(lib().shaw_fom_get_state(self._h, _p(vp), _p(sp)))
+        return vp, sp
+
+    def set_state(self, vp, sp):
+        _check(lib().shaw_fom_set_state(self._h, _p(_f64(vp)), _p(_f64(sp))))
+
+    def time(self, nsteps: int):
+        tot, per = C.c_double(), C.c_double()
+        _check(lib().shaw_fom_time(self._h, nsteps, C.byref(tot), C.byref(per)))
+        return tot.value, per.value
+
+    def bytes_per_step(self):
+        k, m = C.c_double(), C.c_double()
+        _check(lib().shaw_fom_bytes_per_step(self._h, C.byref(k), C.byref(m)))
+        return k.value, m.value
+
+    def config(self):
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().shaw_fom_config(self._h, *[C.byref(x) for x in v]))
+        return {"kernel": v[0].value, "cols_per_thread": v[1].value, "strip": v[2].value, "segment": v[3].value}
+
+
+def fom_run(app: App, dt: float, nsteps: int, sources, rank2: bool, receivers=(), seis_freq=0,
+            snap_freq_vp=0, snap_freq_sp=0, signal="ricker", **fom_kw):
+    """One runFom call (fom_run.hpp:57-161) on the GPU for the realizations
+    ``sources`` = [(depth_km, angle_deg, period_s, delay_s), ...]; same result
+    layout as the oracle: seismogram [m][T][R], snaps [m][T][n], final states."""
+    m = len(sources)
+    if not rank2 and m != 1:
+        raise ShawError("rank-1 run needs one source")
+    fom = Fom(app, m, dt, rank1_forcing=not rank2, **fom_kw)
+    fom.set_sources([app.map_source(d, a) for d, a, p, dl in sources])
+    fom.set_forcing(forcing_table([signal] * m, [s[3] for s in sources], [s[2] for s in sources], dt, nsteps))
+    if seis_freq and len(receivers):
+        fom.set_receivers([app.map_receiver(a) for a in receivers], seis_freq, nsteps // seis_freq)
+    cols = list(range(m))
+    if snap_freq_vp:
+        fom.set_snapshots("vp", snap_freq_vp, cols, nsteps // snap_freq_vp)
+    if snap_freq_sp:
+        fom.set_snapshots("sp", snap_freq_sp, cols, nsteps // snap_freq_sp)
+    fom.advance(nsteps)
+    fom.sync()
+    vp, sp = fom.state()
+    out = {"final_vp": vp, "final_sp": sp}
+    if seis_freq and len(receivers):
+        out["seismogram"] = fom.seismogram()
+    if snap_freq_vp:
+        out["snaps_vp"] = fom.snapshots("vp")
+    if snap_freq_sp:
+        out["snaps_sp"] = fom.snapshots("sp")
+    fom.close()
+    return out

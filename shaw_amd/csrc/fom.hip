@@ -1,0 +1,735 @@
+// FOM leapfrog on gfx950: matrix-free stencil kernels for
+//   x_v += dt J_vp x_s + dt rho^-1 f      (fom_update_kernels.hpp:60-90, 133-167)
+//   x_s += dt J_sp x_v                     (fom_update_kernels.hpp:92-104, 169-181)
+// for m forcing columns, plus the engine behind the shaw_fom_* C ABI.
+//
+// HBM layout. States are stored in column blocks of MC forcing columns:
+//   x[b][gid][c]  (b = column / MC, c = column % MC), gids as in the mesh
+// files, so MC*sizeof(T) contiguous bytes per grid point (256 B for the FP64
+// MC = 32 case). Coefficients: coefVp[gid][4] (slots N, S, W, E with merged
+// entries in N / W), coefSp[gid][2]. Neighbours are addressed by the
+// structured index formulas of the mesh (shaw_host.hpp).
+//
+// Arithmetic. Every per-point update evaluates the CSR row in the
+// reference's order: s = ((cN*xN + cS*xS) + cW*xW) + cE*xE; x += dt*s, with
+// FMA contraction disabled for this file, so FP64 results are bit-identical
+// to a no-FMA CPU evaluation of the reference's CSR spmv.
+//
+// Kernels.
+//  * k_velocity / k_stress: the reference's two half-steps, one thread per
+//    (point, CPT columns), updating in place.
+//  * k_fused: one launch per leapfrog step. A block owns a strip of P-1
+//    theta points (plus one east halo point) x SEG radial rows x one column
+//    block, marches up the rows keeping the previous row's new velocities,
+//    the old srp row below and the old stp row in registers, exchanges the
+//    theta neighbours through LDS, and writes new v and s to the other
+//    ping-pong buffer. Each state value is read once and written once per
+//    step (vs. 3 passes of x_s and x_v traffic for the split pair); the
+//    halos cost one row per segment and one point per strip row.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "shaw_app.hpp"
+#include "shaw_common.hpp"
+
+namespace shaw {
+namespace {
+
+template <class T, int N>
+struct alignas(sizeof(T) * N) Vec {
+  T v[N];
+};
+
+template <class T, int N>
+__device__ __forceinline__ void load(T (&dst)[N], const T* p) {
+  const Vec<T, N> x = *reinterpret_cast<const Vec<T, N>*>(p);
+#pragma unroll
+  for (int k = 0; k < N; ++k) dst[k] = x.v[k];
+}
+template <class T, int N>
+__device__ __forceinline__ void store(T* p, const T (&src)[N]) {
+  Vec<T, N> x;
+#pragma unroll
+  for (int k = 0; k < N; ++k) x.v[k] = src[k];
+  *reinterpret_cast<Vec<T, N>*>(p) = x;
+}
+
+template <class T>
+struct StepArgs {
+  const T* vIn;
+  T* vOut;
+  const T* sIn;
+  T* sOut;
+  const T* cv;          // [nVp][4]
+  const T* cs;          // [nSp][2]
+  const int64_t* src;   // [nb*MC] source gid per column (-1: none)
+  const T* srcRhoInv;   // [nb*MC]
+  const double* frow;   // forcing values of this step, [M]
+  const int32_t* recv;  // [nVp] receiver index or -1; nullptr: no sample this step
+  double* seis;         // LayoutLeft [R x Tcap x M]
+  uint64_t sample, R, Tcap;
+  uint32_t nr, nth, M, MC;
+  T dt;
+  int rank1;
+  uint32_t P, SEG, nStrips, nSegs, nb;
+};
+
+__device__ __forceinline__ uint64_t srpIdx(uint64_t i, uint64_t j, uint64_t rs) { return j * rs + 2 * i; }
+__device__ __forceinline__ uint64_t stpIdx(uint64_t i, uint64_t j, uint64_t rs, uint64_t nr) {
+  return j + 1 < nr ? j * rs + 2 * i + 1 : (nr - 1) * rs + i;
+}
+
+// forcing and receiver epilogue of the velocity update for CPT columns
+template <class T, int CPT>
+__device__ __forceinline__ void velocityEpilogue(const StepArgs<T>& a, T (&vn)[CPT], uint64_t p,
+                                                 uint32_t colBase) {
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const uint32_t col = colBase + k;
+    if (col < a.M && a.src[col] == (int64_t)p) {
+      const T f = (T)a.frow[col];
+      if (a.rank1) vn[k] = vn[k] + a.dt * a.srcRhoInv[col] * f;   // KokkosBlas::mult
+      else vn[k] += a.srcRhoInv[col] * f * a.dt;                  // AddForcingRank2
+    }
+  }
+}
+
+template <class T, int CPT>
+__device__ __forceinline__ void recordReceivers(const StepArgs<T>& a, const T (&vn)[CPT], uint64_t p,
+                                                uint32_t colBase) {
+  if (a.recv == nullptr) return;
+  const int32_t r = a.recv[p];
+  if (r < 0) return;
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const uint32_t col = colBase + k;
+    if (col < a.M) a.seis[(uint64_t)r + a.R * (a.sample + a.Tcap * col)] = (double)vn[k];
+  }
+}
+
+// ------------------------------------------------------------ split pair
+template <class T, int CPT>
+__global__ void __launch_bounds__(256) k_velocity(StepArgs<T> a) {
+  const uint32_t tpp = a.MC / CPT;
+  const uint64_t nVp = (uint64_t)a.nr * a.nth, nSp = (uint64_t)(a.nr - 1) * a.nth + (uint64_t)a.nr * (a.nth - 1);
+  const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t p = gt / tpp;
+  if (p >= nVp) return;
+  const uint32_t cb = blockIdx.y, m0 = (uint32_t)(gt % tpp) * CPT;
+  const uint64_t rs = 2ull * a.nth - 1, i = p % a.nth, j = p / a.nth;
+  const uint64_t iN = srpIdx(i, j < a.nr - 1 ? j : a.nr - 2, rs);
+  const uint64_t iS = srpIdx(i, j > 0 ? j - 1 : 0, rs);
+  const uint64_t iW = stpIdx(i > 0 ? i - 1 : 0, j, rs, a.nr);
+  const uint64_t iE = stpIdx(i < a.nth - 2 ? i : a.nth - 2, j, rs, a.nr);
+  const T* s = a.sIn + (uint64_t)cb * nSp * a.MC + m0;
+  T* v = a.vOut + ((uint64_t)cb * nVp + p) * a.MC + m0;
+  T xN[CPT], xS[CPT], xW[CPT], xE[CPT], vo[CPT], vn[CPT];
+  load(xN, s + iN * a.MC);
+  load(xS, s + iS * a.MC);
+  load(xW, s + iW * a.MC);
+  load(xE, s + iE * a.MC);
+  load(vo, v);
+  T c[4];
+  load(c, a.cv + 4 * p);
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    T sum = c[0] * xN[k] + c[1] * xS[k];
+    sum = sum + c[2] * xW[k];
+    sum = sum + c[3] * xE[k];
+    vn[k] = vo[k] + a.dt * sum;
+  }
+  velocityEpilogue<T, CPT>(a, vn, p, cb * a.MC + m0);
+  store(v, vn);
+  recordReceivers<T, CPT>(a, vn, p, cb * a.MC + m0);
+}
+
+template <class T, int CPT>
+__global__ void __launch_bounds__(256) k_stress(StepArgs<T> a) {
+  const uint32_t tpp = a.MC / CPT;
+  const uint64_t nVp = (uint64_t)a.nr * a.nth, nSp = (uint64_t)(a.nr - 1) * a.nth + (uint64_t)a.nr * (a.nth - 1);
+  const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t q = gt / tpp;
+  if (q >= nSp) return;
+  const uint32_t cb = blockIdx.y, m0 = (uint32_t)(gt % tpp) * CPT;
+  const uint64_t rs = 2ull * a.nth - 1, last = (uint64_t)(a.nr - 1) * rs;
+  uint64_t n0, n1;
+  if (q < last) {
+    const uint64_t j = q / rs, k = q % rs;
+    if ((k & 1) == 0) { n0 = (j + 1) * a.nth + k / 2; n1 = j * a.nth + k / 2; }   // srp: N, S
+    else { n0 = j * a.nth + (k - 1) / 2; n1 = n0 + 1; }                           // stp: W, E
+  } else {
+    n0 = (uint64_t)(a.nr - 1) * a.nth + (q - last);
+    n1 = n0 + 1;
+  }
+  const T* v = a.vIn + (uint64_t)cb * nVp * a.MC + m0;
+  T* s = a.sOut + ((uint64_t)cb * nSp + q) * a.MC + m0;
+  T x0[CPT], x1[CPT], so[CPT];
+  load(x0, v + n0 * a.MC);
+  load(x1, v + n1 * a.MC);
+  load(so, s);
+  T c[2];
+  load(c, a.cs + 2 * q);
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) so[k] = so[k] + a.dt * (c[0] * x0[k] + c[1] * x1[k]);
+  store(s, so);
+}
+
+// ------------------------------------------------------------ fused step
+template <class T, int CPT>
+__global__ void __launch_bounds__(512) k_fused(StepArgs<T> a) {
+  extern __shared__ unsigned char smem_raw[];
+  T* stpRow = reinterpret_cast<T*>(smem_raw);    // [P+1][MC]: old stp(i0-1 .. i0+P-1) of row r
+  T* vRow = stpRow + (a.P + 1) * a.MC;            // [P][MC]: new v of row jj
+
+  const unsigned lb = xcdLogicalBlock(blockIdx.x, gridDim.x);
+  const uint32_t strip = lb % a.nStrips;
+  const uint32_t seg = (lb / a.nStrips) % a.nSegs;
+  const uint32_t cb = lb / (a.nStrips * a.nSegs);
+
+  const uint32_t tpp = a.MC / CPT;
+  const uint32_t pl = threadIdx.x / tpp;
+  const uint32_t m0 = (threadIdx.x % tpp) * CPT;
+  const uint64_t nr = a.nr, nth = a.nth, rs = 2 * nth - 1;
+  const uint64_t nVp = nr * nth, nSp = (nr - 1) * nth + nr * (nth - 1);
+  const uint64_t i = (uint64_t)strip * (a.P - 1) + pl;
+  const bool active = i < nth;
+  const bool owner = active && pl + 1 < a.P;  // east halo point is recomputed, not owned
+  const uint64_t j0 = (uint64_t)seg * a.SEG;
+  const uint64_t j1 = j0 + a.SEG < nr ? j0 + a.SEG : nr;
+  const uint32_t colBase = cb * a.MC + m0;
+
+  const T* vIn = a.vIn + (uint64_t)cb * nVp * a.MC + m0;
+  T* vOut = a.vOut + (uint64_t)cb * nVp * a.MC + m0;
+  const T* sIn = a.sIn + (uint64_t)cb * nSp * a.MC + m0;
+  T* sOut = a.sOut + (uint64_t)cb * nSp * a.MC + m0;
+
+  T S[CPT], vj[CPT], Eold[CPT];
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) { vj[k] = T(0); Eold[k] = T(0); S[k] = T(0); }
+  if (active) load(S, sIn + srpIdx(i, j0 > 0 ? j0 - 1 : 0, rs) * a.MC);
+
+  for (int64_t jj = (int64_t)j0 - 1; jj < (int64_t)j1; ++jj) {
+    const uint64_t r = (uint64_t)(jj + 1);
+    const bool haveRow = r < nr;
+    T N[CPT], vo[CPT], E[CPT], vn[CPT];
+    T c[4];
+    if (active && haveRow) {
+      const uint64_t p = r * nth + i;
+      load(N, sIn + srpIdx(i, r < nr - 1 ? r : nr - 2, rs) * a.MC);
+      load(E, sIn + stpIdx(i < nth - 2 ? i : nth - 2, r, rs, nr) * a.MC);
+      load(vo, vIn + p * a.MC);
+      load(c, a.cv + 4 * p);
+      store(stpRow + (pl + 1) * a.MC + m0, E);
+      if (pl == 0) {
+        T Wh[CPT];
+        load(Wh, sIn + stpIdx(i > 0 ? i - 1 : 0, r, rs, nr) * a.MC);
+        store(stpRow + m0, Wh);
+      }
+    }
+    __syncthreads();
+    if (active && haveRow) {
+      const uint64_t p = r * nth + i;
+      T W[CPT];
+      load(W, stpRow + pl * a.MC + m0);
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        T sum = c[0] * N[k] + c[1] * S[k];
+        sum = sum + c[2] * W[k];
+        sum = sum + c[3] * E[k];
+        vn[k] = vo[k] + a.dt * sum;
+      }
+      velocityEpilogue<T, CPT>(a, vn, p, colBase);
+      if (owner && r < j1) {
+        store(vOut + p * a.MC, vn);
+        recordReceivers<T, CPT>(a, vn, p, colBase);
+      }
+    }
+    // stress rows jj (owned rows only): srp(i, jj) needs v_new rows jj and jj+1,
+    // stp(i, jj) needs v_new(i, jj) and v_new(i+1, jj) (from LDS)
+    if (owner && jj >= (int64_t)j0) {
+      const uint64_t jr = (uint64_t)jj;
+      if (jr + 1 < nr) {
+        const uint64_t q = srpIdx(i, jr, rs);
+        T b[2], so[CPT];
+        load(b, a.cs + 2 * q);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) so[k] = S[k] + a.dt * (b[0] * vn[k] + b[1] * vj[k]);
+        store(sOut + q * a.MC, so);
+      }
+      if (i + 1 < nth) {
+        const uint64_t q = stpIdx(i, jr, rs, nr);
+        T b[2], vE[CPT], so[CPT];
+        load(b, a.cs + 2 * q);
+        load(vE, vRow + (pl + 1) * a.MC + m0);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) so[k] = Eold[k] + a.dt * (b[0] * vj[k] + b[1] * vE[k]);
+        store(sOut + q * a.MC, so);
+      }
+    }
+    __syncthreads();
+    if (active && haveRow) {
+      store(vRow + pl * a.MC + m0, vn);
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) { vj[k] = vn[k]; S[k] = N[k]; Eold[k] = E[k]; }
+    }
+  }
+}
+
+// ------------------------------------------------------------ gathers
+template <class T>
+__global__ void k_snapshot(const T* __restrict__ x, uint64_t n, uint32_t MC, const uint32_t* __restrict__ cols,
+                           uint32_t ncols, double* __restrict__ out, uint64_t sample, uint64_t Tcap) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c = blockIdx.y;
+  if (g >= n || c >= ncols) return;
+  const uint32_t col = cols[c];
+  out[g + n * (sample + Tcap * c)] = (double)x[((uint64_t)(col / MC) * n + g) * MC + col % MC];
+}
+
+// ------------------------------------------------------------ engine
+struct FomBase {
+  virtual ~FomBase() = default;
+};
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  void alloc(size_t count) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    n = count;
+    if (count) SHAW_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+  }
+};
+
+}  // namespace
+}  // namespace shaw
+
+struct shaw_fom {
+  virtual ~shaw_fom() = default;
+  virtual void setSources(const uint64_t* gid) = 0;
+  virtual void setForcing(uint64_t nsteps, const double* tab) = 0;
+  virtual void setReceivers(uint64_t R, const uint64_t* gid, uint64_t every, uint64_t maxSamples) = 0;
+  virtual void setSnapshots(int dof, uint64_t every, uint64_t ncols, const uint64_t* cols, uint64_t maxSamples) = 0;
+  virtual void reset() = 0;
+  virtual void advance(uint64_t nsteps) = 0;
+  virtual void sync() = 0;
+  virtual uint64_t step() const = 0;
+  virtual uint64_t getSeismogram(double* out) = 0;
+  virtual uint64_t getSnapshots(int dof, double* out) = 0;
+  virtual void getState(double* vp, double* sp) = 0;
+  virtual void setState(const double* vp, const double* sp) = 0;
+  virtual void time(uint64_t nsteps, double* total, double* perStep) = 0;
+  virtual void bytes(double* kernel, double* model) const = 0;
+  virtual void config(int* kernel, int* cpt, int* strip, int* seg) const = 0;
+};
+
+namespace shaw {
+namespace {
+
+template <class T>
+class FomEngine final : public shaw_fom {
+ public:
+  FomEngine(const shaw_app& app, uint64_t M, double dt, const shaw_fom_options& o) {
+    const Grid& g = app.ops.grid;
+    nr_ = g.nr;
+    nth_ = g.nth;
+    nVp_ = g.nVp();
+    nSp_ = g.nSp();
+    M_ = M;
+    dt_ = dt;
+    rank1_ = o.rank1_forcing;
+    SHAW_REQUIRE(M >= 1 && M <= (1u << 20), SHAW_ERR_ARG, "forcing column count out of range");
+    SHAW_REQUIRE(nr_ >= 2 && nth_ >= 3 && nVp_ < (1ull << 40), SHAW_ERR_ARG, "bad grid");
+    // column block: MC*sizeof(T) <= 256 B, power of two >= M when M is small
+    const uint32_t cap = 256 / sizeof(T);
+    MC_ = 1;
+    while (MC_ < M && MC_ < cap) MC_ *= 2;
+    nb_ = (uint32_t)((M + MC_ - 1) / MC_);
+    kernel_ = o.kernel == SHAW_KERNEL_SPLIT ? SHAW_KERNEL_SPLIT : SHAW_KERNEL_FUSED;
+    cpt_ = o.cols_per_thread > 0 ? o.cols_per_thread : (int)std::min<uint32_t>(MC_, sizeof(T) == 8 ? 4 : 8);
+    SHAW_REQUIRE(cpt_ == 1 || cpt_ == 2 || cpt_ == 4 || cpt_ == 8, SHAW_ERR_ARG, "cols_per_thread must be 1, 2, 4 or 8");
+    SHAW_REQUIRE(MC_ % cpt_ == 0, SHAW_ERR_ARG, "cols_per_thread must divide the column block width");
+    const uint32_t tpp = MC_ / cpt_;
+    if (kernel_ == SHAW_KERNEL_FUSED) {
+      P_ = o.strip > 0 ? (uint32_t)o.strip : std::max<uint32_t>(2, std::min<uint32_t>(64, 512 / tpp));
+      SHAW_REQUIRE(P_ >= 2 && P_ * tpp <= 512, SHAW_ERR_ARG, "strip too wide for a 512-thread block");
+      SEG_ = o.segment > 0 ? (uint32_t)o.segment : 64;
+      nStrips_ = (uint32_t)((nth_ + (P_ - 1) - 1) / (P_ - 1));
+      nSegs_ = (uint32_t)((nr_ + SEG_ - 1) / SEG_);
+    }
+    SHAW_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    SHAW_HIP(hipEventCreate(&ev0_));
+    SHAW_HIP(hipEventCreate(&ev1_));
+    const size_t nv = (size_t)nb_ * nVp_ * MC_, ns = (size_t)nb_ * nSp_ * MC_;
+    const int nbuf = kernel_ == SHAW_KERNEL_FUSED ? 2 : 1;
+    for (int b = 0; b < nbuf; ++b) { v_[b].alloc(nv); s_[b].alloc(ns); }
+    // coefficients in the state precision
+    std::vector<T> cv(app.ops.coefVp.begin(), app.ops.coefVp.end());
+    std::vector<T> cs(app.ops.coefSp.begin(), app.ops.coefSp.end());
+    cv_.alloc(cv.size());
+    cs_.alloc(cs.size());
+    SHAW_HIP(hipMemcpy(cv_.p, cv.data(), cv.size() * sizeof(T), hipMemcpyHostToDevice));
+    SHAW_HIP(hipMemcpy(cs_.p, cs.data(), cs.size() * sizeof(T), hipMemcpyHostToDevice));
+    rhoInv_ = app.ops.rhoInv;
+    src_.alloc((size_t)nb_ * MC_);
+    srcRho_.alloc((size_t)nb_ * MC_);
+    std::vector<int64_t> none((size_t)nb_ * MC_, -1);
+    SHAW_HIP(hipMemcpy(src_.p, none.data(), none.size() * 8, hipMemcpyHostToDevice));
+    SHAW_HIP(hipMemset(srcRho_.p, 0, srcRho_.n * sizeof(T)));
+    reset();
+  }
+  ~FomEngine() override {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  void setSources(const uint64_t* gid) override {
+    std::vector<int64_t> s((size_t)nb_ * MC_, -1);
+    std::vector<T> r((size_t)nb_ * MC_, T(0));
+    for (uint64_t m = 0; m < M_; ++m) {
+      SHAW_REQUIRE(gid[m] < nVp_, SHAW_ERR_ARG, "source gid out of range");
+      s[m] = (int64_t)gid[m];
+      r[m] = (T)rhoInv_[gid[m]];
+    }
+    SHAW_HIP(hipMemcpyAsync(src_.p, s.data(), s.size() * 8, hipMemcpyHostToDevice, stream_));
+    SHAW_HIP(hipMemcpyAsync(srcRho_.p, r.data(), r.size() * sizeof(T), hipMemcpyHostToDevice, stream_));
+    SHAW_HIP(hipStreamSynchronize(stream_));
+    haveSources_ = true;
+  }
+  void setForcing(uint64_t nsteps, const double* tab) override {
+    ftab_.alloc(nsteps * M_);
+    SHAW_HIP(hipMemcpy(ftab_.p, tab, nsteps * M_ * sizeof(double), hipMemcpyHostToDevice));
+    ftabSteps_ = nsteps;
+  }
+  void setReceivers(uint64_t R, const uint64_t* gid, uint64_t every, uint64_t maxSamples) override {
+    SHAW_REQUIRE(R == 0 || every > 0, SHAW_ERR_ARG, "receiver sampling period must be > 0");
+    std::vector<int32_t> idx(nVp_, -1);
+    dupRecv_.clear();
+    for (uint64_t r = 0; r < R; ++r) {
+      SHAW_REQUIRE(gid[r] < nVp_, SHAW_ERR_ARG, "receiver gid out of range");
+      // several receivers on one grid point: keep each, record via a duplicate list
+      if (idx[gid[r]] >= 0) dupRecv_.push_back({(uint32_t)r, (uint32_t)idx[gid[r]]});
+      else idx[gid[r]] = (int32_t)r;
+    }
+    recv_.alloc(R ? nVp_ : 0);
+    if (R) SHAW_HIP(hipMemcpy(recv_.p, idx.data(), nVp_ * 4, hipMemcpyHostToDevice));
+    R_ = R;
+    every_ = every;
+    Tcap_ = maxSamples;
+    seis_.alloc(R * maxSamples * M_);
+    if (seis_.n) SHAW_HIP(hipMemset(seis_.p, 0, seis_.n * sizeof(double)));
+  }
+  void setSnapshots(int dof, uint64_t every, uint64_t ncols, const uint64_t* cols, uint64_t maxSamples) override {
+    SHAW_REQUIRE(dof == 0 || dof == 1, SHAW_ERR_ARG, "dof must be 0 (vp) or 1 (sp)");
+    SHAW_REQUIRE(ncols == 0 || every > 0, SHAW_ERR_ARG, "snapshot period must be > 0");
+    Snap& s = snap_[dof];
+    std::vector<uint32_t> c(ncols);
+    for (uint64_t k = 0; k < ncols; ++k) {
+      SHAW_REQUIRE(cols[k] < M_, SHAW_ERR_ARG, "snapshot column out of range");
+      c[k] = (uint32_t)cols[k];
+    }
+    s.cols.alloc(ncols);
+    if (ncols) SHAW_HIP(hipMemcpy(s.cols.p, c.data(), ncols * 4, hipMemcpyHostToDevice));
+    s.ncols = ncols;
+    s.every = every;
+    s.Tcap = maxSamples;
+    const uint64_t n = dof == 0 ? nVp_ : nSp_;
+    s.buf.alloc(n * maxSamples * ncols);
+  }
+  void reset() override {
+    for (int b = 0; b < 2; ++b) {
+      if (v_[b].p) SHAW_HIP(hipMemsetAsync(v_[b].p, 0, v_[b].n * sizeof(T), stream_));
+      if (s_[b].p) SHAW_HIP(hipMemsetAsync(s_[b].p, 0, s_[b].n * sizeof(T), stream_));
+    }
+    SHAW_HIP(hipStreamSynchronize(stream_));
+    step_ = 0;
+    cur_ = 0;
+  }
+
+  void advance(uint64_t nsteps) override {
+    SHAW_REQUIRE(haveSources_, SHAW_ERR_STATE, "shaw_fom_set_sources must be called before advancing");
+    SHAW_REQUIRE(ftab_.p && step_ + nsteps <= ftabSteps_, SHAW_ERR_STATE,
+                 "forcing table shorter than the requested steps");
+    const uint64_t last = step_ + nsteps;
+    if (R_) SHAW_REQUIRE(last / every_ <= Tcap_, SHAW_ERR_STATE, "seismogram buffer too small for the requested steps");
+    for (int d = 0; d < 2; ++d)
+      if (snap_[d].ncols) SHAW_REQUIRE(last / snap_[d].every <= snap_[d].Tcap, SHAW_ERR_STATE, "snapshot buffer too small");
+    for (uint64_t k = 0; k < nsteps; ++k) launchStep();
+    SHAW_HIP(hipGetLastError());
+  }
+  void sync() override { SHAW_HIP(hipStreamSynchronize(stream_)); }
+  uint64_t step() const override { return step_; }
+
+  uint64_t getSeismogram(double* out) override {
+    sync();
+    const uint64_t T_ = R_ ? step_ / every_ : 0;
+    // device [R x Tcap x M] -> host [R x T x M]
+    for (uint64_t m = 0; m < M_ && T_; ++m)
+      SHAW_HIP(hipMemcpy(out + R_ * T_ * m, seis_.p + R_ * Tcap_ * m, R_ * T_ * sizeof(double), hipMemcpyDeviceToHost));
+    // receivers sharing a grid point with an earlier receiver
+    for (const auto& d : dupRecv_)
+      for (uint64_t m = 0; m < M_; ++m)
+        for (uint64_t t = 0; t < T_; ++t) out[d.first + R_ * (t + T_ * m)] = out[d.second + R_ * (t + T_ * m)];
+    return T_;
+  }
+  uint64_t getSnapshots(int dof, double* out) override {
+    sync();
+    const Snap& s = snap_[dof];
+    const uint64_t T_ = s.ncols ? step_ / s.every : 0, n = dof == 0 ? nVp_ : nSp_;
+    for (uint64_t c = 0; c < s.ncols && T_; ++c)
+      SHAW_HIP(hipMemcpy(out + n * T_ * c, s.buf.p + n * s.Tcap * c, n * T_ * sizeof(double), hipMemcpyDeviceToHost));
+    return T_;
+  }
+  void getState(double* vp, double* sp) override {
+    sync();
+    copyState(vp, v_[cur_].p, nVp_, true);
+    copyState(sp, s_[cur_].p, nSp_, true);
+  }
+  void setState(const double* vp, const double* sp) override {
+    sync();
+    copyState(const_cast<double*>(vp), v_[cur_].p, nVp_, false);
+    copyState(const_cast<double*>(sp), s_[cur_].p, nSp_, false);
+  }
+  void time(uint64_t nsteps, double* total, double* perStep) override {
+    SHAW_HIP(hipEventRecord(ev0_, stream_));
+    advance(nsteps);
+    SHAW_HIP(hipEventRecord(ev1_, stream_));
+    SHAW_HIP(hipEventSynchronize(ev1_));
+    float ms = 0;
+    SHAW_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    *total = ms;
+    *perStep = nsteps ? ms / (double)nsteps : 0.0;
+  }
+  void bytes(double* kernel, double* model) const override {
+    const double s = sizeof(T), cols = (double)nb_ * MC_;
+    const double coef = (4.0 * nVp_ + 2.0 * nSp_) * s * nb_;
+    if (kernel) {
+      // fused: read + write each state once; split: v (r+w) + s (r) / v (r) + s (r+w)
+      const double passes = kernel_ == SHAW_KERNEL_FUSED ? 2.0 : 3.0;
+      *kernel = s * cols * passes * (double)(nVp_ + nSp_) + coef;
+    }
+    if (model) *model = s * (double)M_ * 3.0 * (double)(nVp_ + nSp_) + 8.0 * (4.0 * nVp_ + 2.0 * nSp_);
+  }
+  void config(int* kernel, int* cpt, int* strip, int* seg) const override {
+    if (kernel) *kernel = kernel_;
+    if (cpt) *cpt = cpt_;
+    if (strip) *strip = (int)P_;
+    if (seg) *seg = (int)SEG_;
+  }
+
+ private:
+  struct Snap {
+    DevBuf<uint32_t> cols;
+    DevBuf<double> buf;
+    uint64_t ncols = 0, every = 0, Tcap = 0;
+  };
+
+  StepArgs<T> args(uint64_t stepNo) {
+    StepArgs<T> a{};
+    a.cv = cv_.p;
+    a.cs = cs_.p;
+    a.src = src_.p;
+    a.srcRhoInv = srcRho_.p;
+    a.frow = ftab_.p + (stepNo - 1) * M_;
+    const bool rec = R_ && stepNo % every_ == 0;
+    a.recv = rec ? recv_.p : nullptr;
+    a.seis = seis_.p;
+    a.sample = rec ? stepNo / every_ - 1 : 0;
+    a.R = R_;
+    a.Tcap = Tcap_;
+    a.nr = (uint32_t)nr_;
+    a.nth = (uint32_t)nth_;
+    a.M = (uint32_t)M_;
+    a.MC = MC_;
+    a.dt = (T)dt_;
+    a.rank1 = rank1_;
+    a.P = P_;
+    a.SEG = SEG_;
+    a.nStrips = nStrips_;
+    a.nSegs = nSegs_;
+    a.nb = nb_;
+    return a;
+  }
+
+  template <int CPT>
+  void launchSplit(StepArgs<T>& a) {
+    const uint64_t tpp = MC_ / CPT;
+    dim3 gv((unsigned)((nVp_ * tpp + 255) / 256), nb_), gs((unsigned)((nSp_ * tpp + 255) / 256), nb_);
+    a.vIn = a.vOut = v_[0].p;
+    a.sIn = a.sOut = s_[0].p;
+    hipLaunchKernelGGL((k_velocity<T, CPT>), gv, dim3(256), 0, stream_, a);
+    snapshot(0, a.vOut);
+    hipLaunchKernelGGL((k_stress<T, CPT>), gs, dim3(256), 0, stream_, a);
+    snapshot(1, a.sOut);
+  }
+  template <int CPT>
+  void launchFused(StepArgs<T>& a) {
+    const uint32_t tpp = MC_ / CPT;
+    a.vIn = v_[cur_].p;
+    a.sIn = s_[cur_].p;
+    a.vOut = v_[cur_ ^ 1].p;
+    a.sOut = s_[cur_ ^ 1].p;
+    const unsigned blocks = nStrips_ * nSegs_ * nb_;
+    const size_t lds = (size_t)(2 * P_ + 1) * MC_ * sizeof(T);
+    hipLaunchKernelGGL((k_fused<T, CPT>), dim3(blocks), dim3(P_ * tpp), lds, stream_, a);
+    cur_ ^= 1;
+    snapshot(0, v_[cur_].p);
+    snapshot(1, s_[cur_].p);
+  }
+  void launchStep() {
+    const uint64_t stepNo = step_ + 1;
+    stepForSnap_ = stepNo;
+    StepArgs<T> a = args(stepNo);
+    const bool fused = kernel_ == SHAW_KERNEL_FUSED;
+    switch (cpt_) {
+      case 1: fused ? launchFused<1>(a) : launchSplit<1>(a); break;
+      case 2: fused ? launchFused<2>(a) : launchSplit<2>(a); break;
+      case 4: fused ? launchFused<4>(a) : launchSplit<4>(a); break;
+      default: fused ? launchFused<8>(a) : launchSplit<8>(a); break;
+    }
+    step_ = stepNo;
+  }
+  void snapshot(int dof, const T* x) {
+    Snap& s = snap_[dof];
+    if (!s.ncols || stepForSnap_ % s.every != 0) return;
+    const uint64_t n = dof == 0 ? nVp_ : nSp_;
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)s.ncols);
+    hipLaunchKernelGGL(k_snapshot<T>, grid, dim3(256), 0, stream_, x, n, MC_, s.cols.p, (uint32_t)s.ncols, s.buf.p,
+                       stepForSnap_ / s.every - 1, s.Tcap);
+  }
+  // host [n][M] double <-> device [nb][n][MC] T
+  void copyState(double* host, T* dev, uint64_t n, bool toHost) {
+    if (!host) return;
+    std::vector<T> tmp((size_t)nb_ * n * MC_, T(0));
+    if (toHost) {
+      SHAW_HIP(hipMemcpy(tmp.data(), dev, tmp.size() * sizeof(T), hipMemcpyDeviceToHost));
+      for (uint64_t g = 0; g < n; ++g)
+        for (uint64_t m = 0; m < M_; ++m) host[g * M_ + m] = (double)tmp[((m / MC_) * n + g) * MC_ + m % MC_];
+    } else {
+      for (uint64_t g = 0; g < n; ++g)
+        for (uint64_t m = 0; m < M_; ++m) tmp[((m / MC_) * n + g) * MC_ + m % MC_] = (T)host[g * M_ + m];
+      SHAW_HIP(hipMemcpy(dev, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+  }
+
+  uint64_t nr_ = 0, nth_ = 0, nVp_ = 0, nSp_ = 0, M_ = 0;
+  uint32_t MC_ = 1, nb_ = 1, P_ = 0, SEG_ = 0, nStrips_ = 0, nSegs_ = 0;
+  int kernel_ = SHAW_KERNEL_FUSED, cpt_ = 1, rank1_ = 0;
+  double dt_ = 0;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  DevBuf<T> v_[2], s_[2], cv_, cs_, srcRho_;
+  DevBuf<int64_t> src_;
+  DevBuf<double> ftab_, seis_;
+  DevBuf<int32_t> recv_;
+  std::vector<double> rhoInv_;
+  std::vector<std::pair<uint32_t, uint32_t>> dupRecv_;
+  uint64_t ftabSteps_ = 0, R_ = 0, every_ = 1, Tcap_ = 0, step_ = 0, stepForSnap_ = 0;
+  int cur_ = 0;
+  bool haveSources_ = false;
+  Snap snap_[2];
+};
+
+}  // namespace
+}  // namespace shaw
+
+// ------------------------------------------------------------------ C ABI
+using shaw::guarded;
+
+extern "C" {
+
+int shaw_fom_create(shaw_app_t app, uint64_t m, double dt, const shaw_fom_options* opts, shaw_fom_t* out) {
+  return guarded([&] {
+    SHAW_REQUIRE(app && out, SHAW_ERR_ARG, "null argument");
+    SHAW_REQUIRE(dt > 0, SHAW_ERR_ARG, "dt must be > 0");
+    int ndev = 0;
+    SHAW_HIP(hipGetDeviceCount(&ndev));
+    SHAW_REQUIRE(ndev > 0, SHAW_ERR_HIP, "no HIP device available");
+    shaw_fom_options o{};
+    if (opts) o = *opts;
+    SHAW_REQUIRE(o.precision == SHAW_FP64 || o.precision == SHAW_FP32, SHAW_ERR_ARG, "unknown precision");
+    if (o.precision == SHAW_FP64) *out = new shaw::FomEngine<double>(*app, m, dt, o);
+    else *out = new shaw::FomEngine<float>(*app, m, dt, o);
+  });
+}
+int shaw_fom_set_sources(shaw_fom_t h, const uint64_t* gid) {
+  return guarded([&] { SHAW_REQUIRE(h && gid, SHAW_ERR_ARG, "null argument"); h->setSources(gid); });
+}
+int shaw_fom_set_forcing(shaw_fom_t h, uint64_t nsteps, const double* tab) {
+  return guarded([&] { SHAW_REQUIRE(h && tab, SHAW_ERR_ARG, "null argument"); h->setForcing(nsteps, tab); });
+}
+int shaw_fom_set_receivers(shaw_fom_t h, uint64_t r, const uint64_t* gid, uint64_t every, uint64_t maxSamples) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && (r == 0 || gid), SHAW_ERR_ARG, "null argument");
+    h->setReceivers(r, gid, every, maxSamples);
+  });
+}
+int shaw_fom_set_snapshots(shaw_fom_t h, int dof, uint64_t every, uint64_t ncols, const uint64_t* cols,
+                           uint64_t maxSamples) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && (ncols == 0 || cols), SHAW_ERR_ARG, "null argument");
+    h->setSnapshots(dof, every, ncols, cols, maxSamples);
+  });
+}
+int shaw_fom_reset(shaw_fom_t h) {
+  return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->reset(); });
+}
+int shaw_fom_advance(shaw_fom_t h, uint64_t nsteps) {
+  return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->advance(nsteps); });
+}
+int shaw_fom_sync(shaw_fom_t h) {
+  return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->sync(); });
+}
+int shaw_fom_step(shaw_fom_t h, uint64_t* step) {
+  return guarded([&] { SHAW_REQUIRE(h && step, SHAW_ERR_ARG, "null argument"); *step = h->step(); });
+}
+int shaw_fom_get_seismogram(shaw_fom_t h, double* out, uint64_t* samples) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && out, SHAW_ERR_ARG, "null argument");
+    const uint64_t t = h->getSeismogram(out);
+    if (samples) *samples = t;
+  });
+}
+int shaw_fom_get_snapshots(shaw_fom_t h, int dof, double* out, uint64_t* samples) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && out && (dof == 0 || dof == 1), SHAW_ERR_ARG, "bad argument");
+    const uint64_t t = h->getSnapshots(dof, out);
+    if (samples) *samples = t;
+  });
+}
+int shaw_fom_get_state(shaw_fom_t h, double* vp, double* sp) {
+  return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->getState(vp, sp); });
+}
+int shaw_fom_set_state(shaw_fom_t h, const double* vp, const double* sp) {
+  return guarded([&] { SHAW_REQUIRE(h && vp && sp, SHAW_ERR_ARG, "null argument"); h->setState(vp, sp); });
+}
+int shaw_fom_time(shaw_fom_t h, uint64_t nsteps, double* total, double* perStep) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && total && perStep, SHAW_ERR_ARG, "null argument");
+    h->time(nsteps, total, perStep);
+  });
+}
+int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel, double* model) {
+  return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->bytes(kernel, model); });
+}
+int shaw_fom_config(shaw_fom_t h, int* kernel, int* cpt, int* strip, int* seg) {
+  return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->config(kernel, cpt, strip, seg); });
+}
+void shaw_fom_destroy(shaw_fom_t h) {
+  if (h) guarded([&] { delete h; });
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+"""GPU parity of the FOM path (HIP kernels through the C ABI) against the oracle.
+
+Bar: FP64 results bit-identical to the oracle (same CSR row order, no FMA on
+either side) for every reference regression input and for random rank-2
+batches, with both kernel families (split half-steps, fused step) and a
+spread of tile shapes; within 1e-12 relative of the reference golds; FP32
+state variant within a stated tolerance; size-independent properties at the
+full config-2 shape.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import shaw_amd
+from tests.helpers import MESHES, fom_golds, load_input, realizations, rel_l2, rel_max
+from tests.test_oracle import FOM_TESTS, gold_view
+
+pytestmark = pytest.mark.gpu
+
+LAYERED = {"kind": "layered", "radius": [3480.0, 4100.0, 4700.0, 5200.0, 5600.0, 5900.0, 6150.0, 6371.0],
+           "density": [5600.0, 5300.0, 4950.0, 4600.0, 4200.0, 3900.0, 3600.0, 3300.0],
+           "velocity": [7300.0, 7000.0, 6600.0, 6200.0, 5700.0, 5200.0, 4800.0, 4400.0]}
+
+KERNELS = [dict(kernel=shaw_amd.KERNEL_SPLIT), dict(kernel=shaw_amd.KERNEL_FUSED),
+           dict(kernel=shaw_amd.KERNEL_FUSED, strip=8, segment=3),
+           dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=1, strip=5, segment=1)]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_both(test, **kw):
+    cfg = load_input(test)
+    g, io, sig = cfg["general"], cfg.get("io", {}), cfg["source"]["signal"]
+    mesh = os.path.join(MESHES, g["meshDir"])
+    app = shaw_amd.App.from_mesh(mesh, cfg["material"])
+    ora = oracle.App(mesh, cfg["material"])
+    nsteps = int(g["finalTime"] / g["dt"])
+    seis, snap = io.get("seismogram", {}), io.get("snapshotMatrix", {})
+    reals = realizations(sig)
+    fs = sig.get("forcingSize")
+    sets = [reals[i:i + fs] for i in range(0, len(reals), fs)] if fs else [[r] for r in reals]
+    args = dict(receivers=seis.get("receivers", []), seis_freq=seis.get("freq", 0),
+                snap_freq_vp=snap.get("velocity", {}).get("freq", 0),
+                snap_freq_sp=snap.get("stress", {}).get("freq", 0), signal=sig["kind"])
+    for rid, s in enumerate(sets):
+        a = shaw_amd.fom_run(app, g["dt"], nsteps, s, fs is not None, **args, **kw)
+        b = ora.fom_run(g["dt"], nsteps, s, fs is not None, **args)
+        yield rid, len(s), fs is not None, a, b
+
+
+@pytest.mark.parametrize("kw", KERNELS, ids=["split", "fused", "fused-narrow", "fused-cpt1"])
+@pytest.mark.parametrize("test", FOM_TESTS)
+def test_regression_inputs_bit_exact(test, kw):
+    say(f"[gpu] {test} {kw}")
+    golds = fom_golds()
+    for rid, M, rank2, a, b in run_both(test, **kw):
+        for key in b:
+            np.testing.assert_array_equal(a[key], b[key], err_msg=f"{test} run {rid} {key}")
+        for key in ("snaps_vp", "snaps_sp", "seismogram"):
+            name = f"{test}/{key}_{rid}"
+            if name in golds.files:
+                ref = gold_view(key, golds[name], rank2, M)
+                for m in range(M):
+                    assert rel_l2(a[key][m], ref[m]) <= 1e-12 and rel_max(a[key][m], ref[m]) <= 1e-12, name
+
+
+def random_sources(rng, M, depth_max=2800.0):
+    return [(float(rng.uniform(10, depth_max)), float(rng.uniform(0, 180)), float(rng.uniform(20, 70)),
+             float(rng.uniform(5, 60))) for _ in range(M)]
+
+
+@pytest.fixture(scope="module")
+def mesh64x256(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("mesh") / "mesh64x256")
+    shaw_amd.mesh_write(d, 64, 256)
+    return d
+
+
+@pytest.mark.parametrize("M,kw", [(32, dict(kernel=shaw_amd.KERNEL_SPLIT)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=8, strip=32, segment=7)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=2, strip=16, segment=64)),
+                                  (7, dict(kernel=shaw_amd.KERNEL_FUSED)),
+                                  (40, dict(kernel=shaw_amd.KERNEL_FUSED)),
+                                  (40, dict(kernel=shaw_amd.KERNEL_SPLIT, cols_per_thread=2))])
+def test_random_rank2_bit_exact(mesh64x256, M, kw):
+    say(f"[gpu] random rank-2 M={M} {kw}")
+    rng = np.random.default_rng(M)
+    app = shaw_amd.App.from_mesh(mesh64x256, LAYERED)
+    ora = oracle.App(mesh64x256, LAYERED)
+    src = random_sources(rng, M)
+    rec = list(np.arange(0, 180, 7.5))
+    dt, n = 0.5, 120
+    a = shaw_amd.fom_run(app, dt, n, src, True, rec, 3, 40, 30, **kw)
+    b = ora.fom_run(dt, n, src, True, rec, 3, 40, 30)
+    for key in b:
+        np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+    assert np.abs(a["seismogram"]).max() > 0
+
+
+def test_config1_shape_rank1_bit_exact(tmp_path):
+    """The config-1 workload shape (200 x 1000, layered mantle, Ricker 640 km /
+    45 s at theta=0, dt=0.25, 180 surface receivers every step) for a
+    600-step prefix: GPU == oracle bit for bit."""
+    say("[gpu] config-1 shape")
+    d = str(tmp_path / "mesh200x1000")
+    shaw_amd.mesh_write(d, 200, 1000)
+    app = shaw_amd.App.from_mesh(d, LAYERED)
+    ora = oracle.App(d, LAYERED)
+    src = [(640.0, 0.0, 45.0, 67.5)]
+    rec = list(range(180))
+    a = shaw_amd.fom_run(app, 0.25, 600, src, False, rec, 1)
+    b = ora.fom_run(0.25, 600, src, False, rec, 1)
+    for key in b:
+        np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+
+
+def test_fp32_state_error_bound(mesh64x256):
+    """FP32 state/arithmetic variant: its seismogram error against FP64 is
+    reported; the stencil differences neighbouring values, so single precision
+    loses about a digit per derivative and ~1e-3 relative L2 after 300 steps
+    is expected. Bound: 2e-2."""
+    say("[gpu] fp32 variant")
+    rng = np.random.default_rng(5)
+    app = shaw_amd.App.from_mesh(mesh64x256, LAYERED)
+    src = random_sources(rng, 16)
+    rec = list(np.arange(0, 180, 5.0))
+    a = shaw_amd.fom_run(app, 0.5, 300, src, True, rec, 2, precision=shaw_amd.FP32)
+    b = shaw_amd.fom_run(app, 0.5, 300, src, True, rec, 2)
+    err = max(rel_l2(a["seismogram"][m], b["seismogram"][m]) for m in range(16))
+    say(f"[gpu] fp32 seismogram max rel L2 error = {err:.3e}")
+    assert err < 2e-2
+
+
+def test_full_config2_shape_properties():
+    """1024 x 4096, M = 32 (config 2's full single-GPU shape), 10 steps:
+    fused == split bit for bit, and doubling the forcing doubles the state
+    exactly (linearity; scaling by 2 is exact in binary floating point)."""
+    say("[gpu] config-2 shape properties")
+    app = shaw_amd.App.structured(1024, 4096, LAYERED)
+    rng = np.random.default_rng(2)
+    M, n, dt = 32, 10, 0.05
+    src = [(float(rng.uniform(1, 700)), 0.0, float(rng.uniform(20, 70)), 0.0) for _ in range(M)]
+    gids = [app.map_source(d, a) for d, a, p, dl in src]
+    tab = shaw_amd.forcing_table(["ricker"] * M, [s[3] for s in src], [s[2] for s in src], dt, n)
+    tab[:] = 1.0 + np.arange(M)  # large constant forcing so the wave reaches many points in 10 steps
+    out = {}
+    for name, kern, scale in [("fused", shaw_amd.KERNEL_FUSED, 1.0), ("split", shaw_amd.KERNEL_SPLIT, 1.0),
+                              ("fused2", shaw_amd.KERNEL_FUSED, 2.0)]:
+        f = shaw_amd.Fom(app, M, dt, kernel=kern, rank1_forcing=False)
+        f.set_sources(gids)
+        f.set_forcing(tab * scale)
+        f.set_receivers([app.map_receiver(a) for a in range(180)], 1, n)
+        f.advance(n)
+        out[name] = f.state() + (f.seismogram(),)
+        f.close()
+    for a, b in zip(out["fused"], out["split"]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(out["fused"], out["fused2"]):
+        np.testing.assert_array_equal(2.0 * a, b)
+    vp = out["fused"][0]
+    assert np.count_nonzero(vp) > 32 * 50
+
+
+def test_errors_from_gpu_entry_points(mesh64x256):
+    app = shaw_amd.App.from_mesh(mesh64x256, LAYERED)
+    f = shaw_amd.Fom(app, 4, 0.5)
+    with pytest.raises(shaw_amd.ShawError, match="set_sources"):
+        f.advance(1)
+    f.set_sources([app.map_source(100.0, 10.0)] * 4)
+    with pytest.raises(shaw_amd.ShawError, match="forcing table"):
+        f.advance(1)
+    f.set_forcing(np.zeros((5, 4)))
+    f.set_receivers([3, 4], 2, 1)
+    with pytest.raises(shaw_amd.ShawError, match="seismogram buffer"):
+        f.advance(5)
+    with pytest.raises(shaw_amd.ShawError):
+        shaw_amd.Fom(app, 4, 0.5, cols_per_thread=3)

@@ -201,3 +201,19 @@ def test_rom_full_basis_equals_fom(app21):
     assert rel_max(rom["qv"], fom["final_vp"]) <= 1e-12
     assert rel_max(rom["qs"], fom["final_sp"]) <= 1e-12
     assert rel_max(rom["seismogram"], fom["seismogram"]) <= 1e-12
+
+
+@pytest.mark.parametrize("mesh,shape", [("fullMesh4x5", (4, 5, 0.0, 180.0, 3480.0, 6371.0)),
+                                        ("fullMesh21x51", (21, 51, 0.0, 180.0, 3480.0, 6371.0)),
+                                        ("nonPhysicalFullMesh5x5", (5, 5, 0.0, 180.0, 1.0, 2.0))])
+def test_structured_oracle_equals_files(mesh, shape):
+    """The oracle's in-memory restatement of the mesh generator reproduces the
+    reference's fixture mesh files exactly (graphs, coordinates, CSR values)."""
+    a = oracle.App(os.path.join(MESHES, mesh), UNIT_MAT)
+    b = oracle.App("", UNIT_MAT, structured=shape)
+    for dof in ("vp", "sp"):
+        for x, y in zip(a.graph(dof), b.graph(dof)):
+            np.testing.assert_array_equal(x, y)
+        for x, y in zip(a.csr(dof), b.csr(dof)):
+            np.testing.assert_array_equal(x, y)
+    assert (a.dth, a.drr) == (b.dth, b.drr)
