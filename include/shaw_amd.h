@@ -209,9 +209,30 @@ int shaw_rom_sync(shaw_rom_t h);
  * (any output may be NULL), then free the slot.                            */
 int shaw_rom_retire(shaw_rom_t h, uint64_t slot, double* seis, double* qv, double* qs,
                     uint64_t* samples);
+/* Current reduced states of an admitted column (kv and ks values) and its
+ * step count; any output may be NULL.                                      */
+int shaw_rom_state(shaw_rom_t h, uint64_t slot, double* qv, double* qs, uint64_t* step);
 int shaw_rom_free_slots(shaw_rom_t h, uint64_t* n);
 int shaw_rom_time(shaw_rom_t h, uint64_t nsteps, double* total_ms);
 void shaw_rom_destroy(shaw_rom_t h);
+
+/* POD basis: the k leading left singular vectors of the snapshot matrix
+ * S LayoutLeft [n x ns] (the lsv_<dof> file content) and ns singular values,
+ * descending, in FP64 on the GPU by a deflated method of snapshots: each pass
+ * eigen-decomposes the Gram matrix of the residual snapshots (Jacobi), keeps
+ * the modes it resolves (sigma > 1e-6 of the residual's largest), orthogonal-
+ * izes them (Cholesky) and projects them out of the residual. Fails if k
+ * exceeds the numerical rank. Replaces computeThinSVD
+ * (src/tools/main_eigen_svd.cc:113-197). phi: LayoutLeft [n x k].          */
+int shaw_pod_basis(const double* s, uint64_t n, uint64_t ns, uint64_t k, double* phi, double* sigma);
+
+/* ---------------------------------------------------------- diagnostics --
+ * One MFMA instruction on one wavefront with lane-wise operands, for
+ * checking the operand/result lane layouts the ROM kernels assume.
+ * kind 0: v_mfma_f64_16x16x4f64, a[64], b[64] -> d[64][4];
+ * kind 1: v_mfma_f32_16x16x16bf16_1k, a[64][4], b[64][4] (rounded to bf16)
+ *         -> d[64][4]. No reference counterpart.                          */
+int shaw_debug_mfma(int kind, const double* a, const double* b, double* d);
 
 #ifdef __cplusplus
 }

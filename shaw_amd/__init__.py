@@ -113,6 +113,9 @@ def lib():
         "shaw_rom_sync": (i32, [vp]),
         "shaw_rom_retire": (i32, [vp, u64, vp, vp, vp, P(u64)]),
         "shaw_rom_free_slots": (i32, [vp, P(u64)]),
+        "shaw_rom_state": (i32, [vp, u64, vp, vp, P(u64)]),
+        "shaw_debug_mfma": (i32, [i32, vp, vp, vp]),
+        "shaw_pod_basis": (i32, [vp, u64, u64, u64, vp, vp]),
         "shaw_rom_time": (i32, [vp, u64, P(dbl)]),
         "shaw_rom_destroy": (None, [vp]),
     }
@@ -386,3 +389,122 @@ def fom_run(app: App, dt: float, nsteps: int, sources, rank2: bool, receivers=()
         out["snaps_sp"] = fom.snapshots("sp")
     fom.close()
     return out
+
+
+# ---------------------------------------------------------------- ROM
+def rom_project(app: App, phi_v: np.ndarray, phi_s: np.ndarray):
+    """Galerkin operators Ar_v = Phi_v^T J_vp Phi_s, Ar_s = Phi_s^T J_sp Phi_v
+    (FP64, on the GPU). phi_v [nVp x Kv], phi_s [nSp x Ks]."""
+    kv, ks = phi_v.shape[1], phi_s.shape[1]
+    pv = np.asfortranarray(phi_v, dtype=np.float64)
+    ps = np.asfortranarray(phi_s, dtype=np.float64)
+    arv = np.zeros((kv, ks))
+    ars = np.zeros((ks, kv))
+    _check(lib().shaw_rom_project(app.handle, _p(pv), kv, _p(ps), ks, _p(arv), _p(ars)))
+    return arv, ars
+
+
+class Rom:
+    """Resident reduced-order models for many forcing columns (slots)."""
+
+    def __init__(self, ar_v, ar_s, dt: float, precision=ROM_FP64, receivers=None, max_cols=16, max_samples=0):
+        """receivers: [R x Kv] rows of Phi_v at the receiver gids (or None)."""
+        self._arv, self._ars = _f64(ar_v), _f64(ar_s)
+        self.kv, self.ks = self._arv.shape
+        self._C = None if receivers is None else _f64(receivers)
+        self.R = 0 if receivers is None else self._C.shape[0]
+        d = _RomDesc(self.kv, self.ks, dt, precision, _p(self._arv), _p(self._ars), self.R, _p(self._C),
+                     max_cols, max_samples)
+        hh = C.c_void_p()
+        _check(lib().shaw_rom_create(C.byref(d), C.byref(hh)))
+        self._h = hh
+        self.every = {}
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.shaw_rom_destroy(self._h)
+            self._h = None
+
+    def close(self):
+        self.__del__()
+
+    def admit(self, b, ftab, horizons=None, every=0):
+        """b [Kv x n], ftab [nsteps x n]; returns the slot ids."""
+        b, ftab = _f64(b), _f64(ftab)
+        n = b.shape[1]
+        hz = _u64(horizons if horizons is not None else [ftab.shape[0]] * n)
+        slots = np.zeros(n, np.uint64)
+        _check(lib().shaw_rom_admit(self._h, n, _p(b), ftab.shape[0], _p(ftab), _p(hz), every, _p(slots)))
+        for s in slots:
+            self.every[int(s)] = every
+        return [int(s) for s in slots]
+
+    def advance(self, nsteps: int):
+        _check(lib().shaw_rom_advance(self._h, nsteps))
+
+    def sync(self):
+        _check(lib().shaw_rom_sync(self._h))
+
+    def state(self, slot: int):
+        qv, qs = np.zeros(self.kv), np.zeros(self.ks)
+        st = C.c_uint64()
+        _check(lib().shaw_rom_state(self._h, slot, _p(qv), _p(qs), C.byref(st)))
+        return qv, qs, int(st.value)
+
+    def retire(self, slot: int, max_samples: int = 0):
+        """-> (seismogram [T][R], qv, qs)."""
+        seis = np.zeros(max(1, self.R * max(1, max_samples)))
+        qv, qs = np.zeros(self.kv), np.zeros(self.ks)
+        t = C.c_uint64()
+        _check(lib().shaw_rom_retire(self._h, slot, _p(seis), _p(qv), _p(qs), C.byref(t)))
+        T = int(t.value)
+        return seis[: self.R * T].reshape(T, self.R), qv, qs
+
+    def free_slots(self) -> int:
+        n = C.c_uint64()
+        _check(lib().shaw_rom_free_slots(self._h, C.byref(n)))
+        return int(n.value)
+
+    def time(self, nsteps: int) -> float:
+        t = C.c_double()
+        _check(lib().shaw_rom_time(self._h, nsteps, C.byref(t)))
+        return t.value
+
+
+def rom_run(ar_v, ar_s, B, ftab, dt, nsteps, C=None, seis_freq=0, precision=ROM_FP64):
+    """Batched reduced leapfrog of all columns of B [Kv x M] for nsteps; same
+    result layout as the oracle (seismogram [M][T][R], qv [Kv x M], qs [Ks x M])."""
+    M = B.shape[1]
+    T = nsteps // seis_freq if seis_freq else 0
+    rom = Rom(ar_v, ar_s, dt, precision, C, max_cols=M, max_samples=T)  # C: receiver rows
+    slots = rom.admit(B, np.asarray(ftab)[:nsteps], [nsteps] * M, seis_freq if C is not None else 0)
+    rom.advance(nsteps)
+    rom.sync()
+    R = 0 if C is None else C.shape[0]
+    seis = np.zeros((M, T, R))
+    qv = np.zeros((ar_v.shape[0], M))
+    qs = np.zeros((ar_s.shape[0], M))
+    for j, s in enumerate(slots):
+        y, qv[:, j], qs[:, j] = rom.retire(s, T)
+        if R and T:
+            seis[j] = y
+    rom.close()
+    return {"seismogram": seis if (R and T) else None, "qv": qv, "qs": qs}
+
+
+def debug_mfma(kind: int, a, b) -> np.ndarray:
+    """Run one MFMA with lane-wise operands (see include/shaw_amd.h)."""
+    a, b = _f64(a).ravel(), _f64(b).ravel()
+    d = np.zeros(256)
+    _check(lib().shaw_debug_mfma(kind, _p(a), _p(b), _p(d)))
+    return d.reshape(64, 4)
+
+
+def pod_basis(S: np.ndarray, k: int):
+    """(Phi [n x k], sigma [ns]) of the snapshot matrix S [n x ns] (GPU, FP64)."""
+    S = np.asfortranarray(S, dtype=np.float64)
+    n, ns = S.shape
+    phi = np.zeros((k, n))  # column-major [n x k]
+    sig = np.zeros(ns)
+    _check(lib().shaw_pod_basis(_p(S), n, ns, k, _p(phi), _p(sig)))
+    return phi.T, sig

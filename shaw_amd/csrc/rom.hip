@@ -1,11 +1,891 @@
-// Galerkin ROM on gfx950 (projection + reduced leapfrog). Filled in below.
+// Galerkin ROM on gfx950.
+//
+// Projection (FP64): Ar_v = Phi_v^T (J_vp Phi_s), Ar_s = Phi_s^T (J_sp Phi_v)
+// (rom_compute_jacobians.hpp:58-109). The sparse products apply the same
+// matrix-free stencil as the FOM kernels to the basis columns; the tall-skinny
+// Phi^T (J Phi) contraction is a split-K FP64 GEMM over row chunks whose
+// partial products are reduced in a fixed order (deterministic).
+//
+// Reduced leapfrog (rom_run_rank_one.hpp:112-151 per column; batched as in
+// torevise/run_rom_rank_two_forcing.hpp:149-188), for resident column slots:
+//   q_v += dt * (Ar_v q_s) ; q_v += (f dt) b ; y = C q_v (sample steps) ;
+//   q_s += dt * (Ar_s q_v)
+// A persistent kernel: one workgroup (4 waves) owns 16 column slots, keeps
+// their q_v, q_s in LDS in FP64 for the whole launch and runs many steps;
+// operators are pre-packed in MFMA operand order and stream from L2.
+//   FP64:   v_mfma_f64_16x16x4f64, FP64 accumulation.
+//   BF16x3: Ar = Ar_hi + Ar_lo and q = q_hi + q_lo in bf16; the product uses
+//           Ar_hi q_hi + Ar_hi q_lo + Ar_lo q_hi on v_mfma_f32_16x16x16bf16_1k
+//           with FP32 accumulation; the state update q += dt * acc is FP64.
+// Each column has its own step counter and horizon; a column past its
+// horizon is frozen (serving with uneven horizons).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <vector>
+
 #include "shaw_app.hpp"
 #include "shaw_common.hpp"
+
+namespace shaw {
+namespace {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef short shrt4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTileCols = 16;  // column slots per workgroup
+constexpr int kWaves = 4;
+
+// ------------------------------------------------------------ bf16 helpers
+__host__ __device__ inline unsigned short f32ToBf16(float x) {
+  // round to nearest even
+  unsigned int u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);  // inf/nan
+  const unsigned int lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  return (unsigned short)(u >> 16);
+}
+__host__ __device__ inline float bf16ToF32(unsigned short h) {
+  const unsigned int u = (unsigned int)h << 16;
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+
+// ------------------------------------------------------------ projection
+// tmp[p + n*k] = (J_vp Phi_s)[p][k] in CSR row order; Phi column-major
+__global__ void k_apply_jvp(const double* __restrict__ cv, const double* __restrict__ phi, double* __restrict__ out,
+                            uint32_t nr, uint32_t nth, uint64_t K) {
+  const uint64_t nVp = (uint64_t)nr * nth, nSp = (uint64_t)(nr - 1) * nth + (uint64_t)nr * (nth - 1);
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t k = blockIdx.y;
+  if (p >= nVp || k >= K) return;
+  const uint64_t rs = 2ull * nth - 1, i = p % nth, j = p / nth;
+  auto stp = [&](uint64_t ii, uint64_t jj) { return jj + 1 < nr ? jj * rs + 2 * ii + 1 : (uint64_t)(nr - 1) * rs + ii; };
+  const uint64_t iN = (j < nr - 1 ? j : nr - 2) * rs + 2 * i, iS = (j > 0 ? j - 1 : 0) * rs + 2 * i;
+  const uint64_t iW = stp(i > 0 ? i - 1 : 0, j), iE = stp(i < nth - 2 ? i : nth - 2, j);
+  const double* x = phi + nSp * k;
+  const double* c = cv + 4 * p;
+  double s = c[0] * x[iN] + c[1] * x[iS];
+  s = s + c[2] * x[iW];
+  s = s + c[3] * x[iE];
+  out[p + nVp * k] = s;
+}
+
+__global__ void k_apply_jsp(const double* __restrict__ cs, const double* __restrict__ phi, double* __restrict__ out,
+                            uint32_t nr, uint32_t nth, uint64_t K) {
+  const uint64_t nVp = (uint64_t)nr * nth, nSp = (uint64_t)(nr - 1) * nth + (uint64_t)nr * (nth - 1);
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t k = blockIdx.y;
+  if (q >= nSp || k >= K) return;
+  const uint64_t rs = 2ull * nth - 1, last = (uint64_t)(nr - 1) * rs;
+  uint64_t n0, n1;
+  if (q < last) {
+    const uint64_t j = q / rs, kk = q % rs;
+    if ((kk & 1) == 0) { n0 = (j + 1) * nth + kk / 2; n1 = j * nth + kk / 2; }
+    else { n0 = j * nth + (kk - 1) / 2; n1 = n0 + 1; }
+  } else {
+    n0 = (uint64_t)(nr - 1) * nth + (q - last);
+    n1 = n0 + 1;
+  }
+  const double* x = phi + nVp * k;
+  out[q + nSp * k] = cs[2 * q] * x[n0] + cs[2 * q + 1] * x[n1];
+}
+
+// part[chunk][a][b] = sum_{r in chunk} A[r + n*a] * B[r + n*b]   (A: n x Ka, B: n x Kb, col-major)
+constexpr int GT = 32;  // output tile
+__global__ void __launch_bounds__(256) k_gemm_tn_partial(const double* __restrict__ A, const double* __restrict__ B,
+                                                         double* __restrict__ part, uint64_t n, uint32_t Ka, uint32_t Kb,
+                                                         uint64_t chunk) {
+  __shared__ double As[GT][GT + 1];
+  __shared__ double Bs[GT][GT + 1];
+  const uint32_t a0 = blockIdx.x * GT, b0 = blockIdx.y * GT;
+  const uint64_t r0 = (uint64_t)blockIdx.z * chunk, r1 = r0 + chunk < n ? r0 + chunk : n;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;  // 16 x 16 threads, 2 x 2 outputs each
+  double acc[2][2] = {{0, 0}, {0, 0}};
+  for (uint64_t r = r0; r < r1; r += GT) {
+    // load GT rows x GT columns of A and B (coalesced along r)
+    for (int e = threadIdx.x; e < GT * GT; e += 256) {
+      const int rr = e % GT, cc = e / GT;
+      const uint64_t row = r + rr;
+      As[rr][cc] = (row < r1 && a0 + cc < Ka) ? A[row + n * (a0 + cc)] : 0.0;
+      Bs[rr][cc] = (row < r1 && b0 + cc < Kb) ? B[row + n * (b0 + cc)] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int rr = 0; rr < GT; ++rr) {
+      const double x0 = As[rr][tx], x1 = As[rr][tx + 16];
+      const double y0 = Bs[rr][ty], y1 = Bs[rr][ty + 16];
+      acc[0][0] += x0 * y0;
+      acc[0][1] += x0 * y1;
+      acc[1][0] += x1 * y0;
+      acc[1][1] += x1 * y1;
+    }
+    __syncthreads();
+  }
+  double* P = part + (uint64_t)blockIdx.z * Ka * Kb;
+  for (int u = 0; u < 2; ++u)
+    for (int v = 0; v < 2; ++v) {
+      const uint32_t a = a0 + tx + 16 * u, b = b0 + ty + 16 * v;
+      if (a < Ka && b < Kb) P[(uint64_t)a * Kb + b] = acc[u][v];
+    }
+}
+
+__global__ void k_reduce_chunks(const double* __restrict__ part, double* __restrict__ out, uint64_t n, uint32_t nchunks) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = 0.0;
+  for (uint32_t c = 0; c < nchunks; ++c) s += part[(uint64_t)c * n + e];
+  out[e] = s;
+}
+
+// out[r + n*c] = sum_j S[r + n*j] * W[j*k + c]   (S: n x ns col-major, W: ns x k row-major)
+constexpr int NNC = 16;  // output columns per thread
+// out = base + alpha * (S W); base may be null (treated as 0) or alias out
+__global__ void __launch_bounds__(256) k_gemm_nn_rows(const double* __restrict__ S, const double* __restrict__ W,
+                                                      double* out, const double* base, double alpha, uint64_t n,
+                                                      uint32_t ns, uint32_t k) {
+  __shared__ double Ws[64][NNC];
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c0 = blockIdx.y * NNC;
+  double acc[NNC];
+#pragma unroll
+  for (int c = 0; c < NNC; ++c) acc[c] = 0.0;
+  for (uint32_t j0 = 0; j0 < ns; j0 += 64) {
+    for (int e = threadIdx.x; e < 64 * NNC; e += blockDim.x) {
+      const uint32_t jj = j0 + e / NNC, cc = c0 + e % NNC;
+      Ws[e / NNC][e % NNC] = (jj < ns && cc < k) ? W[(uint64_t)jj * k + cc] : 0.0;
+    }
+    __syncthreads();
+    if (r < n) {
+      const uint32_t jn = ns - j0 < 64 ? ns - j0 : 64;
+      for (uint32_t j = 0; j < jn; ++j) {
+        const double x = S[r + n * (j0 + j)];
+#pragma unroll
+        for (int c = 0; c < NNC; ++c) acc[c] += x * Ws[j][c];
+      }
+    }
+    __syncthreads();
+  }
+  if (r < n)
+    for (int c = 0; c < NNC; ++c)
+      if (c0 + c < k) {
+        const uint64_t o = r + n * (c0 + c);
+        out[o] = (base ? base[o] : 0.0) + alpha * acc[c];
+      }
+}
+
+// host: symmetric eigen-decomposition by cyclic Jacobi (A row-major n x n is
+// destroyed; eigenvalues in w, eigenvectors in the columns of V)
+void jacobiEigen(std::vector<double>& A, int n, std::vector<double>& w, std::vector<double>& V) {
+  V.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < n; ++i) {
+      diag += A[(size_t)i * n + i] * A[(size_t)i * n + i];
+      for (int j = i + 1; j < n; ++j) off += A[(size_t)i * n + j] * A[(size_t)i * n + j];
+    }
+    if (off <= 1e-30 * diag || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[(size_t)p * n + q];
+        if (apq == 0.0) continue;
+        const double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::abs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < n; ++k) {  // columns p, q
+          const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+          A[(size_t)k * n + p] = c * akp - sn * akq;
+          A[(size_t)k * n + q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // rows p, q
+          const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+          A[(size_t)p * n + k] = c * apk - sn * aqk;
+          A[(size_t)q * n + k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+          V[(size_t)k * n + p] = c * vkp - sn * vkq;
+          V[(size_t)k * n + q] = sn * vkp + c * vkq;
+        }
+      }
+  }
+  w.resize(n);
+  for (int i = 0; i < n; ++i) w[i] = A[(size_t)i * n + i];
+}
+
+// ------------------------------------------------------------ MFMA lane layouts
+// v_mfma_f64_16x16x4f64 (one 16x16 output block, K = 4):
+//   A: lane l holds A[l % 16][l / 16];  B: lane l holds B[l / 16][l % 16]
+//   D: lane l, register r holds D[l / 16 + 4 * r][l % 16]
+// v_mfma_f32_16x16x16bf16_1k:
+//   A: lane l holds A[l % 16][4 * (l / 16) + e], e = 0..3
+//   B: lane l holds B[4 * (l / 16) + e][l % 16]
+//   D: lane l, register r holds D[4 * (l / 16) + r][l % 16]
+// These are checked on the device by shaw_debug_mfma (tests/test_gpu_rom.py).
+__device__ inline dbl4 mfmaF64(double a, double b, dbl4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline flt4 mfmaBf16(shrt4 a, shrt4 b, flt4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+// result row of register r in lane l
+__device__ inline int rowF64(int lane, int r) { return lane / 16 + 4 * r; }
+__device__ inline int rowBf16(int lane, int r) { return 4 * (lane / 16) + r; }
+
+__global__ void k_mfma_probe(int kind, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ d) {
+  const int l = threadIdx.x;
+  if (kind == 0) {
+    dbl4 acc = {0, 0, 0, 0};
+    acc = mfmaF64(a[l], b[l], acc);
+    for (int r = 0; r < 4; ++r) d[4 * l + r] = acc[r];
+  } else {
+    shrt4 x, y;
+    for (int e = 0; e < 4; ++e) {
+      x[e] = (short)f32ToBf16((float)a[4 * l + e]);
+      y[e] = (short)f32ToBf16((float)b[4 * l + e]);
+    }
+    flt4 acc = {0, 0, 0, 0};
+    acc = mfmaBf16(x, y, acc);
+    for (int r = 0; r < 4; ++r) d[4 * l + r] = acc[r];
+  }
+}
+
+// ------------------------------------------------------------ reduced leapfrog
+struct RomArgs {
+  // operators packed in A-operand order: f64: [rowTile][kStep][64 lanes];
+  // bf16: [rowTile][kStep][64 lanes][4] for hi and lo
+  const double* avF64;
+  const double* asF64;
+  const double* cF64;
+  const unsigned short* avHi;
+  const unsigned short* avLo;
+  const unsigned short* asHi;
+  const unsigned short* asLo;
+  const unsigned short* cHi;
+  const unsigned short* cLo;
+  double* qv;           // [slot][Kvp]
+  double* qs;           // [slot][Ksp]
+  const double* b;      // [slot][Kvp]
+  const double* ftab;   // [slot][maxSteps]
+  uint64_t* stepCol;    // [slot]
+  const uint64_t* horizon;
+  const uint64_t* every;
+  double* seis;         // [slot][R][Tcap]
+  uint32_t Kvp, Ksp, Rp, R;
+  uint64_t maxSteps, Tcap, nsteps;
+  double dt;
+};
+
+template <bool BF16>
+__device__ inline void halfStep(const RomArgs& a, int rowTiles, int kSteps, const double* Apk, const unsigned short* Ahi,
+                                const unsigned short* Alo, const double* Xs, int ldx, double* Ys, int ldy, int wave, int lane,
+                                const bool* active, const double* fdt, const double* bcol, int slot0) {
+  // Y[rows][cols] += dt * (A X), rows owned per wave in groups of 4 row tiles
+  for (int rt0 = wave; rt0 < rowTiles; rt0 += kWaves * 4) {
+    int nt = 0;
+    int rts[4];
+    for (int u = 0; u < 4; ++u) {
+      const int rt = rt0 + u * kWaves;
+      if (rt < rowTiles) rts[nt++] = rt;
+    }
+    if (!BF16) {
+      dbl4 acc[4];
+      for (int u = 0; u < 4; ++u) acc[u] = dbl4{0, 0, 0, 0};
+      for (int ks = 0; ks < kSteps; ++ks) {
+        const double xb = Xs[(ks * 4 + lane / 16) * ldx + lane % 16];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (u < nt) acc[u] = mfmaF64(Apk[((uint64_t)rts[u] * kSteps + ks) * 64 + lane], xb, acc[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (u < nt)
+          for (int r = 0; r < 4; ++r) {
+            const int row = rts[u] * 16 + rowF64(lane, r), col = lane % 16;
+            if (active[col]) {
+              double y = Ys[row * ldy + col] + a.dt * acc[u][r];
+              if (bcol) y = y + fdt[col] * bcol[(uint64_t)(slot0 + col) * a.Kvp + row];
+              Ys[row * ldy + col] = y;
+            }
+          }
+    } else {
+      flt4 acc[4];
+      for (int u = 0; u < 4; ++u) acc[u] = flt4{0, 0, 0, 0};
+      for (int ks = 0; ks < kSteps; ++ks) {
+        shrt4 xh, xl;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float x = (float)Xs[(ks * 16 + 4 * (lane / 16) + e) * ldx + lane % 16];
+          const unsigned short h = f32ToBf16(x);
+          xh[e] = (short)h;
+          xl[e] = (short)f32ToBf16(x - bf16ToF32(h));
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (u < nt) {
+            const uint64_t o = (((uint64_t)rts[u] * kSteps + ks) * 64 + lane) * 4;
+            const shrt4 ah = *reinterpret_cast<const shrt4*>(Ahi + o);
+            const shrt4 al = *reinterpret_cast<const shrt4*>(Alo + o);
+            acc[u] = mfmaBf16(ah, xh, acc[u]);
+            acc[u] = mfmaBf16(ah, xl, acc[u]);
+            acc[u] = mfmaBf16(al, xh, acc[u]);
+          }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (u < nt)
+          for (int r = 0; r < 4; ++r) {
+            const int row = rts[u] * 16 + rowBf16(lane, r), col = lane % 16;
+            if (active[col]) {
+              double y = Ys[row * ldy + col] + a.dt * (double)acc[u][r];
+              if (bcol) y = y + fdt[col] * bcol[(uint64_t)(slot0 + col) * a.Kvp + row];
+              Ys[row * ldy + col] = y;
+            }
+          }
+    }
+  }
+}
+
+template <bool BF16>
+__device__ inline void receivers(const RomArgs& a, const double* Qv, int wave, int lane, const bool* sample,
+                                 const uint64_t* sampleIdx, int slot0) {
+  const int rowTiles = a.Rp / 16, kSteps = BF16 ? a.Kvp / 16 : a.Kvp / 4;
+  for (int rt = wave; rt < rowTiles; rt += kWaves) {
+    double out[4];
+    if (!BF16) {
+      dbl4 acc = {0, 0, 0, 0};
+      for (int ks = 0; ks < kSteps; ++ks)
+        acc = mfmaF64(a.cF64[((uint64_t)rt * kSteps + ks) * 64 + lane], Qv[(ks * 4 + lane / 16) * kTileCols + lane % 16], acc);
+      for (int r = 0; r < 4; ++r) out[r] = acc[r];
+    } else {
+      flt4 acc = {0, 0, 0, 0};
+      for (int ks = 0; ks < kSteps; ++ks) {
+        shrt4 xh, xl;
+        for (int e = 0; e < 4; ++e) {
+          const float x = (float)Qv[(ks * 16 + 4 * (lane / 16) + e) * kTileCols + lane % 16];
+          const unsigned short h = f32ToBf16(x);
+          xh[e] = (short)h;
+          xl[e] = (short)f32ToBf16(x - bf16ToF32(h));
+        }
+        const uint64_t o = (((uint64_t)rt * kSteps + ks) * 64 + lane) * 4;
+        const shrt4 ah = *reinterpret_cast<const shrt4*>(a.cHi + o);
+        const shrt4 al = *reinterpret_cast<const shrt4*>(a.cLo + o);
+        acc = mfmaBf16(ah, xh, acc);
+        acc = mfmaBf16(ah, xl, acc);
+        acc = mfmaBf16(al, xh, acc);
+      }
+      for (int r = 0; r < 4; ++r) out[r] = (double)acc[r];
+    }
+    const int col = lane % 16;
+    if (sample[col])
+      for (int r = 0; r < 4; ++r) {
+        const int row = rt * 16 + (BF16 ? rowBf16(lane, r) : rowF64(lane, r));
+        // per slot LayoutLeft [R x Tcap]
+        if (row < (int)a.R) a.seis[(uint64_t)(slot0 + col) * a.R * a.Tcap + (uint64_t)a.R * sampleIdx[col] + row] = out[r];
+      }
+  }
+}
+
+template <bool BF16>
+__global__ void __launch_bounds__(256) k_rom_steps(RomArgs a) {
+  extern __shared__ double sm[];
+  double* Qv = sm;                                   // [Kvp][16]
+  double* Qs = Qv + (size_t)a.Kvp * kTileCols;       // [Ksp][16]
+  __shared__ bool active[kTileCols], sample[kTileCols];
+  __shared__ double fdt[kTileCols];
+  __shared__ uint64_t sampleIdx[kTileCols];
+  __shared__ uint64_t s0[kTileCols];
+  const int slot0 = blockIdx.x * kTileCols;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  for (int e = tid; e < (int)a.Kvp * kTileCols; e += blockDim.x) {
+    const int row = e / kTileCols, col = e % kTileCols;
+    Qv[e] = a.qv[(uint64_t)(slot0 + col) * a.Kvp + row];
+  }
+  for (int e = tid; e < (int)a.Ksp * kTileCols; e += blockDim.x) {
+    const int row = e / kTileCols, col = e % kTileCols;
+    Qs[e] = a.qs[(uint64_t)(slot0 + col) * a.Ksp + row];
+  }
+  if (tid < kTileCols) s0[tid] = a.stepCol[slot0 + tid];
+  __syncthreads();
+  const int kvTiles = a.Kvp / 16, ksTiles = a.Ksp / 16;
+  const int kStepsS = BF16 ? a.Ksp / 16 : a.Ksp / 4, kStepsV = BF16 ? a.Kvp / 16 : a.Kvp / 4;
+  bool anyLeft = true;
+  for (uint64_t it = 0; it < a.nsteps && anyLeft; ++it) {
+    if (tid < kTileCols) {
+      const uint64_t s = s0[tid] + it;  // steps already taken by this column
+      const bool act = s < a.horizon[slot0 + tid];
+      active[tid] = act;
+      fdt[tid] = act ? a.ftab[(uint64_t)(slot0 + tid) * a.maxSteps + s] * a.dt : 0.0;
+      const uint64_t ev = a.every[slot0 + tid];
+      sample[tid] = act && ev > 0 && (s + 1) % ev == 0;
+      sampleIdx[tid] = ev > 0 ? (s + 1) / ev - 1 : 0;
+    }
+    __syncthreads();
+    bool any = false, anySample = false;
+    for (int c = 0; c < kTileCols; ++c) { any |= active[c]; anySample |= sample[c]; }
+    anyLeft = any;
+    if (!any) break;
+    // velocity: q_v += dt Ar_v q_s ; q_v += (f dt) b
+    halfStep<BF16>(a, kvTiles, kStepsS, a.avF64, a.avHi, a.avLo, Qs, kTileCols, Qv, kTileCols, wave, lane, active, fdt, a.b,
+                   slot0);
+    __syncthreads();
+    if (anySample) receivers<BF16>(a, Qv, wave, lane, sample, sampleIdx, slot0);
+    // stress: q_s += dt Ar_s q_v
+    halfStep<BF16>(a, ksTiles, kStepsV, a.asF64, a.asHi, a.asLo, Qv, kTileCols, Qs, kTileCols, wave, lane, active, fdt,
+                   nullptr, slot0);
+    __syncthreads();
+  }
+  for (int e = tid; e < (int)a.Kvp * kTileCols; e += blockDim.x) {
+    const int row = e / kTileCols, col = e % kTileCols;
+    a.qv[(uint64_t)(slot0 + col) * a.Kvp + row] = Qv[e];
+  }
+  for (int e = tid; e < (int)a.Ksp * kTileCols; e += blockDim.x) {
+    const int row = e / kTileCols, col = e % kTileCols;
+    a.qs[(uint64_t)(slot0 + col) * a.Ksp + row] = Qs[e];
+  }
+  if (tid < kTileCols) {
+    const uint64_t h = a.horizon[slot0 + tid];
+    const uint64_t s = s0[tid] + a.nsteps;
+    a.stepCol[slot0 + tid] = s < h ? s : (s0[tid] < h ? h : s0[tid]);
+  }
+}
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  void alloc(size_t count) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    n = count;
+    if (count) SHAW_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+  }
+  void upload(const T* h, size_t count, size_t off = 0) {
+    if (count) SHAW_HIP(hipMemcpy(p + off, h, count * sizeof(T), hipMemcpyHostToDevice));
+  }
+};
+
+uint32_t roundUp(uint64_t x, uint32_t m) { return (uint32_t)((x + m - 1) / m * m); }
+
+// pack a row-major [rows x cols] operator (zero padded to rowsP x colsP) in
+// MFMA A-operand order
+void packF64(const double* A, uint64_t rows, uint64_t cols, uint32_t rowsP, uint32_t colsP, std::vector<double>& out) {
+  const uint32_t rt = rowsP / 16, ks = colsP / 4;
+  out.assign((size_t)rt * ks * 64, 0.0);
+  for (uint32_t t = 0; t < rt; ++t)
+    for (uint32_t k = 0; k < ks; ++k)
+      for (int l = 0; l < 64; ++l) {
+        const uint64_t r = t * 16 + l % 16, c = k * 4 + l / 16;
+        out[((size_t)t * ks + k) * 64 + l] = (r < rows && c < cols) ? A[r * cols + c] : 0.0;
+      }
+}
+void packBf16(const double* A, uint64_t rows, uint64_t cols, uint32_t rowsP, uint32_t colsP,
+              std::vector<unsigned short>& hi, std::vector<unsigned short>& lo) {
+  const uint32_t rt = rowsP / 16, ks = colsP / 16;
+  hi.assign((size_t)rt * ks * 64 * 4, 0);
+  lo.assign(hi.size(), 0);
+  for (uint32_t t = 0; t < rt; ++t)
+    for (uint32_t k = 0; k < ks; ++k)
+      for (int l = 0; l < 64; ++l)
+        for (int e = 0; e < 4; ++e) {
+          const uint64_t r = t * 16 + l % 16, c = k * 16 + 4 * (l / 16) + e;
+          const double v = (r < rows && c < cols) ? A[r * cols + c] : 0.0;
+          const float vf = (float)v;
+          const unsigned short h = f32ToBf16(vf);
+          const size_t o = (((size_t)t * ks + k) * 64 + l) * 4 + e;
+          hi[o] = h;
+          lo[o] = f32ToBf16((float)(v - (double)bf16ToF32(h)));
+        }
+}
+
+}  // namespace
+}  // namespace shaw
+
+struct shaw_rom {
+  uint64_t kv = 0, ks = 0, R = 0, maxCols = 0, maxSamples = 0, maxSteps = 0;
+  uint32_t kvp = 0, ksp = 0, rp = 0;
+  double dt = 0;
+  int precision = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  shaw::DevBuf<double> avF64, asF64, cF64, qv, qs, b, ftab, seis;
+  shaw::DevBuf<unsigned short> avHi, avLo, asHi, asLo, cHi, cLo;
+  shaw::DevBuf<uint64_t> stepCol, horizon, every;
+  std::vector<uint64_t> hStep, hHorizon, hEvery;
+  std::vector<uint8_t> used;
+  ~shaw_rom() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void launch(uint64_t nsteps) {
+    using namespace shaw;
+    RomArgs a{};
+    a.avF64 = avF64.p; a.asF64 = asF64.p; a.cF64 = cF64.p;
+    a.avHi = avHi.p; a.avLo = avLo.p; a.asHi = asHi.p; a.asLo = asLo.p; a.cHi = cHi.p; a.cLo = cLo.p;
+    a.qv = qv.p; a.qs = qs.p; a.b = b.p; a.ftab = ftab.p;
+    a.stepCol = stepCol.p; a.horizon = horizon.p; a.every = every.p; a.seis = seis.p;
+    a.Kvp = kvp; a.Ksp = ksp; a.Rp = rp; a.R = (uint32_t)R;
+    a.maxSteps = maxSteps; a.Tcap = maxSamples; a.nsteps = nsteps; a.dt = dt;
+    const unsigned blocks = (unsigned)(maxCols / kTileCols);
+    const size_t lds = (size_t)(kvp + ksp) * kTileCols * sizeof(double);
+    if (precision == SHAW_ROM_FP64) {
+      (void)hipFuncSetAttribute((const void*)k_rom_steps<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(k_rom_steps<false>, dim3(blocks), dim3(256), lds, stream, a);
+    } else {
+      (void)hipFuncSetAttribute((const void*)k_rom_steps<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(k_rom_steps<true>, dim3(blocks), dim3(256), lds, stream, a);
+    }
+    SHAW_HIP(hipGetLastError());
+    for (uint64_t c = 0; c < maxCols; ++c) {
+      const uint64_t s = hStep[c] + nsteps;
+      hStep[c] = s < hHorizon[c] ? s : std::max(hStep[c], hHorizon[c]);
+    }
+  }
+};
 
 using shaw::guarded;
 
 extern "C" {
-int shaw_rom_project(shaw_app_t, const double*, uint64_t, const double*, uint64_t, double*, double*) {
-  return guarded([&] { throw shaw::Error(SHAW_ERR_STATE, "ROM not built yet"); });
+
+int shaw_rom_project(shaw_app_t app, const double* phiV, uint64_t kv, const double* phiS, uint64_t ks, double* arV,
+                     double* arS) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(app && phiV && phiS && arV && arS && kv > 0 && ks > 0, SHAW_ERR_ARG, "bad argument");
+    const Grid& g = app->ops.grid;
+    const uint64_t nVp = g.nVp(), nSp = g.nSp();
+    DevBuf<double> cv, cs, pv, ps, tmp, part, out;
+    cv.alloc(app->ops.coefVp.size()); cv.upload(app->ops.coefVp.data(), cv.n);
+    cs.alloc(app->ops.coefSp.size()); cs.upload(app->ops.coefSp.data(), cs.n);
+    pv.alloc(nVp * kv); pv.upload(phiV, pv.n);
+    ps.alloc(nSp * ks); ps.upload(phiS, ps.n);
+    auto gemm = [&](const double* A, uint64_t n, uint32_t Ka, const double* B, uint32_t Kb, double* host) {
+      const uint64_t chunk = std::max<uint64_t>(4096, (n + 255) / 256 / GT * GT + GT);
+      const uint32_t nch = (uint32_t)((n + chunk - 1) / chunk);
+      part.alloc((size_t)nch * Ka * Kb);
+      out.alloc((size_t)Ka * Kb);
+      dim3 grid((Ka + GT - 1) / GT, (Kb + GT - 1) / GT, nch);
+      hipLaunchKernelGGL(k_gemm_tn_partial, grid, dim3(256), 0, 0, A, B, part.p, n, Ka, Kb, chunk);
+      const uint64_t ne = (uint64_t)Ka * Kb;
+      hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, 0, part.p, out.p, ne, nch);
+      SHAW_HIP(hipGetLastError());
+      SHAW_HIP(hipMemcpy(host, out.p, ne * sizeof(double), hipMemcpyDeviceToHost));
+    };
+    // velocity: tmp = J_vp Phi_s (nVp x ks); Ar_v = Phi_v^T tmp
+    tmp.alloc(nVp * ks);
+    hipLaunchKernelGGL(k_apply_jvp, dim3((unsigned)((nVp + 255) / 256), (unsigned)ks), dim3(256), 0, 0, cv.p, ps.p, tmp.p,
+                       (uint32_t)g.nr, (uint32_t)g.nth, ks);
+    gemm(pv.p, nVp, (uint32_t)kv, tmp.p, (uint32_t)ks, arV);
+    // stress: tmp = J_sp Phi_v (nSp x kv); Ar_s = Phi_s^T tmp
+    tmp.alloc(nSp * kv);
+    hipLaunchKernelGGL(k_apply_jsp, dim3((unsigned)((nSp + 255) / 256), (unsigned)kv), dim3(256), 0, 0, cs.p, pv.p, tmp.p,
+                       (uint32_t)g.nr, (uint32_t)g.nth, kv);
+    gemm(ps.p, nSp, (uint32_t)ks, tmp.p, (uint32_t)kv, arS);
+  });
 }
+
+int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(d && out && d->ar_v && d->ar_s, SHAW_ERR_ARG, "null argument");
+    SHAW_REQUIRE(d->kv > 0 && d->ks > 0 && d->kv <= 2048 && d->ks <= 2048, SHAW_ERR_ARG, "ROM size must be in 1..2048");
+    SHAW_REQUIRE(d->precision == SHAW_ROM_FP64 || d->precision == SHAW_ROM_BF16X3, SHAW_ERR_ARG, "unknown ROM precision");
+    SHAW_REQUIRE(d->max_cols > 0 && d->dt > 0, SHAW_ERR_ARG, "max_cols and dt must be > 0");
+    SHAW_REQUIRE(d->r == 0 || d->c, SHAW_ERR_ARG, "receiver rows missing");
+    int ndev = 0;
+    SHAW_HIP(hipGetDeviceCount(&ndev));
+    SHAW_REQUIRE(ndev > 0, SHAW_ERR_HIP, "no HIP device available");
+    auto h = std::make_unique<shaw_rom>();
+    h->kv = d->kv; h->ks = d->ks; h->R = d->r; h->dt = d->dt; h->precision = d->precision;
+    h->kvp = roundUp(d->kv, 16);
+    h->ksp = roundUp(d->ks, 16);
+    h->rp = roundUp(std::max<uint64_t>(d->r, 1), 16);
+    h->maxCols = roundUp(d->max_cols, kTileCols);
+    h->maxSamples = d->max_samples;
+    const size_t lds = (size_t)(h->kvp + h->ksp) * kTileCols * sizeof(double);
+    SHAW_REQUIRE(lds <= 160 * 1024, SHAW_ERR_ARG, "ROM size too large for the LDS-resident state tile");
+    SHAW_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    SHAW_HIP(hipEventCreate(&h->ev0));
+    SHAW_HIP(hipEventCreate(&h->ev1));
+    std::vector<double> cPad((size_t)h->rp * d->kv, 0.0);
+    if (d->r) std::memcpy(cPad.data(), d->c, d->r * d->kv * sizeof(double));
+    if (d->precision == SHAW_ROM_FP64) {
+      std::vector<double> p;
+      packF64(d->ar_v, d->kv, d->ks, h->kvp, h->ksp, p); h->avF64.alloc(p.size()); h->avF64.upload(p.data(), p.size());
+      packF64(d->ar_s, d->ks, d->kv, h->ksp, h->kvp, p); h->asF64.alloc(p.size()); h->asF64.upload(p.data(), p.size());
+      packF64(cPad.data(), h->rp, d->kv, h->rp, h->kvp, p); h->cF64.alloc(p.size()); h->cF64.upload(p.data(), p.size());
+    } else {
+      std::vector<unsigned short> hi, lo;
+      packBf16(d->ar_v, d->kv, d->ks, h->kvp, h->ksp, hi, lo);
+      h->avHi.alloc(hi.size()); h->avHi.upload(hi.data(), hi.size()); h->avLo.alloc(lo.size()); h->avLo.upload(lo.data(), lo.size());
+      packBf16(d->ar_s, d->ks, d->kv, h->ksp, h->kvp, hi, lo);
+      h->asHi.alloc(hi.size()); h->asHi.upload(hi.data(), hi.size()); h->asLo.alloc(lo.size()); h->asLo.upload(lo.data(), lo.size());
+      packBf16(cPad.data(), h->rp, d->kv, h->rp, h->kvp, hi, lo);
+      h->cHi.alloc(hi.size()); h->cHi.upload(hi.data(), hi.size()); h->cLo.alloc(lo.size()); h->cLo.upload(lo.data(), lo.size());
+    }
+    const uint64_t C = h->maxCols;
+    h->qv.alloc(C * h->kvp); h->qs.alloc(C * h->ksp); h->b.alloc(C * h->kvp);
+    SHAW_HIP(hipMemset(h->qv.p, 0, h->qv.n * 8)); SHAW_HIP(hipMemset(h->qs.p, 0, h->qs.n * 8));
+    SHAW_HIP(hipMemset(h->b.p, 0, h->b.n * 8));
+    h->stepCol.alloc(C); h->horizon.alloc(C); h->every.alloc(C);
+    h->hStep.assign(C, 0); h->hHorizon.assign(C, 0); h->hEvery.assign(C, 0); h->used.assign(C, 0);
+    h->stepCol.upload(h->hStep.data(), C); h->horizon.upload(h->hHorizon.data(), C); h->every.upload(h->hEvery.data(), C);
+    h->seis.alloc(C * std::max<uint64_t>(h->R, 1) * std::max<uint64_t>(h->maxSamples, 1));
+    *out = h.release();
+  });
 }
+
+int shaw_rom_admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const double* ftab, const uint64_t* horizons,
+                   uint64_t every, uint64_t* slots) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && b && ftab && horizons && slots, SHAW_ERR_ARG, "null argument");
+    SHAW_REQUIRE(h->R == 0 || every > 0, SHAW_ERR_ARG, "sampling period must be > 0 when receivers are set");
+    std::vector<uint64_t> free;
+    for (uint64_t c = 0; c < h->maxCols && free.size() < n; ++c)
+      if (!h->used[c]) free.push_back(c);
+    SHAW_REQUIRE(free.size() == n, SHAW_ERR_MEMORY, "not enough free ROM column slots");
+    for (uint64_t j = 0; j < n; ++j) {
+      SHAW_REQUIRE(horizons[j] <= nsteps, SHAW_ERR_ARG, "horizon longer than the forcing table");
+      if (h->R) SHAW_REQUIRE(horizons[j] / every <= h->maxSamples, SHAW_ERR_ARG, "horizon exceeds the seismogram capacity");
+    }
+    const uint64_t need = std::max<uint64_t>(h->maxSteps, nsteps);
+    if (need > h->maxSteps) {  // grow the per-slot forcing store, keeping admitted columns
+      shaw::DevBuf<double> nb;
+      nb.alloc(h->maxCols * need);
+      SHAW_HIP(hipMemset(nb.p, 0, nb.n * 8));
+      if (h->ftab.p && h->maxSteps)
+        SHAW_HIP(hipMemcpy2D(nb.p, need * 8, h->ftab.p, h->maxSteps * 8, h->maxSteps * 8, h->maxCols, hipMemcpyDeviceToDevice));
+      std::swap(nb.p, h->ftab.p);
+      std::swap(nb.n, h->ftab.n);
+      h->maxSteps = need;
+    }
+    std::vector<double> col(h->kvp), f(nsteps);
+    const std::vector<double> zv(std::max(h->kvp, h->ksp), 0.0);
+    for (uint64_t j = 0; j < n; ++j) {
+      const uint64_t s = free[j];
+      for (uint64_t k = 0; k < h->kvp; ++k) col[k] = k < h->kv ? b[k * n + j] : 0.0;
+      h->b.upload(col.data(), h->kvp, s * h->kvp);
+      for (uint64_t t = 0; t < nsteps; ++t) f[t] = ftab[t * n + j];
+      h->ftab.upload(f.data(), nsteps, s * h->maxSteps);
+      h->qv.upload(zv.data(), h->kvp, s * h->kvp);
+      h->qs.upload(zv.data(), h->ksp, s * h->ksp);
+      h->hStep[s] = 0;
+      h->hHorizon[s] = horizons[j];
+      h->hEvery[s] = h->R ? every : 0;
+      h->used[s] = 1;
+      slots[j] = s;
+    }
+    h->stepCol.upload(h->hStep.data(), h->maxCols);
+    h->horizon.upload(h->hHorizon.data(), h->maxCols);
+    h->every.upload(h->hEvery.data(), h->maxCols);
+  });
+}
+
+int shaw_rom_advance(shaw_rom_t h, uint64_t nsteps) {
+  return guarded([&] {
+    SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle");
+    if (nsteps) h->launch(nsteps);
+  });
+}
+
+int shaw_rom_sync(shaw_rom_t h) {
+  return guarded([&] {
+    SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle");
+    SHAW_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int shaw_rom_state(shaw_rom_t h, uint64_t slot, double* qv, double* qs, uint64_t* step) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && slot < h->maxCols && h->used[slot], SHAW_ERR_ARG, "bad slot");
+    SHAW_HIP(hipStreamSynchronize(h->stream));
+    if (qv) SHAW_HIP(hipMemcpy(qv, h->qv.p + slot * h->kvp, h->kv * 8, hipMemcpyDeviceToHost));
+    if (qs) SHAW_HIP(hipMemcpy(qs, h->qs.p + slot * h->ksp, h->ks * 8, hipMemcpyDeviceToHost));
+    if (step) *step = h->hStep[slot];
+  });
+}
+
+int shaw_rom_retire(shaw_rom_t h, uint64_t slot, double* seis, double* qv, double* qs, uint64_t* samples) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && slot < h->maxCols && h->used[slot], SHAW_ERR_ARG, "bad slot");
+    SHAW_HIP(hipStreamSynchronize(h->stream));
+    const uint64_t T = h->hEvery[slot] ? h->hStep[slot] / h->hEvery[slot] : 0;
+    if (seis && T && h->R)
+      SHAW_HIP(hipMemcpy(seis, h->seis.p + slot * h->R * h->maxSamples, h->R * T * 8, hipMemcpyDeviceToHost));
+    if (qv) SHAW_HIP(hipMemcpy(qv, h->qv.p + slot * h->kvp, h->kv * 8, hipMemcpyDeviceToHost));
+    if (qs) SHAW_HIP(hipMemcpy(qs, h->qs.p + slot * h->ksp, h->ks * 8, hipMemcpyDeviceToHost));
+    if (samples) *samples = T;
+    h->used[slot] = 0;
+    h->hHorizon[slot] = 0;
+    h->hStep[slot] = 0;
+    h->hEvery[slot] = 0;
+    h->horizon.upload(&h->hHorizon[slot], 1, slot);
+    h->stepCol.upload(&h->hStep[slot], 1, slot);
+    h->every.upload(&h->hEvery[slot], 1, slot);
+  });
+}
+
+int shaw_rom_free_slots(shaw_rom_t h, uint64_t* n) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && n, SHAW_ERR_ARG, "null argument");
+    *n = (uint64_t)std::count(h->used.begin(), h->used.end(), 0);
+  });
+}
+
+int shaw_rom_time(shaw_rom_t h, uint64_t nsteps, double* total) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && total, SHAW_ERR_ARG, "null argument");
+    SHAW_HIP(hipEventRecord(h->ev0, h->stream));
+    h->launch(nsteps);
+    SHAW_HIP(hipEventRecord(h->ev1, h->stream));
+    SHAW_HIP(hipEventSynchronize(h->ev1));
+    float ms = 0;
+    SHAW_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    *total = ms;
+  });
+}
+
+void shaw_rom_destroy(shaw_rom_t h) {
+  if (h) guarded([&] { delete h; });
+}
+
+int shaw_pod_basis(const double* S, uint64_t n, uint64_t ns, uint64_t k, double* phi, double* sigma) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(S && phi && n > 0 && ns > 0 && k > 0, SHAW_ERR_ARG, "bad argument");
+    SHAW_REQUIRE(k <= ns && k <= n, SHAW_ERR_ARG, "POD: number of modes exceeds the number of snapshots");
+    SHAW_REQUIRE(ns <= 4096, SHAW_ERR_ARG, "POD: at most 4096 snapshots");
+    int ndev = 0;
+    SHAW_HIP(hipGetDeviceCount(&ndev));
+    SHAW_REQUIRE(ndev > 0, SHAW_ERR_HIP, "no HIP device available");
+    DevBuf<double> dS, part, dG, dW, dPhi, dNew, dTmp;
+    dS.alloc(n * ns);
+    dS.upload(S, n * ns);
+    dPhi.alloc(n * k);
+    // host <- A^T B (A: n x ka, B: n x kb, column-major on the device)
+    auto gramTN = [&](const double* A, uint32_t ka, const double* B, uint32_t kb, std::vector<double>& G) {
+      const uint64_t chunk = std::max<uint64_t>(4096, (n + 255) / 256 / GT * GT + GT);
+      const uint32_t nch = (uint32_t)((n + chunk - 1) / chunk);
+      part.alloc((size_t)nch * ka * kb);
+      dG.alloc((size_t)ka * kb);
+      hipLaunchKernelGGL(k_gemm_tn_partial, dim3((ka + GT - 1) / GT, (kb + GT - 1) / GT, nch), dim3(256), 0, 0, A, B,
+                         part.p, n, ka, kb, chunk);
+      hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)(((uint64_t)ka * kb + 255) / 256)), dim3(256), 0, 0, part.p,
+                         dG.p, (uint64_t)ka * kb, nch);
+      SHAW_HIP(hipGetLastError());
+      G.resize((size_t)ka * kb);
+      SHAW_HIP(hipMemcpy(G.data(), dG.p, G.size() * 8, hipMemcpyDeviceToHost));
+    };
+    // out = base + alpha * X W (X: n x kin on the device, W: kin x kout row-major on the host)
+    auto applyNN = [&](const double* X, uint32_t kin, const std::vector<double>& Wh, uint32_t kout, double* out,
+                       const double* base, double alpha) {
+      dW.alloc(Wh.size());
+      dW.upload(Wh.data(), Wh.size());
+      hipLaunchKernelGGL(k_gemm_nn_rows, dim3((unsigned)((n + 255) / 256), (kout + NNC - 1) / NNC), dim3(256), 0, 0, X,
+                         dW.p, out, base, alpha, n, kin, kout);
+      SHAW_HIP(hipGetLastError());
+    };
+    // Phi_new <- Phi_new L^-T, Phi_new^T Phi_new = L L^T
+    auto cholOrth = [&](double* P, uint32_t r) {
+      std::vector<double> G2;
+      gramTN(P, r, P, r, G2);
+      std::vector<double> L((size_t)r * r, 0.0);
+      for (uint32_t i = 0; i < r; ++i)
+        for (uint32_t j = 0; j <= i; ++j) {
+          double sum = G2[(size_t)i * r + j];
+          for (uint32_t q = 0; q < j; ++q) sum -= L[(size_t)i * r + q] * L[(size_t)j * r + q];
+          if (i == j) {
+            SHAW_REQUIRE(sum > 0, SHAW_ERR_ARG, "POD: basis lost orthogonality (snapshots too ill-conditioned)");
+            L[(size_t)i * r + i] = std::sqrt(sum);
+          } else {
+            L[(size_t)i * r + j] = sum / L[(size_t)j * r + j];
+          }
+        }
+      std::vector<double> X((size_t)r * r, 0.0), W2((size_t)r * r);
+      for (uint32_t c = 0; c < r; ++c)
+        for (uint32_t i = 0; i < r; ++i) {
+          double sum = i == c ? 1.0 : 0.0;
+          for (uint32_t q = 0; q < i; ++q) sum -= L[(size_t)i * r + q] * X[(size_t)q * r + c];
+          X[(size_t)i * r + c] = sum / L[(size_t)i * r + i];
+        }
+      for (uint32_t i = 0; i < r; ++i)
+        for (uint32_t j = 0; j < r; ++j) W2[(size_t)i * r + j] = X[(size_t)j * r + i];
+      dTmp.alloc((size_t)n * r);
+      applyNN(P, r, W2, r, dTmp.p, nullptr, 1.0);
+      SHAW_HIP(hipMemcpy(P, dTmp.p, (size_t)n * r * 8, hipMemcpyDeviceToDevice));
+    };
+    // Deflated method of snapshots: each pass takes the modes of the current
+    // residual snapshots S_c whose singular values are resolved by its Gram
+    // matrix (sigma > 1e-6 sigma_max(S_c)), orthogonalizes them, and removes
+    // them from S_c; small singular values are then computed relative to the
+    // residual, not to sigma_1 (accuracy of a QR-based SVD, not of S^T S).
+    std::vector<double> sv;
+    std::vector<double> lastTail;
+    uint64_t have = 0;
+    double sigma1 = 0.0;
+    for (int pass = 0; pass < 8 && have < k; ++pass) {
+      std::vector<double> G, w, V;
+      gramTN(dS.p, (uint32_t)ns, dS.p, (uint32_t)ns, G);
+      jacobiEigen(G, (int)ns, w, V);
+      std::vector<int> order(ns);
+      for (uint64_t i = 0; i < ns; ++i) order[i] = (int)i;
+      std::sort(order.begin(), order.end(), [&](int x, int y) { return w[x] > w[y]; });
+      std::vector<double> sc(ns);
+      for (uint64_t i = 0; i < ns; ++i) sc[i] = std::sqrt(std::max(w[order[i]], 0.0));
+      if (pass == 0) sigma1 = sc[0];
+      lastTail.assign(sc.begin(), sc.end());
+      uint64_t r = 0;
+      while (r < k - have && sc[r] > 1e-6 * sc[0] && sc[r] > 1e-14 * sigma1) ++r;
+      if (r == 0) break;
+      std::vector<double> W(ns * r);
+      for (uint64_t j = 0; j < ns; ++j)
+        for (uint64_t c = 0; c < r; ++c) W[j * r + c] = V[(size_t)j * ns + order[c]] / sc[c];
+      dNew.alloc(n * r);
+      applyNN(dS.p, (uint32_t)ns, W, (uint32_t)r, dNew.p, nullptr, 1.0);
+      if (have) {  // orthogonalize against the modes already taken
+        std::vector<double> X;
+        gramTN(dPhi.p, (uint32_t)have, dNew.p, (uint32_t)r, X);
+        applyNN(dPhi.p, (uint32_t)have, X, (uint32_t)r, dNew.p, dNew.p, -1.0);
+      }
+      cholOrth(dNew.p, (uint32_t)r);
+      SHAW_HIP(hipMemcpy(dPhi.p + n * have, dNew.p, n * r * 8, hipMemcpyDeviceToDevice));
+      for (uint64_t c = 0; c < r; ++c) sv.push_back(sc[c]);
+      have += r;
+      // S_c <- S_c - Phi_new (Phi_new^T S_c)
+      std::vector<double> Y;
+      gramTN(dNew.p, (uint32_t)r, dS.p, (uint32_t)ns, Y);
+      applyNN(dNew.p, (uint32_t)r, Y, (uint32_t)ns, dS.p, dS.p, -1.0);
+    }
+    SHAW_REQUIRE(have == k, SHAW_ERR_ARG,
+                 "POD: requested modes exceed the numerical rank of the snapshots (" + std::to_string(have) +
+                     " resolved)");
+    if (sigma) {
+      for (uint64_t i = 0; i < ns; ++i) sigma[i] = i < have ? sv[i] : (i - have < lastTail.size() ? lastTail[i - have] : 0.0);
+    }
+    SHAW_HIP(hipMemcpy(phi, dPhi.p, n * k * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+int shaw_debug_mfma(int kind, const double* a, const double* b, double* d) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(a && b && d && (kind == 0 || kind == 1), SHAW_ERR_ARG, "bad argument");
+    const size_t na = kind == 0 ? 64 : 256;
+    DevBuf<double> da, db, dd;
+    da.alloc(na); db.alloc(na); dd.alloc(256);
+    da.upload(a, na); db.upload(b, na);
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, kind, da.p, db.p, dd.p);
+    SHAW_HIP(hipGetLastError());
+    SHAW_HIP(hipMemcpy(d, dd.p, 256 * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"

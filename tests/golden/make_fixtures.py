@@ -8,6 +8,7 @@ and stores them as data:
                             nonPhysicalFullMesh5x5
   mesh_sha256.json          SHA-256 of every file of the four fixture meshes
   inputs/<test>.yaml        the regression tests' input files
+  inputs/parser/*.yaml      the parser unit tests' input files
   fom_golds.npz             every present FOM gold (snaps_vp/snaps_sp/seismogram)
                             as float64, lossless (17 significant digits parse back
                             to the exact double)
@@ -99,6 +100,10 @@ def main(ref: str) -> None:
                 continue
             arrays[f"{t}/{fn[:-5]}"] = _gold_array(os.path.join(tests, t, fn), not fn.startswith("seismogram"))
     np.savez_compressed(os.path.join(HERE, "fom_golds.npz"), **arrays)
+    os.makedirs(os.path.join(HERE, "inputs", "parser"), exist_ok=True)
+    for i in range(1, 7):  # tests/parser/input{1..6}.yaml
+        shutil.copyfile(os.path.join(tests, "parser", f"input{i}.yaml"),
+                        os.path.join(HERE, "inputs", "parser", f"input{i}.yaml"))
 
     # unit golds
     rd = lambda p: open(os.path.join(tests, p)).read()
