@@ -166,6 +166,10 @@ int shaw_fom_time(shaw_fom_t h, uint64_t nsteps, double* total_ms, double* ms_pe
 int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel_bytes, double* model_bytes);
 /* Kernel configuration actually used: kernel, cols_per_thread, strip, segment. */
 int shaw_fom_config(shaw_fom_t h, int* kernel, int* cols_per_thread, int* strip, int* segment);
+/* POD basis of the device snapshot buffer of `dof` (all configured columns,
+ * all samples: LayoutLeft [n x (T * ncols)]) without a host round trip, as
+ * shaw_pod_basis; the snapshot buffer is consumed. FP64 FOM only.         */
+int shaw_fom_pod(shaw_fom_t h, int dof, uint64_t k, double* phi, double* sigma);
 void shaw_fom_destroy(shaw_fom_t h);
 
 /* ------------------------------------------------------------------ ROM --
@@ -220,7 +224,7 @@ void shaw_rom_destroy(shaw_rom_t h);
  * S LayoutLeft [n x ns] (the lsv_<dof> file content) and ns singular values,
  * descending, in FP64 on the GPU by a deflated method of snapshots: each pass
  * eigen-decomposes the Gram matrix of the residual snapshots (Jacobi), keeps
- * the modes it resolves (sigma > 1e-6 of the residual's largest), orthogonal-
+ * the modes it resolves (sigma > 1e-3 of the residual's largest), orthogonal-
  * izes them (Cholesky) and projects them out of the residual. Fails if k
  * exceeds the numerical rank. Replaces computeThinSVD
  * (src/tools/main_eigen_svd.cc:113-197). phi: LayoutLeft [n x k].          */

@@ -104,6 +104,7 @@ def lib():
         "shaw_fom_bytes_per_step": (i32, [vp, P(dbl), P(dbl)]),
         "shaw_fom_config": (i32, [vp, P(i32), P(i32), P(i32), P(i32)]),
         "shaw_fom_destroy": (None, [vp]),
+        "shaw_fom_pod": (i32, [vp, i32, u64, vp, vp]),
     }
     optional = {
         "shaw_rom_project": (i32, [vp, vp, u64, vp, u64, vp, vp]),
@@ -352,6 +353,14 @@ class Fom:
         k, m = C.c_double(), C.c_double()
         _check(lib().shaw_fom_bytes_per_step(self._h, C.byref(k), C.byref(m)))
         return k.value, m.value
+
+    def pod(self, dof: str, k: int, nsnap: int):
+        """(Phi [n x k], sigma) of this run's snapshots of `dof` (consumes them)."""
+        n = self.app.nVp if dof == "vp" else self.app.nSp
+        phi = np.zeros((k, n))
+        sig = np.zeros(nsnap)
+        _check(lib().shaw_fom_pod(self._h, 0 if dof == "vp" else 1, k, _p(phi), _p(sig)))
+        return phi.T, sig
 
     def config(self):
         v = [C.c_int() for _ in range(4)]

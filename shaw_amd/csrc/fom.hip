@@ -35,6 +35,7 @@
 #include <memory>
 #include <vector>
 
+#include "rom_internal.hpp"
 #include "shaw_app.hpp"
 #include "shaw_common.hpp"
 
@@ -214,23 +215,32 @@ __global__ void __launch_bounds__(512) k_fused(StepArgs<T> a) {
   for (int k = 0; k < CPT; ++k) { vj[k] = T(0); Eold[k] = T(0); S[k] = T(0); }
   if (active) load(S, sIn + srpIdx(i, j0 > 0 ? j0 - 1 : 0, rs) * a.MC);
 
+  // software pipeline: the old values of row r + 1 are loaded while row r is
+  // computed (nN/nE/nVo/nC/nW hold the next row)
+  T nN[CPT], nE[CPT], nVo[CPT], nW[CPT], nC[4];
+  auto fetch = [&](uint64_t r) {
+    const uint64_t p = r * nth + i;
+    load(nN, sIn + srpIdx(i, r < nr - 1 ? r : nr - 2, rs) * a.MC);
+    load(nE, sIn + stpIdx(i < nth - 2 ? i : nth - 2, r, rs, nr) * a.MC);
+    load(nVo, vIn + p * a.MC);
+    load(nC, a.cv + 4 * p);
+    if (pl == 0) load(nW, sIn + stpIdx(i > 0 ? i - 1 : 0, r, rs, nr) * a.MC);
+  };
+  if (active && j0 < nr) fetch(j0);
+
   for (int64_t jj = (int64_t)j0 - 1; jj < (int64_t)j1; ++jj) {
     const uint64_t r = (uint64_t)(jj + 1);
     const bool haveRow = r < nr;
     T N[CPT], vo[CPT], E[CPT], vn[CPT];
     T c[4];
     if (active && haveRow) {
-      const uint64_t p = r * nth + i;
-      load(N, sIn + srpIdx(i, r < nr - 1 ? r : nr - 2, rs) * a.MC);
-      load(E, sIn + stpIdx(i < nth - 2 ? i : nth - 2, r, rs, nr) * a.MC);
-      load(vo, vIn + p * a.MC);
-      load(c, a.cv + 4 * p);
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) { N[k] = nN[k]; E[k] = nE[k]; vo[k] = nVo[k]; }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) c[k] = nC[k];
       store(stpRow + (pl + 1) * a.MC + m0, E);
-      if (pl == 0) {
-        T Wh[CPT];
-        load(Wh, sIn + stpIdx(i > 0 ? i - 1 : 0, r, rs, nr) * a.MC);
-        store(stpRow + m0, Wh);
-      }
+      if (pl == 0) store(stpRow + m0, nW);
+      if (r + 1 < nr && r + 1 <= j1) fetch(r + 1);
     }
     __syncthreads();
     if (active && haveRow) {
@@ -332,6 +342,7 @@ struct shaw_fom {
   virtual void time(uint64_t nsteps, double* total, double* perStep) = 0;
   virtual void bytes(double* kernel, double* model) const = 0;
   virtual void config(int* kernel, int* cpt, int* strip, int* seg) const = 0;
+  virtual void pod(int dof, uint64_t k, double* phi, double* sigma) = 0;
 };
 
 namespace shaw {
@@ -522,6 +533,16 @@ class FomEngine final : public shaw_fom {
       *kernel = s * cols * passes * (double)(nVp_ + nSp_) + coef;
     }
     if (model) *model = s * (double)M_ * 3.0 * (double)(nVp_ + nSp_) + 8.0 * (4.0 * nVp_ + 2.0 * nSp_);
+  }
+  void pod(int dof, uint64_t k, double* phi, double* sigma) override {
+    SHAW_REQUIRE(sizeof(T) == 8, SHAW_ERR_ARG, "POD of snapshots needs the FP64 FOM");
+    Snap& s = snap_[dof];
+    SHAW_REQUIRE(s.ncols > 0, SHAW_ERR_STATE, "no snapshots were configured for this dof");
+    const uint64_t T_ = step_ / s.every, n = dof == 0 ? nVp_ : nSp_;
+    SHAW_REQUIRE(T_ == s.Tcap, SHAW_ERR_STATE, "POD needs a full snapshot buffer (advance to max_samples)");
+    sync();
+    podDevice(s.buf.p, n, T_ * s.ncols, k, phi, sigma);
+    s.ncols = 0;  // the buffer was consumed
   }
   void config(int* kernel, int* cpt, int* strip, int* seg) const override {
     if (kernel) *kernel = kernel_;
@@ -727,6 +748,12 @@ int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel, double* model) {
 }
 int shaw_fom_config(shaw_fom_t h, int* kernel, int* cpt, int* strip, int* seg) {
   return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->config(kernel, cpt, strip, seg); });
+}
+int shaw_fom_pod(shaw_fom_t h, int dof, uint64_t k, double* phi, double* sigma) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && phi && (dof == 0 || dof == 1), SHAW_ERR_ARG, "bad argument");
+    h->pod(dof, k, phi, sigma);
+  });
 }
 void shaw_fom_destroy(shaw_fom_t h) {
   if (h) guarded([&] { delete h; });

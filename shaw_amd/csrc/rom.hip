@@ -26,6 +26,7 @@
 #include <memory>
 #include <vector>
 
+#include "rom_internal.hpp"
 #include "shaw_app.hpp"
 #include "shaw_common.hpp"
 
@@ -507,6 +508,121 @@ void packBf16(const double* A, uint64_t rows, uint64_t cols, uint32_t rowsP, uin
 }  // namespace
 }  // namespace shaw
 
+namespace shaw {
+// POD of device snapshots dS [n x ns] column-major (overwritten by the
+// deflation); phi (host, [n x k] column-major) and sigma (host, ns values).
+void podDevice(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* phi, double* sigma) {
+    SHAW_REQUIRE(dSp && phi && n > 0 && ns > 0 && k > 0, SHAW_ERR_ARG, "bad argument");
+    SHAW_REQUIRE(k <= ns && k <= n, SHAW_ERR_ARG, "POD: number of modes exceeds the number of snapshots");
+    SHAW_REQUIRE(ns <= 4096, SHAW_ERR_ARG, "POD: at most 4096 snapshots");
+    DevBuf<double> part, dG, dW, dPhi, dNew, dTmp;
+    struct { double* p; } dS{dSp};
+    dPhi.alloc(n * k);
+    // host <- A^T B (A: n x ka, B: n x kb, column-major on the device)
+    auto gramTN = [&](const double* A, uint32_t ka, const double* B, uint32_t kb, std::vector<double>& G) {
+      const uint64_t chunk = std::max<uint64_t>(4096, (n + 255) / 256 / GT * GT + GT);
+      const uint32_t nch = (uint32_t)((n + chunk - 1) / chunk);
+      part.alloc((size_t)nch * ka * kb);
+      dG.alloc((size_t)ka * kb);
+      hipLaunchKernelGGL(k_gemm_tn_partial, dim3((ka + GT - 1) / GT, (kb + GT - 1) / GT, nch), dim3(256), 0, 0, A, B,
+                         part.p, n, ka, kb, chunk);
+      hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)(((uint64_t)ka * kb + 255) / 256)), dim3(256), 0, 0, part.p,
+                         dG.p, (uint64_t)ka * kb, nch);
+      SHAW_HIP(hipGetLastError());
+      G.resize((size_t)ka * kb);
+      SHAW_HIP(hipMemcpy(G.data(), dG.p, G.size() * 8, hipMemcpyDeviceToHost));
+    };
+    // out = base + alpha * X W (X: n x kin on the device, W: kin x kout row-major on the host)
+    auto applyNN = [&](const double* X, uint32_t kin, const std::vector<double>& Wh, uint32_t kout, double* out,
+                       const double* base, double alpha) {
+      dW.alloc(Wh.size());
+      dW.upload(Wh.data(), Wh.size());
+      hipLaunchKernelGGL(k_gemm_nn_rows, dim3((unsigned)((n + 255) / 256), (kout + NNC - 1) / NNC), dim3(256), 0, 0, X,
+                         dW.p, out, base, alpha, n, kin, kout);
+      SHAW_HIP(hipGetLastError());
+    };
+    // Phi_new <- Phi_new L^-T, Phi_new^T Phi_new = L L^T
+    auto cholOrth = [&](double* P, uint32_t r) {
+      std::vector<double> G2;
+      gramTN(P, r, P, r, G2);
+      std::vector<double> L((size_t)r * r, 0.0);
+      for (uint32_t i = 0; i < r; ++i)
+        for (uint32_t j = 0; j <= i; ++j) {
+          double sum = G2[(size_t)i * r + j];
+          for (uint32_t q = 0; q < j; ++q) sum -= L[(size_t)i * r + q] * L[(size_t)j * r + q];
+          if (i == j) {
+            SHAW_REQUIRE(sum > 0, SHAW_ERR_ARG, "POD: basis lost orthogonality (snapshots too ill-conditioned)");
+            L[(size_t)i * r + i] = std::sqrt(sum);
+          } else {
+            L[(size_t)i * r + j] = sum / L[(size_t)j * r + j];
+          }
+        }
+      std::vector<double> X((size_t)r * r, 0.0), W2((size_t)r * r);
+      for (uint32_t c = 0; c < r; ++c)
+        for (uint32_t i = 0; i < r; ++i) {
+          double sum = i == c ? 1.0 : 0.0;
+          for (uint32_t q = 0; q < i; ++q) sum -= L[(size_t)i * r + q] * X[(size_t)q * r + c];
+          X[(size_t)i * r + c] = sum / L[(size_t)i * r + i];
+        }
+      for (uint32_t i = 0; i < r; ++i)
+        for (uint32_t j = 0; j < r; ++j) W2[(size_t)i * r + j] = X[(size_t)j * r + i];
+      dTmp.alloc((size_t)n * r);
+      applyNN(P, r, W2, r, dTmp.p, nullptr, 1.0);
+      SHAW_HIP(hipMemcpy(P, dTmp.p, (size_t)n * r * 8, hipMemcpyDeviceToDevice));
+    };
+    // Deflated method of snapshots: each pass takes the modes of the current
+    // residual snapshots S_c whose singular values its Gram matrix resolves
+    // to ~eps sigma_max(S_c) (sigma > 1e-3 sigma_max(S_c): the eigenvalue error
+    // eps sigma_max^2 costs < 1e-13 sigma_max in sigma), orthogonalizes them and
+    // removes them from S_c; smaller singular values are computed relative to
+    // the residual, not to sigma_1 (the accuracy of a QR-based SVD).
+    std::vector<double> sv;
+    std::vector<double> lastTail;
+    uint64_t have = 0;
+    double sigma1 = 0.0;
+    for (int pass = 0; pass < 24 && have < k; ++pass) {
+      std::vector<double> G, w, V;
+      gramTN(dS.p, (uint32_t)ns, dS.p, (uint32_t)ns, G);
+      jacobiEigen(G, (int)ns, w, V);
+      std::vector<int> order(ns);
+      for (uint64_t i = 0; i < ns; ++i) order[i] = (int)i;
+      std::sort(order.begin(), order.end(), [&](int x, int y) { return w[x] > w[y]; });
+      std::vector<double> sc(ns);
+      for (uint64_t i = 0; i < ns; ++i) sc[i] = std::sqrt(std::max(w[order[i]], 0.0));
+      if (pass == 0) sigma1 = sc[0];
+      lastTail.assign(sc.begin(), sc.end());
+      uint64_t r = 0;
+      while (r < k - have && sc[r] > 1e-3 * sc[0] && sc[r] > 1e-14 * sigma1) ++r;
+      if (r == 0) break;
+      std::vector<double> W(ns * r);
+      for (uint64_t j = 0; j < ns; ++j)
+        for (uint64_t c = 0; c < r; ++c) W[j * r + c] = V[(size_t)j * ns + order[c]] / sc[c];
+      dNew.alloc(n * r);
+      applyNN(dS.p, (uint32_t)ns, W, (uint32_t)r, dNew.p, nullptr, 1.0);
+      if (have) {  // orthogonalize against the modes already taken
+        std::vector<double> X;
+        gramTN(dPhi.p, (uint32_t)have, dNew.p, (uint32_t)r, X);
+        applyNN(dPhi.p, (uint32_t)have, X, (uint32_t)r, dNew.p, dNew.p, -1.0);
+      }
+      cholOrth(dNew.p, (uint32_t)r);
+      SHAW_HIP(hipMemcpy(dPhi.p + n * have, dNew.p, n * r * 8, hipMemcpyDeviceToDevice));
+      for (uint64_t c = 0; c < r; ++c) sv.push_back(sc[c]);
+      have += r;
+      // S_c <- S_c - Phi_new (Phi_new^T S_c)
+      std::vector<double> Y;
+      gramTN(dNew.p, (uint32_t)r, dS.p, (uint32_t)ns, Y);
+      applyNN(dNew.p, (uint32_t)r, Y, (uint32_t)ns, dS.p, dS.p, -1.0);
+    }
+    SHAW_REQUIRE(have == k, SHAW_ERR_ARG,
+                 "POD: requested modes exceed the numerical rank of the snapshots (" + std::to_string(have) +
+                     " resolved)");
+    if (sigma) {
+      for (uint64_t i = 0; i < ns; ++i) sigma[i] = i < have ? sv[i] : (i - have < lastTail.size() ? lastTail[i - have] : 0.0);
+    }
+    SHAW_HIP(hipMemcpy(phi, dPhi.p, n * k * 8, hipMemcpyDeviceToHost));
+}
+}  // namespace shaw
+
 struct shaw_rom {
   uint64_t kv = 0, ks = 0, R = 0, maxCols = 0, maxSamples = 0, maxSteps = 0;
   uint32_t kvp = 0, ksp = 0, rp = 0;
@@ -757,120 +873,18 @@ void shaw_rom_destroy(shaw_rom_t h) {
   if (h) guarded([&] { delete h; });
 }
 
+
 int shaw_pod_basis(const double* S, uint64_t n, uint64_t ns, uint64_t k, double* phi, double* sigma) {
   using namespace shaw;
   return guarded([&] {
     SHAW_REQUIRE(S && phi && n > 0 && ns > 0 && k > 0, SHAW_ERR_ARG, "bad argument");
-    SHAW_REQUIRE(k <= ns && k <= n, SHAW_ERR_ARG, "POD: number of modes exceeds the number of snapshots");
-    SHAW_REQUIRE(ns <= 4096, SHAW_ERR_ARG, "POD: at most 4096 snapshots");
     int ndev = 0;
     SHAW_HIP(hipGetDeviceCount(&ndev));
     SHAW_REQUIRE(ndev > 0, SHAW_ERR_HIP, "no HIP device available");
-    DevBuf<double> dS, part, dG, dW, dPhi, dNew, dTmp;
+    DevBuf<double> dS;
     dS.alloc(n * ns);
     dS.upload(S, n * ns);
-    dPhi.alloc(n * k);
-    // host <- A^T B (A: n x ka, B: n x kb, column-major on the device)
-    auto gramTN = [&](const double* A, uint32_t ka, const double* B, uint32_t kb, std::vector<double>& G) {
-      const uint64_t chunk = std::max<uint64_t>(4096, (n + 255) / 256 / GT * GT + GT);
-      const uint32_t nch = (uint32_t)((n + chunk - 1) / chunk);
-      part.alloc((size_t)nch * ka * kb);
-      dG.alloc((size_t)ka * kb);
-      hipLaunchKernelGGL(k_gemm_tn_partial, dim3((ka + GT - 1) / GT, (kb + GT - 1) / GT, nch), dim3(256), 0, 0, A, B,
-                         part.p, n, ka, kb, chunk);
-      hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)(((uint64_t)ka * kb + 255) / 256)), dim3(256), 0, 0, part.p,
-                         dG.p, (uint64_t)ka * kb, nch);
-      SHAW_HIP(hipGetLastError());
-      G.resize((size_t)ka * kb);
-      SHAW_HIP(hipMemcpy(G.data(), dG.p, G.size() * 8, hipMemcpyDeviceToHost));
-    };
-    // out = base + alpha * X W (X: n x kin on the device, W: kin x kout row-major on the host)
-    auto applyNN = [&](const double* X, uint32_t kin, const std::vector<double>& Wh, uint32_t kout, double* out,
-                       const double* base, double alpha) {
-      dW.alloc(Wh.size());
-      dW.upload(Wh.data(), Wh.size());
-      hipLaunchKernelGGL(k_gemm_nn_rows, dim3((unsigned)((n + 255) / 256), (kout + NNC - 1) / NNC), dim3(256), 0, 0, X,
-                         dW.p, out, base, alpha, n, kin, kout);
-      SHAW_HIP(hipGetLastError());
-    };
-    // Phi_new <- Phi_new L^-T, Phi_new^T Phi_new = L L^T
-    auto cholOrth = [&](double* P, uint32_t r) {
-      std::vector<double> G2;
-      gramTN(P, r, P, r, G2);
-      std::vector<double> L((size_t)r * r, 0.0);
-      for (uint32_t i = 0; i < r; ++i)
-        for (uint32_t j = 0; j <= i; ++j) {
-          double sum = G2[(size_t)i * r + j];
-          for (uint32_t q = 0; q < j; ++q) sum -= L[(size_t)i * r + q] * L[(size_t)j * r + q];
-          if (i == j) {
-            SHAW_REQUIRE(sum > 0, SHAW_ERR_ARG, "POD: basis lost orthogonality (snapshots too ill-conditioned)");
-            L[(size_t)i * r + i] = std::sqrt(sum);
-          } else {
-            L[(size_t)i * r + j] = sum / L[(size_t)j * r + j];
-          }
-        }
-      std::vector<double> X((size_t)r * r, 0.0), W2((size_t)r * r);
-      for (uint32_t c = 0; c < r; ++c)
-        for (uint32_t i = 0; i < r; ++i) {
-          double sum = i == c ? 1.0 : 0.0;
-          for (uint32_t q = 0; q < i; ++q) sum -= L[(size_t)i * r + q] * X[(size_t)q * r + c];
-          X[(size_t)i * r + c] = sum / L[(size_t)i * r + i];
-        }
-      for (uint32_t i = 0; i < r; ++i)
-        for (uint32_t j = 0; j < r; ++j) W2[(size_t)i * r + j] = X[(size_t)j * r + i];
-      dTmp.alloc((size_t)n * r);
-      applyNN(P, r, W2, r, dTmp.p, nullptr, 1.0);
-      SHAW_HIP(hipMemcpy(P, dTmp.p, (size_t)n * r * 8, hipMemcpyDeviceToDevice));
-    };
-    // Deflated method of snapshots: each pass takes the modes of the current
-    // residual snapshots S_c whose singular values are resolved by its Gram
-    // matrix (sigma > 1e-6 sigma_max(S_c)), orthogonalizes them, and removes
-    // them from S_c; small singular values are then computed relative to the
-    // residual, not to sigma_1 (accuracy of a QR-based SVD, not of S^T S).
-    std::vector<double> sv;
-    std::vector<double> lastTail;
-    uint64_t have = 0;
-    double sigma1 = 0.0;
-    for (int pass = 0; pass < 8 && have < k; ++pass) {
-      std::vector<double> G, w, V;
-      gramTN(dS.p, (uint32_t)ns, dS.p, (uint32_t)ns, G);
-      jacobiEigen(G, (int)ns, w, V);
-      std::vector<int> order(ns);
-      for (uint64_t i = 0; i < ns; ++i) order[i] = (int)i;
-      std::sort(order.begin(), order.end(), [&](int x, int y) { return w[x] > w[y]; });
-      std::vector<double> sc(ns);
-      for (uint64_t i = 0; i < ns; ++i) sc[i] = std::sqrt(std::max(w[order[i]], 0.0));
-      if (pass == 0) sigma1 = sc[0];
-      lastTail.assign(sc.begin(), sc.end());
-      uint64_t r = 0;
-      while (r < k - have && sc[r] > 1e-6 * sc[0] && sc[r] > 1e-14 * sigma1) ++r;
-      if (r == 0) break;
-      std::vector<double> W(ns * r);
-      for (uint64_t j = 0; j < ns; ++j)
-        for (uint64_t c = 0; c < r; ++c) W[j * r + c] = V[(size_t)j * ns + order[c]] / sc[c];
-      dNew.alloc(n * r);
-      applyNN(dS.p, (uint32_t)ns, W, (uint32_t)r, dNew.p, nullptr, 1.0);
-      if (have) {  // orthogonalize against the modes already taken
-        std::vector<double> X;
-        gramTN(dPhi.p, (uint32_t)have, dNew.p, (uint32_t)r, X);
-        applyNN(dPhi.p, (uint32_t)have, X, (uint32_t)r, dNew.p, dNew.p, -1.0);
-      }
-      cholOrth(dNew.p, (uint32_t)r);
-      SHAW_HIP(hipMemcpy(dPhi.p + n * have, dNew.p, n * r * 8, hipMemcpyDeviceToDevice));
-      for (uint64_t c = 0; c < r; ++c) sv.push_back(sc[c]);
-      have += r;
-      // S_c <- S_c - Phi_new (Phi_new^T S_c)
-      std::vector<double> Y;
-      gramTN(dNew.p, (uint32_t)r, dS.p, (uint32_t)ns, Y);
-      applyNN(dNew.p, (uint32_t)r, Y, (uint32_t)ns, dS.p, dS.p, -1.0);
-    }
-    SHAW_REQUIRE(have == k, SHAW_ERR_ARG,
-                 "POD: requested modes exceed the numerical rank of the snapshots (" + std::to_string(have) +
-                     " resolved)");
-    if (sigma) {
-      for (uint64_t i = 0; i < ns; ++i) sigma[i] = i < have ? sv[i] : (i - have < lastTail.size() ? lastTail[i - have] : 0.0);
-    }
-    SHAW_HIP(hipMemcpy(phi, dPhi.p, n * k * 8, hipMemcpyDeviceToHost));
+    podDevice(dS.p, n, ns, k, phi, sigma);
   });
 }
 

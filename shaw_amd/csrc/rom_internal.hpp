@@ -1,0 +1,9 @@
+// Entry points shared between the FOM and ROM translation units.
+#pragma once
+#include <cstdint>
+
+namespace shaw {
+// POD of device-resident snapshots dS [n x ns] column-major (overwritten);
+// phi: host [n x k] column-major, sigma: host [ns] (may be null).
+void podDevice(double* dS, uint64_t n, uint64_t ns, uint64_t k, double* phi, double* sigma);
+}  // namespace shaw

@@ -1,0 +1,85 @@
+"""Galerkin ROM workflow on one GPU, from FOM snapshots to batched ROM runs.
+
+The reference's offline chain is: shawExe (FOM with snapshot matrices) ->
+computeThinSVD (POD, src/tools/main_eigen_svd.cc) -> shawExe with a `rom`
+section (rom_load_basis.hpp, rom_compute_jacobians.hpp, rom_run_rank_one.hpp)
+-> reconstructSeismogram (src/tools/main_reconstruct_seismo.cc). Here the same
+chain stays on the device: the training FOM keeps its snapshots in HBM, POD
+runs on them in FP64, the projection runs on the GPU in FP64, and the batched
+reduced leapfrog runs on MFMA with receivers fused (y = Phi_v[recv, :] q_v).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+import shaw_amd
+
+
+@dataclass
+class RomBasis:
+    phi_v: np.ndarray   # [nVp x K]
+    phi_s: np.ndarray   # [nSp x K]
+    sigma_v: np.ndarray
+    sigma_s: np.ndarray
+    seconds: dict
+
+
+def train_basis(app: shaw_amd.App, sources, dt: float, nsteps: int, snap_every: int, k: int,
+                rank1_forcing: bool = False) -> RomBasis:
+    """Run the FOM for the training forcings with snapshots of both dofs every
+    `snap_every` steps (all in HBM), then POD each dof to k modes (FP64)."""
+    t0 = time.time()
+    m = len(sources)
+    fom = shaw_amd.Fom(app, m, dt, rank1_forcing=rank1_forcing)
+    fom.set_sources([app.map_source(d, a) for d, a, p, dl in sources])
+    fom.set_forcing(shaw_amd.forcing_table(["ricker"] * m, [s[3] for s in sources], [s[2] for s in sources],
+                                           dt, nsteps))
+    T = nsteps // snap_every
+    cols = list(range(m))
+    fom.set_snapshots("vp", snap_every, cols, T)
+    fom.set_snapshots("sp", snap_every, cols, T)
+    fom.advance(nsteps)
+    fom.sync()
+    t1 = time.time()
+    phi_v, sig_v = fom.pod("vp", k, T * m)
+    phi_s, sig_s = fom.pod("sp", k, T * m)
+    fom.close()
+    t2 = time.time()
+    return RomBasis(phi_v, phi_s, sig_v, sig_s, {"fom_training": t1 - t0, "pod": t2 - t1})
+
+
+def operators(app: shaw_amd.App, basis: RomBasis, k: int):
+    """Galerkin operators of the first k modes (POD bases are nested)."""
+    return shaw_amd.rom_project(app, basis.phi_v[:, :k], basis.phi_s[:, :k])
+
+
+def forcing_inputs(app: shaw_amd.App, phi_v: np.ndarray, sources, dt: float, nsteps: int):
+    """B [K x m] with column j = rho^-1[g_j] Phi_v[g_j, :] (rom_problem_rank_one.hpp:133-147)
+    and the forcing table [nsteps x m]."""
+    _, _, rho_inv, _ = app.operators()
+    gids = [app.map_source(d, a) for d, a, p, dl in sources]
+    B = np.stack([rho_inv[g] * phi_v[g, :] for g in gids], axis=1)
+    ftab = shaw_amd.forcing_table(["ricker"] * len(sources), [s[3] for s in sources], [s[2] for s in sources],
+                                  dt, nsteps)
+    return B, ftab
+
+
+def receiver_rows(app: shaw_amd.App, phi_v: np.ndarray, receivers):
+    """C [R x K] = Phi_v[recv, :] (main_reconstruct_seismo.cc:13-41)."""
+    return np.stack([phi_v[app.map_receiver(a), :] for a in receivers], axis=0)
+
+
+def fom_seismograms(app: shaw_amd.App, sources, dt: float, nsteps: int, receivers, every: int,
+                    rank1_forcing: bool = False):
+    """FP64 FOM seismograms [m][T][R] of the given forcings (the reference FOM)."""
+    out = shaw_amd.fom_run(app, dt, nsteps, sources, True, receivers, every, rank1_forcing=rank1_forcing) \
+        if len(sources) > 1 else shaw_amd.fom_run(app, dt, nsteps, sources, False, receivers, every)
+    return out["seismogram"]
+
+
+def rel_l2(x, ref) -> float:
+    n = np.linalg.norm(ref)
+    return float(np.linalg.norm(x - ref) / n) if n > 0 else float(np.linalg.norm(x))

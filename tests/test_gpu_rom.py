@@ -183,11 +183,15 @@ def read_matrix_binary(path):
 
 
 def check_pod(phi, sig, U, s, k):
-    """Singular values within 1e-13 sigma_1 + 1e-8 sigma_k; orthonormal modes;
-    every mode separated from its neighbours by > 1 % (relative gap) spans the
-    same direction as the reference's, |cos| > 1 - 1e-6 (backward-stable SVDs
-    agree on a mode to ~ eps sigma_1 / (gap sigma_k))."""
-    np.testing.assert_array_less(np.abs(sig[:k] - s[:k]), 1e-13 * s[0] + 1e-8 * s[:k])
+    """Singular values within 1e-12 sigma_1 (the scale of a backward-stable
+    SVD's absolute error) and 1e-10 relative where sigma_k > 1e-3 sigma_1;
+    orthonormal modes; every mode separated from its neighbours by > 1 %
+    (relative gap) spans the same direction as the reference's,
+    |cos| > 1 - 1e-6 (SVDs agree on a mode to ~ eps sigma_1 / (gap sigma_k))."""
+    err = np.abs(sig[:k] - s[:k])
+    np.testing.assert_array_less(err, 1e-12 * s[0])
+    big = s[:k] > 1e-3 * s[0]
+    np.testing.assert_array_less(err[big], 1e-10 * s[:k][big])
     np.testing.assert_allclose(phi.T @ phi, np.eye(k), atol=1e-10)
     for i in range(k):
         gap = min([abs(s[i] - s[j]) / s[i] for j in (i - 1, i + 1) if 0 <= j < len(s)])
