@@ -237,6 +237,16 @@ int shaw_pod_basis(const double* s, uint64_t n, uint64_t ns, uint64_t k, double*
  * kind 1: v_mfma_f32_16x16x16bf16_1k, a[64][4], b[64][4] (rounded to bf16)
  *         -> d[64][4]. No reference counterpart.                          */
 int shaw_debug_mfma(int kind, const double* a, const double* b, double* d);
+/* Host symmetric eigen-decomposition used by the POD (Householder
+ * tridiagonalization + implicit-shift QL): a row-major [n x n] symmetric,
+ * w[n] eigenvalues (unordered), v row-major [n x n] with eigenvectors in its
+ * columns. CPU only. No reference counterpart.                             */
+int shaw_debug_symeig(int n, const double* a, double* w, double* v);
+/* Time `reps` launches of a plain streaming copy kernel (k_copy) of `bytes`
+ * bytes (read `bytes`, write `bytes`); used to calibrate the HBM traffic
+ * counters and to state the achievable copy bandwidth. No reference
+ * counterpart.                                                             */
+int shaw_debug_copy(uint64_t bytes, int reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }

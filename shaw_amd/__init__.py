@@ -117,6 +117,8 @@ def lib():
         "shaw_rom_state": (i32, [vp, u64, vp, vp, P(u64)]),
         "shaw_debug_mfma": (i32, [i32, vp, vp, vp]),
         "shaw_pod_basis": (i32, [vp, u64, u64, u64, vp, vp]),
+        "shaw_debug_copy": (i32, [u64, i32, P(dbl)]),
+        "shaw_debug_symeig": (i32, [i32, vp, vp, vp]),
         "shaw_rom_time": (i32, [vp, u64, P(dbl)]),
         "shaw_rom_destroy": (None, [vp]),
     }
@@ -376,7 +378,8 @@ def fom_run(app: App, dt: float, nsteps: int, sources, rank2: bool, receivers=()
     m = len(sources)
     if not rank2 and m != 1:
         raise ShawError("rank-1 run needs one source")
-    fom = Fom(app, m, dt, rank1_forcing=not rank2, **fom_kw)
+    rank1_forcing = fom_kw.pop("rank1_forcing", not rank2)
+    fom = Fom(app, m, dt, rank1_forcing=rank1_forcing, **fom_kw)
     fom.set_sources([app.map_source(d, a) for d, a, p, dl in sources])
     fom.set_forcing(forcing_table([signal] * m, [s[3] for s in sources], [s[2] for s in sources], dt, nsteps))
     if seis_freq and len(receivers):
@@ -517,3 +520,20 @@ def pod_basis(S: np.ndarray, k: int):
     sig = np.zeros(ns)
     _check(lib().shaw_pod_basis(_p(S), n, ns, k, _p(phi), _p(sig)))
     return phi.T, sig
+
+
+def debug_copy(nbytes: int, reps: int = 10) -> float:
+    """ms per launch of the plain copy kernel over nbytes (read + write)."""
+    ms = C.c_double()
+    _check(lib().shaw_debug_copy(nbytes, reps, C.byref(ms)))
+    return ms.value
+
+
+def debug_symeig(a: np.ndarray):
+    """(w, V) of the host eigensolver the POD uses (CPU only)."""
+    a = _f64(a)
+    n = a.shape[0]
+    w = np.zeros(n)
+    v = np.zeros((n, n))
+    _check(lib().shaw_debug_symeig(n, _p(a), _p(w), _p(v)))
+    return w, v

@@ -302,6 +302,13 @@ __global__ void k_snapshot(const T* __restrict__ x, uint64_t n, uint32_t MC, con
   out[g + n * (sample + Tcap * c)] = (double)x[((uint64_t)(col / MC) * n + g) * MC + col % MC];
 }
 
+// plain streaming copy (16 B per lane per iteration), for bandwidth and
+// counter calibration
+__global__ void __launch_bounds__(256) k_copy(const double2* __restrict__ src, double2* __restrict__ dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // ------------------------------------------------------------ engine
 struct FomBase {
   virtual ~FomBase() = default;
@@ -755,6 +762,33 @@ int shaw_fom_pod(shaw_fom_t h, int dof, uint64_t k, double* phi, double* sigma) 
     h->pod(dof, k, phi, sigma);
   });
 }
+int shaw_debug_copy(uint64_t bytes, int reps, double* ms_per_copy) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(ms_per_copy && reps > 0 && bytes >= 16, SHAW_ERR_ARG, "bad argument");
+    const uint64_t n = bytes / 16;
+    DevBuf<double> a, b;
+    a.alloc(2 * n);
+    b.alloc(2 * n);
+    SHAW_HIP(hipMemset(a.p, 0, n * 16));
+    hipEvent_t e0, e1;
+    SHAW_HIP(hipEventCreate(&e0));
+    SHAW_HIP(hipEventCreate(&e1));
+    const unsigned blocks = 256 * 16;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, (const double2*)a.p, (double2*)b.p, n);  // warm
+    SHAW_HIP(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, (const double2*)a.p, (double2*)b.p, n);
+    SHAW_HIP(hipEventRecord(e1, 0));
+    SHAW_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SHAW_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_copy = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}
+
 void shaw_fom_destroy(shaw_fom_t h) {
   if (h) guarded([&] { delete h; });
 }

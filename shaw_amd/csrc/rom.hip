@@ -22,7 +22,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <vector>
 
@@ -180,45 +182,116 @@ __global__ void __launch_bounds__(256) k_gemm_nn_rows(const double* __restrict__
       }
 }
 
-// host: symmetric eigen-decomposition by cyclic Jacobi (A row-major n x n is
-// destroyed; eigenvalues in w, eigenvectors in the columns of V)
-void jacobiEigen(std::vector<double>& A, int n, std::vector<double>& w, std::vector<double>& V) {
-  V.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0.0, diag = 0.0;
-    for (int i = 0; i < n; ++i) {
-      diag += A[(size_t)i * n + i] * A[(size_t)i * n + i];
-      for (int j = i + 1; j < n; ++j) off += A[(size_t)i * n + j] * A[(size_t)i * n + j];
-    }
-    if (off <= 1e-30 * diag || off == 0.0) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = A[(size_t)p * n + q];
-        if (apq == 0.0) continue;
-        const double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::abs(theta) + std::sqrt(theta * theta + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
-        for (int k = 0; k < n; ++k) {  // columns p, q
-          const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
-          A[(size_t)k * n + p] = c * akp - sn * akq;
-          A[(size_t)k * n + q] = sn * akp + c * akq;
+// host: symmetric eigen-decomposition (A row-major n x n, destroyed):
+// Householder reduction to tridiagonal form with the transformations
+// accumulated, then the QL algorithm with implicit Wilkinson shifts
+// (Wilkinson & Reinsch, Handbook for Automatic Computation II, tred2/tql2).
+// On return w holds the eigenvalues (unordered) and column k of V the
+// eigenvector of w[k].
+void symmetricEigen(std::vector<double>& A, int n, std::vector<double>& w, std::vector<double>& V) {
+  auto a = [&](int i, int j) -> double& { return A[(size_t)i * n + j]; };
+  std::vector<double> d(n, 0.0), e(n, 0.0);
+  for (int i = n - 1; i > 0; --i) {
+    const int l = i - 1;
+    double h = 0.0;
+    if (l > 0) {
+      double scale = 0.0;
+      for (int k = 0; k <= l; ++k) scale += std::abs(a(i, k));
+      if (scale == 0.0) {
+        e[i] = a(i, l);
+      } else {
+        for (int k = 0; k <= l; ++k) {
+          a(i, k) /= scale;
+          h += a(i, k) * a(i, k);
         }
-        for (int k = 0; k < n; ++k) {  // rows p, q
-          const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
-          A[(size_t)p * n + k] = c * apk - sn * aqk;
-          A[(size_t)q * n + k] = sn * apk + c * aqk;
+        double f = a(i, l);
+        double g = f >= 0.0 ? -std::sqrt(h) : std::sqrt(h);
+        e[i] = scale * g;
+        h -= f * g;
+        a(i, l) = f - g;
+        f = 0.0;
+        for (int j = 0; j <= l; ++j) {
+          a(j, i) = a(i, j) / h;
+          g = 0.0;
+          for (int k = 0; k <= j; ++k) g += a(j, k) * a(i, k);
+          for (int k = j + 1; k <= l; ++k) g += a(k, j) * a(i, k);
+          e[j] = g / h;
+          f += e[j] * a(i, j);
         }
-        for (int k = 0; k < n; ++k) {
-          const double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
-          V[(size_t)k * n + p] = c * vkp - sn * vkq;
-          V[(size_t)k * n + q] = sn * vkp + c * vkq;
+        const double hh = f / (h + h);
+        for (int j = 0; j <= l; ++j) {
+          f = a(i, j);
+          e[j] = g = e[j] - hh * f;
+          for (int k = 0; k <= j; ++k) a(j, k) -= (f * e[k] + g * a(i, k));
         }
       }
+    } else {
+      e[i] = a(i, l);
+    }
+    d[i] = h;
   }
-  w.resize(n);
-  for (int i = 0; i < n; ++i) w[i] = A[(size_t)i * n + i];
+  d[0] = 0.0;
+  e[0] = 0.0;
+  for (int i = 0; i < n; ++i) {  // accumulate the transformations
+    const int l = i - 1;
+    if (d[i] != 0.0) {
+      for (int j = 0; j <= l; ++j) {
+        double g = 0.0;
+        for (int k = 0; k <= l; ++k) g += a(i, k) * a(k, j);
+        for (int k = 0; k <= l; ++k) a(k, j) -= g * a(k, i);
+      }
+    }
+    d[i] = a(i, i);
+    a(i, i) = 1.0;
+    for (int j = 0; j <= l; ++j) a(j, i) = a(i, j) = 0.0;
+  }
+  // QL with implicit shifts on the tridiagonal (d, e)
+  for (int i = 1; i < n; ++i) e[i - 1] = e[i];
+  if (n > 0) e[n - 1] = 0.0;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0, m;
+    do {
+      for (m = l; m < n - 1; ++m) {
+        const double dd = std::abs(d[m]) + std::abs(d[m + 1]);
+        if (std::abs(e[m]) <= std::numeric_limits<double>::epsilon() * dd) break;
+      }
+      if (m != l) {
+        SHAW_REQUIRE(iter++ < 60, SHAW_ERR_ARG, "eigensolver did not converge");
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = std::hypot(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        for (i = m - 1; i >= l; --i) {
+          double f = s * e[i];
+          const double b = c * e[i];
+          e[i + 1] = (r = std::hypot(f, g));
+          if (r == 0.0) {
+            d[i + 1] -= p;
+            e[m] = 0.0;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - p;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          d[i + 1] = g + (p = s * r);
+          g = c * r - b;
+          for (int k = 0; k < n; ++k) {
+            f = a(k, i + 1);
+            a(k, i + 1) = s * a(k, i) + c * f;
+            a(k, i) = c * a(k, i) - s * f;
+          }
+        }
+        if (r == 0.0 && i >= l) continue;
+        d[l] -= p;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+    } while (m != l);
+  }
+  w = d;
+  V = A;
 }
 
 // ------------------------------------------------------------ MFMA lane layouts
@@ -583,7 +656,7 @@ void podDevice(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* phi, do
     for (int pass = 0; pass < 24 && have < k; ++pass) {
       std::vector<double> G, w, V;
       gramTN(dS.p, (uint32_t)ns, dS.p, (uint32_t)ns, G);
-      jacobiEigen(G, (int)ns, w, V);
+      symmetricEigen(G, (int)ns, w, V);
       std::vector<int> order(ns);
       for (uint64_t i = 0; i < ns; ++i) order[i] = (int)i;
       std::sort(order.begin(), order.end(), [&](int x, int y) { return w[x] > w[y]; });
@@ -885,6 +958,17 @@ int shaw_pod_basis(const double* S, uint64_t n, uint64_t ns, uint64_t k, double*
     dS.alloc(n * ns);
     dS.upload(S, n * ns);
     podDevice(dS.p, n, ns, k, phi, sigma);
+  });
+}
+
+int shaw_debug_symeig(int n, const double* a, double* w, double* v) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(n > 0 && a && w && v, SHAW_ERR_ARG, "bad argument");
+    std::vector<double> A(a, a + (size_t)n * n), ww, VV;
+    symmetricEigen(A, n, ww, VV);
+    std::copy(ww.begin(), ww.end(), w);
+    std::copy(VV.begin(), VV.end(), v);
   });
 }
 
