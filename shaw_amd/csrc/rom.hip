@@ -10,9 +10,12 @@
 // torevise/run_rom_rank_two_forcing.hpp:149-188), for resident column slots:
 //   q_v += dt * (Ar_v q_s) ; q_v += (f dt) b ; y = C q_v (sample steps) ;
 //   q_s += dt * (Ar_s q_v)
-// A persistent kernel: one workgroup (4 waves) owns 16 column slots, keeps
-// their q_v, q_s in LDS in FP64 for the whole launch and runs many steps;
-// operators are pre-packed in MFMA operand order and stream from L2.
+// A persistent kernel: one workgroup (8 waves) owns 16 column slots, keeps
+// their q_v, q_s in LDS in FP64 for the whole launch and runs many steps.
+// Each wave owns up to 4 row tiles of 16 rows of each operator; operators are
+// pre-packed in MFMA operand order as 32-byte lane items and stream from L2
+// through a 4-deep register ring per row tile, so the loads of later k-steps
+// are in flight while the MFMAs of earlier ones run.
 //   FP64:   v_mfma_f64_16x16x4f64, FP64 accumulation.
 //   BF16x3: Ar = Ar_hi + Ar_lo and q = q_hi + q_lo in bf16; the product uses
 //           Ar_hi q_hi + Ar_hi q_lo + Ar_lo q_hi on v_mfma_f32_16x16x16bf16_1k
@@ -38,9 +41,10 @@ namespace {
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 typedef short shrt4 __attribute__((ext_vector_type(4)));
+typedef short shrt16 __attribute__((ext_vector_type(16)));
 
 constexpr int kTileCols = 16;  // column slots per workgroup
-constexpr int kWaves = 4;
+constexpr int kWaves = 8;      // waves per ROM workgroup
 
 // ------------------------------------------------------------ bf16 helpers
 __host__ __device__ inline unsigned short f32ToBf16(float x) {
@@ -328,18 +332,18 @@ __global__ void k_mfma_probe(int kind, const double* __restrict__ a, const doubl
 }
 
 // ------------------------------------------------------------ reduced leapfrog
+// Operators are packed for the MFMA A operand in 32-byte lane items, laid out
+// [rowTile][group][lane] so that one wave reads 2 KB contiguous per group:
+//   FP64:   group g of row tile t covers k-steps 4g..4g+3 of 16 x 16 x 4;
+//           lane l holds A[16t + l%16][16g + 4s + l/16], s = 0..3.
+//   BF16x3: group g covers k-steps 2g, 2g+1 of 16 x 16 x 16; lane l holds,
+//           for s = 0, 1, the hi then the lo bf16 parts of
+//           A[16t + l%16][32g + 16s + 4(l/16) + e], e = 0..3.
+// Padded sizes are multiples of kRing groups (64 for FP64, 128 for BF16x3).
 struct RomArgs {
-  // operators packed in A-operand order: f64: [rowTile][kStep][64 lanes];
-  // bf16: [rowTile][kStep][64 lanes][4] for hi and lo
-  const double* avF64;
-  const double* asF64;
-  const double* cF64;
-  const unsigned short* avHi;
-  const unsigned short* avLo;
-  const unsigned short* asHi;
-  const unsigned short* asLo;
-  const unsigned short* cHi;
-  const unsigned short* cLo;
+  const dbl4* av;       // Ar_v: rows Kvp, k-dim Ksp
+  const dbl4* as;       // Ar_s: rows Ksp, k-dim Kvp
+  const dbl4* c;        // C:    rows Rp,  k-dim Kvp
   double* qv;           // [slot][Kvp]
   double* qs;           // [slot][Ksp]
   const double* b;      // [slot][Kvp]
@@ -353,177 +357,218 @@ struct RomArgs {
   double dt;
 };
 
+constexpr int kRing = 4;  // operator groups kept in flight per row tile
+
 template <bool BF16>
-__device__ inline void halfStep(const RomArgs& a, int rowTiles, int kSteps, const double* Apk, const unsigned short* Ahi,
-                                const unsigned short* Alo, const double* Xs, int ldx, double* Ys, int ldy, int wave, int lane,
-                                const bool* active, const double* fdt, const double* bcol, int slot0) {
-  // Y[rows][cols] += dt * (A X), rows owned per wave in groups of 4 row tiles
-  for (int rt0 = wave; rt0 < rowTiles; rt0 += kWaves * 4) {
-    int nt = 0;
-    int rts[4];
-    for (int u = 0; u < 4; ++u) {
-      const int rt = rt0 + u * kWaves;
-      if (rt < rowTiles) rts[nt++] = rt;
+struct Acc;
+template <>
+struct Acc<false> { using T = dbl4; };
+template <>
+struct Acc<true> { using T = flt4; };
+
+// acc[t] = A[rows of tile rt[t]] X for the wave's TPW row tiles, X the 16
+// state columns in LDS ([K][16] FP64). The operator groups stream through a
+// register ring kRing deep; the B fragment of each k-step (and, for BF16x3,
+// its hi/lo split) is built once and used by all TPW tiles. kGroups is a
+// multiple of kRing (the padded sizes guarantee it), so the loop body has no
+// branches and the loads of the next kRing - 1 groups stay in flight while
+// a group's MFMAs run.
+template <bool BF16, int TPW>
+__device__ __forceinline__ void groupProducts(const dbl4 (&ag)[TPW], int g, const double* X, int lane,
+                                              typename Acc<BF16>::T (&acc)[TPW]) {
+  if constexpr (!BF16) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const double xb = X[((4 * g + s) * 4 + lane / 16) * kTileCols + lane % 16];
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) acc[t] = mfmaF64(ag[t][s], xb, acc[t]);
     }
-    if (!BF16) {
-      dbl4 acc[4];
-      for (int u = 0; u < 4; ++u) acc[u] = dbl4{0, 0, 0, 0};
-      for (int ks = 0; ks < kSteps; ++ks) {
-        const double xb = Xs[(ks * 4 + lane / 16) * ldx + lane % 16];
+  } else {
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (u < nt) acc[u] = mfmaF64(Apk[((uint64_t)rts[u] * kSteps + ks) * 64 + lane], xb, acc[u]);
+    for (int s = 0; s < 2; ++s) {
+      shrt4 xh, xl;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float x = (float)X[(16 * (2 * g + s) + 4 * (lane / 16) + e) * kTileCols + lane % 16];
+        const unsigned short h = f32ToBf16(x);
+        xh[e] = (short)h;
+        xl[e] = (short)f32ToBf16(x - bf16ToF32(h));
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (u < nt)
-          for (int r = 0; r < 4; ++r) {
-            const int row = rts[u] * 16 + rowF64(lane, r), col = lane % 16;
-            if (active[col]) {
-              double y = Ys[row * ldy + col] + a.dt * acc[u][r];
-              if (bcol) y = y + fdt[col] * bcol[(uint64_t)(slot0 + col) * a.Kvp + row];
-              Ys[row * ldy + col] = y;
-            }
-          }
-    } else {
-      flt4 acc[4];
-      for (int u = 0; u < 4; ++u) acc[u] = flt4{0, 0, 0, 0};
-      for (int ks = 0; ks < kSteps; ++ks) {
-        shrt4 xh, xl;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const float x = (float)Xs[(ks * 16 + 4 * (lane / 16) + e) * ldx + lane % 16];
-          const unsigned short h = f32ToBf16(x);
-          xh[e] = (short)h;
-          xl[e] = (short)f32ToBf16(x - bf16ToF32(h));
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (u < nt) {
-            const uint64_t o = (((uint64_t)rts[u] * kSteps + ks) * 64 + lane) * 4;
-            const shrt4 ah = *reinterpret_cast<const shrt4*>(Ahi + o);
-            const shrt4 al = *reinterpret_cast<const shrt4*>(Alo + o);
-            acc[u] = mfmaBf16(ah, xh, acc[u]);
-            acc[u] = mfmaBf16(ah, xl, acc[u]);
-            acc[u] = mfmaBf16(al, xh, acc[u]);
-          }
+      for (int t = 0; t < TPW; ++t) {
+        const shrt16 w = __builtin_bit_cast(shrt16, ag[t]);
+        const shrt4 ah = {w[8 * s + 0], w[8 * s + 1], w[8 * s + 2], w[8 * s + 3]};
+        const shrt4 al = {w[8 * s + 4], w[8 * s + 5], w[8 * s + 6], w[8 * s + 7]};
+        acc[t] = mfmaBf16(ah, xh, acc[t]);
+        acc[t] = mfmaBf16(ah, xl, acc[t]);
+        acc[t] = mfmaBf16(al, xh, acc[t]);
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (u < nt)
-          for (int r = 0; r < 4; ++r) {
-            const int row = rts[u] * 16 + rowBf16(lane, r), col = lane % 16;
-            if (active[col]) {
-              double y = Ys[row * ldy + col] + a.dt * (double)acc[u][r];
-              if (bcol) y = y + fdt[col] * bcol[(uint64_t)(slot0 + col) * a.Kvp + row];
-              Ys[row * ldy + col] = y;
-            }
-          }
     }
   }
 }
 
-template <bool BF16>
-__device__ inline void receivers(const RomArgs& a, const double* Qv, int wave, int lane, const bool* sample,
-                                 const uint64_t* sampleIdx, int slot0) {
-  const int rowTiles = a.Rp / 16, kSteps = BF16 ? a.Kvp / 16 : a.Kvp / 4;
-  for (int rt = wave; rt < rowTiles; rt += kWaves) {
-    double out[4];
-    if (!BF16) {
-      dbl4 acc = {0, 0, 0, 0};
-      for (int ks = 0; ks < kSteps; ++ks)
-        acc = mfmaF64(a.cF64[((uint64_t)rt * kSteps + ks) * 64 + lane], Qv[(ks * 4 + lane / 16) * kTileCols + lane % 16], acc);
-      for (int r = 0; r < 4; ++r) out[r] = acc[r];
-    } else {
-      flt4 acc = {0, 0, 0, 0};
-      for (int ks = 0; ks < kSteps; ++ks) {
-        shrt4 xh, xl;
-        for (int e = 0; e < 4; ++e) {
-          const float x = (float)Qv[(ks * 16 + 4 * (lane / 16) + e) * kTileCols + lane % 16];
-          const unsigned short h = f32ToBf16(x);
-          xh[e] = (short)h;
-          xl[e] = (short)f32ToBf16(x - bf16ToF32(h));
-        }
-        const uint64_t o = (((uint64_t)rt * kSteps + ks) * 64 + lane) * 4;
-        const shrt4 ah = *reinterpret_cast<const shrt4*>(a.cHi + o);
-        const shrt4 al = *reinterpret_cast<const shrt4*>(a.cLo + o);
-        acc = mfmaBf16(ah, xh, acc);
-        acc = mfmaBf16(ah, xl, acc);
-        acc = mfmaBf16(al, xh, acc);
-      }
-      for (int r = 0; r < 4; ++r) out[r] = (double)acc[r];
+template <bool BF16, int TPW>
+__device__ __forceinline__ void tileProducts(const dbl4* __restrict__ A, int kGroups, const int (&rt)[TPW],
+                                             const double* X, int lane, typename Acc<BF16>::T (&acc)[TPW]) {
+  dbl4 ring[kRing][TPW];
+  const dbl4* row[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    acc[t] = typename Acc<BF16>::T{0, 0, 0, 0};
+    row[t] = A + (size_t)rt[t] * kGroups * 64 + lane;
+  }
+#pragma unroll
+  for (int d = 0; d < kRing; ++d) {
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) ring[d][t] = row[t][d * 64];
+    __builtin_amdgcn_sched_barrier(0);  // issue order = consumption order
+  }
+  int g0 = 0;
+  for (; g0 + kRing < kGroups; g0 += kRing) {
+#pragma unroll
+    for (int d = 0; d < kRing; ++d) {
+      groupProducts<BF16, TPW>(ring[d], g0 + d, X, lane, acc);
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) ring[d][t] = row[t][(g0 + d + kRing) * 64];
+      // keep this group's refill here: the scheduler would otherwise sink all
+      // refills of the round to its end, shortening the prefetch distance
+      __builtin_amdgcn_sched_barrier(0);
     }
+  }
+#pragma unroll
+  for (int d = 0; d < kRing; ++d) groupProducts<BF16, TPW>(ring[d], g0 + d, X, lane, acc);
+}
+
+__device__ __forceinline__ int accRow(bool bf16, int lane, int r) { return bf16 ? rowBf16(lane, r) : rowF64(lane, r); }
+
+// Y[rows][cols] += dt * (A X) (+ (f dt) b for the velocity half), active
+// columns only. Row tiles rt0 + t * kWaves, t < TPW, belong to this wave.
+template <bool BF16, int TPW>
+__device__ __forceinline__ void halfStep(const RomArgs& a, const dbl4* A, int rowTiles, int kGroups, const double* Xs,
+                                         double* Ys, int wave, int lane, bool active, double fdt, const double* bcol,
+                                         int slot0) {
+  const int col = lane % 16;
+  for (int rt0 = wave; rt0 < rowTiles; rt0 += kWaves * TPW) {
+    int rt[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) rt[t] = rt0 + t * kWaves < rowTiles ? rt0 + t * kWaves : rt0;  // spare: recompute
+    double bv[TPW][4];
+    if (bcol)  // issued before the products, consumed after them
+#pragma unroll
+      for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bv[t][r] = bcol[(uint64_t)(slot0 + col) * a.Kvp + rt[t] * 16 + accRow(BF16, lane, r)];
+    typename Acc<BF16>::T acc[TPW];
+    tileProducts<BF16, TPW>(A, kGroups, rt, Xs, lane, acc);
+#pragma unroll
+    for (int t = 0; t < TPW; ++t)
+      if (rt0 + t * kWaves < rowTiles && active)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rt[t] * 16 + accRow(BF16, lane, r);
+          double y = Ys[row * kTileCols + col] + a.dt * (double)acc[t][r];
+          if (bcol) y = y + fdt * bv[t][r];
+          Ys[row * kTileCols + col] = y;
+        }
+  }
+}
+
+template <bool BF16, int TPW>
+__device__ __forceinline__ void receivers(const RomArgs& a, const double* Qv, int wave, int lane, bool sample,
+                                          uint32_t sampleIdx, int slot0) {
+  const int rowTiles = a.Rp / 16, kGroups = BF16 ? a.Kvp / 32 : a.Kvp / 16;
+  for (int rt0 = wave; rt0 < rowTiles; rt0 += kWaves * TPW) {
+    int rt[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) rt[t] = rt0 + t * kWaves < rowTiles ? rt0 + t * kWaves : rt0;
+    typename Acc<BF16>::T acc[TPW];
+    tileProducts<BF16, TPW>(a.c, kGroups, rt, Qv, lane, acc);
     const int col = lane % 16;
-    if (sample[col])
-      for (int r = 0; r < 4; ++r) {
-        const int row = rt * 16 + (BF16 ? rowBf16(lane, r) : rowF64(lane, r));
-        // per slot LayoutLeft [R x Tcap]
-        if (row < (int)a.R) a.seis[(uint64_t)(slot0 + col) * a.R * a.Tcap + (uint64_t)a.R * sampleIdx[col] + row] = out[r];
-      }
+#pragma unroll
+    for (int t = 0; t < TPW; ++t)
+      if (rt0 + t * kWaves < rowTiles && sample)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rt[t] * 16 + accRow(BF16, lane, r);
+          // per slot LayoutLeft [R x Tcap]
+          if (row < (int)a.R)
+            a.seis[(uint64_t)(slot0 + col) * a.R * a.Tcap + (uint64_t)a.R * sampleIdx + row] = (double)acc[t][r];
+        }
   }
 }
 
-template <bool BF16>
-__global__ void __launch_bounds__(256) k_rom_steps(RomArgs a) {
+// One workgroup (kWaves waves) owns kTileCols column slots and advances them
+// a.nsteps steps, states resident in LDS; each wave owns up to TPW row tiles
+// of each operator. Lane l works on column l % 16 in every product, so each
+// lane keeps its column's step counter, horizon, sampling period and next
+// forcing value in registers (the forcing of step s + 1 is loaded during
+// step s); the block-uniform decisions (stop, sample) come from values every
+// wave computes identically, so a step needs only the two barriers between
+// the half-steps.
+template <bool BF16, int TPW>
+__global__ void __launch_bounds__(kWaves * 64) k_rom_steps(RomArgs a) {
   extern __shared__ double sm[];
   double* Qv = sm;                                   // [Kvp][16]
   double* Qs = Qv + (size_t)a.Kvp * kTileCols;       // [Ksp][16]
-  __shared__ bool active[kTileCols], sample[kTileCols];
-  __shared__ double fdt[kTileCols];
-  __shared__ uint64_t sampleIdx[kTileCols];
-  __shared__ uint64_t s0[kTileCols];
+  __shared__ uint32_t remain[kTileCols];
   const int slot0 = blockIdx.x * kTileCols;
-  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64, col = lane % 16;
   for (int e = tid; e < (int)a.Kvp * kTileCols; e += blockDim.x) {
-    const int row = e / kTileCols, col = e % kTileCols;
-    Qv[e] = a.qv[(uint64_t)(slot0 + col) * a.Kvp + row];
+    const int row = e / kTileCols, c = e % kTileCols;
+    Qv[e] = a.qv[(uint64_t)(slot0 + c) * a.Kvp + row];
   }
   for (int e = tid; e < (int)a.Ksp * kTileCols; e += blockDim.x) {
-    const int row = e / kTileCols, col = e % kTileCols;
-    Qs[e] = a.qs[(uint64_t)(slot0 + col) * a.Ksp + row];
+    const int row = e / kTileCols, c = e % kTileCols;
+    Qs[e] = a.qs[(uint64_t)(slot0 + c) * a.Ksp + row];
   }
-  if (tid < kTileCols) s0[tid] = a.stepCol[slot0 + tid];
+  // this lane's column (step counts fit 32 bits: the host checks horizons)
+  const uint32_t s0 = (uint32_t)a.stepCol[slot0 + col];
+  const uint32_t hz = (uint32_t)a.horizon[slot0 + col];
+  const uint32_t ev = (uint32_t)a.every[slot0 + col];
+  const double* fcol = a.ftab + (uint64_t)(slot0 + col) * a.maxSteps;
+  if (tid < kTileCols) remain[tid] = hz > s0 ? hz - s0 : 0;
   __syncthreads();
+  uint32_t steps = 0;  // block-uniform: steps until every column is done
+  for (int c = 0; c < kTileCols; ++c) steps = remain[c] > steps ? remain[c] : steps;
+  if ((uint64_t)steps > a.nsteps) steps = (uint32_t)a.nsteps;
   const int kvTiles = a.Kvp / 16, ksTiles = a.Ksp / 16;
-  const int kStepsS = BF16 ? a.Ksp / 16 : a.Ksp / 4, kStepsV = BF16 ? a.Kvp / 16 : a.Kvp / 4;
-  bool anyLeft = true;
-  for (uint64_t it = 0; it < a.nsteps && anyLeft; ++it) {
-    if (tid < kTileCols) {
-      const uint64_t s = s0[tid] + it;  // steps already taken by this column
-      const bool act = s < a.horizon[slot0 + tid];
-      active[tid] = act;
-      fdt[tid] = act ? a.ftab[(uint64_t)(slot0 + tid) * a.maxSteps + s] * a.dt : 0.0;
-      const uint64_t ev = a.every[slot0 + tid];
-      sample[tid] = act && ev > 0 && (s + 1) % ev == 0;
-      sampleIdx[tid] = ev > 0 ? (s + 1) / ev - 1 : 0;
-    }
-    __syncthreads();
-    bool any = false, anySample = false;
-    for (int c = 0; c < kTileCols; ++c) { any |= active[c]; anySample |= sample[c]; }
-    anyLeft = any;
-    if (!any) break;
+  const int kGroupsS = BF16 ? a.Ksp / 32 : a.Ksp / 16, kGroupsV = BF16 ? a.Kvp / 32 : a.Kvp / 16;
+  double fcur = s0 < hz ? fcol[s0] : 0.0;
+  uint32_t phase = ev ? (s0 + 1) % ev : 0;     // (s + 1) mod every
+  uint32_t nextIdx = ev ? (s0 + 1) / ev : 0;   // index of the next sample
+  for (uint32_t it = 0; it < steps; ++it) {
+    const uint32_t s = s0 + it;  // steps already taken by this column
+    const bool act = s < hz;
+    const double fnext = act && s + 1 < hz ? fcol[s + 1] : 0.0;  // consumed next step
+    const double fdt = act ? fcur * a.dt : 0.0;
+    const bool smp = act && ev > 0 && phase == 0;
+    // lanes 0..15 of every wave hold columns 0..15: the same answer in all waves
+    const bool anySample = (__ballot(smp) & 0xffffull) != 0;
     // velocity: q_v += dt Ar_v q_s ; q_v += (f dt) b
-    halfStep<BF16>(a, kvTiles, kStepsS, a.avF64, a.avHi, a.avLo, Qs, kTileCols, Qv, kTileCols, wave, lane, active, fdt, a.b,
-                   slot0);
+    halfStep<BF16, TPW>(a, a.av, kvTiles, kGroupsS, Qs, Qv, wave, lane, act, fdt, a.b, slot0);
     __syncthreads();
-    if (anySample) receivers<BF16>(a, Qv, wave, lane, sample, sampleIdx, slot0);
+    if (anySample) receivers<BF16, TPW>(a, Qv, wave, lane, smp, nextIdx - 1, slot0);
     // stress: q_s += dt Ar_s q_v
-    halfStep<BF16>(a, ksTiles, kStepsV, a.asF64, a.asHi, a.asLo, Qv, kTileCols, Qs, kTileCols, wave, lane, active, fdt,
-                   nullptr, slot0);
+    halfStep<BF16, TPW>(a, a.as, ksTiles, kGroupsV, Qv, Qs, wave, lane, act, 0.0, nullptr, slot0);
     __syncthreads();
+    fcur = fnext;
+    if (ev && act) {
+      phase = phase + 1 == ev ? 0 : phase + 1;
+      if (phase == 0) ++nextIdx;
+    }
   }
   for (int e = tid; e < (int)a.Kvp * kTileCols; e += blockDim.x) {
-    const int row = e / kTileCols, col = e % kTileCols;
-    a.qv[(uint64_t)(slot0 + col) * a.Kvp + row] = Qv[e];
+    const int row = e / kTileCols, c = e % kTileCols;
+    a.qv[(uint64_t)(slot0 + c) * a.Kvp + row] = Qv[e];
   }
   for (int e = tid; e < (int)a.Ksp * kTileCols; e += blockDim.x) {
-    const int row = e / kTileCols, col = e % kTileCols;
-    a.qs[(uint64_t)(slot0 + col) * a.Ksp + row] = Qs[e];
+    const int row = e / kTileCols, c = e % kTileCols;
+    a.qs[(uint64_t)(slot0 + c) * a.Ksp + row] = Qs[e];
   }
   if (tid < kTileCols) {
     const uint64_t h = a.horizon[slot0 + tid];
-    const uint64_t s = s0[tid] + a.nsteps;
-    a.stepCol[slot0 + tid] = s < h ? s : (s0[tid] < h ? h : s0[tid]);
+    const uint64_t st = a.stepCol[slot0 + tid], s = st + a.nsteps;
+    a.stepCol[slot0 + tid] = s < h ? s : (st < h ? h : st);
   }
 }
 
@@ -547,35 +592,52 @@ struct DevBuf {
 
 uint32_t roundUp(uint64_t x, uint32_t m) { return (uint32_t)((x + m - 1) / m * m); }
 
-// pack a row-major [rows x cols] operator (zero padded to rowsP x colsP) in
-// MFMA A-operand order
-void packF64(const double* A, uint64_t rows, uint64_t cols, uint32_t rowsP, uint32_t colsP, std::vector<double>& out) {
-  const uint32_t rt = rowsP / 16, ks = colsP / 4;
-  out.assign((size_t)rt * ks * 64, 0.0);
+// pack a row-major [rows x cols] operator (zero padded to rowsP x colsP,
+// whole groups) into the 32-byte lane items described above; returns the
+// packed bytes
+std::vector<double> packF64(const double* A, uint64_t rows, uint64_t cols, uint32_t rowsP, uint32_t colsP) {
+  const uint32_t rt = rowsP / 16, G = colsP / 16;
+  std::vector<double> out((size_t)rt * G * 64 * 4, 0.0);
   for (uint32_t t = 0; t < rt; ++t)
-    for (uint32_t k = 0; k < ks; ++k)
-      for (int l = 0; l < 64; ++l) {
-        const uint64_t r = t * 16 + l % 16, c = k * 4 + l / 16;
-        out[((size_t)t * ks + k) * 64 + l] = (r < rows && c < cols) ? A[r * cols + c] : 0.0;
-      }
-}
-void packBf16(const double* A, uint64_t rows, uint64_t cols, uint32_t rowsP, uint32_t colsP,
-              std::vector<unsigned short>& hi, std::vector<unsigned short>& lo) {
-  const uint32_t rt = rowsP / 16, ks = colsP / 16;
-  hi.assign((size_t)rt * ks * 64 * 4, 0);
-  lo.assign(hi.size(), 0);
-  for (uint32_t t = 0; t < rt; ++t)
-    for (uint32_t k = 0; k < ks; ++k)
+    for (uint32_t g = 0; g < G; ++g)
       for (int l = 0; l < 64; ++l)
-        for (int e = 0; e < 4; ++e) {
-          const uint64_t r = t * 16 + l % 16, c = k * 16 + 4 * (l / 16) + e;
-          const double v = (r < rows && c < cols) ? A[r * cols + c] : 0.0;
-          const float vf = (float)v;
-          const unsigned short h = f32ToBf16(vf);
-          const size_t o = (((size_t)t * ks + k) * 64 + l) * 4 + e;
-          hi[o] = h;
-          lo[o] = f32ToBf16((float)(v - (double)bf16ToF32(h)));
+        for (int s = 0; s < 4; ++s) {
+          const uint64_t r = t * 16 + l % 16, c = g * 16 + 4 * s + l / 16;
+          out[(((size_t)t * G + g) * 64 + l) * 4 + s] = (r < rows && c < cols) ? A[r * cols + c] : 0.0;
         }
+  return out;
+}
+std::vector<double> packBf16(const double* A, uint64_t rows, uint64_t cols, uint32_t rowsP, uint32_t colsP) {
+  const uint32_t rt = rowsP / 16, G = colsP / 32;
+  std::vector<double> out((size_t)rt * G * 64 * 4, 0.0);
+  unsigned short* o16 = reinterpret_cast<unsigned short*>(out.data());
+  for (uint32_t t = 0; t < rt; ++t)
+    for (uint32_t g = 0; g < G; ++g)
+      for (int l = 0; l < 64; ++l)
+        for (int s = 0; s < 2; ++s)
+          for (int e = 0; e < 4; ++e) {
+            const uint64_t r = t * 16 + l % 16, c = g * 32 + 16 * s + 4 * (l / 16) + e;
+            const double v = (r < rows && c < cols) ? A[r * cols + c] : 0.0;
+            const unsigned short h = f32ToBf16((float)v);
+            const size_t o = (((size_t)t * G + g) * 64 + l) * 16 + 8 * s + e;
+            o16[o] = h;
+            o16[o + 4] = f32ToBf16((float)(v - (double)bf16ToF32(h)));
+          }
+  return out;
+}
+
+template <bool BF16>
+void launchRomSteps(int tpw, dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a) {
+  auto go = [&](auto kernel) {
+    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kernel, grid, dim3(kWaves * 64), lds, stream, a);
+  };
+  switch (tpw) {
+    case 1: go(k_rom_steps<BF16, 1>); break;
+    case 2: go(k_rom_steps<BF16, 2>); break;
+    case 3: go(k_rom_steps<BF16, 3>); break;
+    default: go(k_rom_steps<BF16, 4>); break;
+  }
 }
 
 }  // namespace
@@ -703,8 +765,8 @@ struct shaw_rom {
   int precision = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  shaw::DevBuf<double> avF64, asF64, cF64, qv, qs, b, ftab, seis;
-  shaw::DevBuf<unsigned short> avHi, avLo, asHi, asLo, cHi, cLo;
+  int tpw = 1;  // row tiles per wave
+  shaw::DevBuf<double> av, as, c, qv, qs, b, ftab, seis;  // av/as/c: packed operator items
   shaw::DevBuf<uint64_t> stepCol, horizon, every;
   std::vector<uint64_t> hStep, hHorizon, hEvery;
   std::vector<uint8_t> used;
@@ -717,21 +779,17 @@ struct shaw_rom {
   void launch(uint64_t nsteps) {
     using namespace shaw;
     RomArgs a{};
-    a.avF64 = avF64.p; a.asF64 = asF64.p; a.cF64 = cF64.p;
-    a.avHi = avHi.p; a.avLo = avLo.p; a.asHi = asHi.p; a.asLo = asLo.p; a.cHi = cHi.p; a.cLo = cLo.p;
+    a.av = reinterpret_cast<const dbl4*>(av.p);
+    a.as = reinterpret_cast<const dbl4*>(as.p);
+    a.c = reinterpret_cast<const dbl4*>(c.p);
     a.qv = qv.p; a.qs = qs.p; a.b = b.p; a.ftab = ftab.p;
     a.stepCol = stepCol.p; a.horizon = horizon.p; a.every = every.p; a.seis = seis.p;
     a.Kvp = kvp; a.Ksp = ksp; a.Rp = rp; a.R = (uint32_t)R;
     a.maxSteps = maxSteps; a.Tcap = maxSamples; a.nsteps = nsteps; a.dt = dt;
-    const unsigned blocks = (unsigned)(maxCols / kTileCols);
+    const dim3 grid((unsigned)(maxCols / kTileCols));
     const size_t lds = (size_t)(kvp + ksp) * kTileCols * sizeof(double);
-    if (precision == SHAW_ROM_FP64) {
-      (void)hipFuncSetAttribute((const void*)k_rom_steps<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL(k_rom_steps<false>, dim3(blocks), dim3(256), lds, stream, a);
-    } else {
-      (void)hipFuncSetAttribute((const void*)k_rom_steps<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL(k_rom_steps<true>, dim3(blocks), dim3(256), lds, stream, a);
-    }
+    if (precision == SHAW_ROM_FP64) launchRomSteps<false>(tpw, grid, lds, stream, a);
+    else launchRomSteps<true>(tpw, grid, lds, stream, a);
     SHAW_HIP(hipGetLastError());
     for (uint64_t c = 0; c < maxCols; ++c) {
       const uint64_t s = hStep[c] + nsteps;
@@ -785,7 +843,7 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
   using namespace shaw;
   return guarded([&] {
     SHAW_REQUIRE(d && out && d->ar_v && d->ar_s, SHAW_ERR_ARG, "null argument");
-    SHAW_REQUIRE(d->kv > 0 && d->ks > 0 && d->kv <= 2048 && d->ks <= 2048, SHAW_ERR_ARG, "ROM size must be in 1..2048");
+    SHAW_REQUIRE(d->kv > 0 && d->ks > 0 && d->kv <= 512 && d->ks <= 512, SHAW_ERR_ARG, "ROM size must be in 1..512 modes per dof");
     SHAW_REQUIRE(d->precision == SHAW_ROM_FP64 || d->precision == SHAW_ROM_BF16X3, SHAW_ERR_ARG, "unknown ROM precision");
     SHAW_REQUIRE(d->max_cols > 0 && d->dt > 0, SHAW_ERR_ARG, "max_cols and dt must be > 0");
     SHAW_REQUIRE(d->r == 0 || d->c, SHAW_ERR_ARG, "receiver rows missing");
@@ -794,11 +852,16 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
     SHAW_REQUIRE(ndev > 0, SHAW_ERR_HIP, "no HIP device available");
     auto h = std::make_unique<shaw_rom>();
     h->kv = d->kv; h->ks = d->ks; h->R = d->r; h->dt = d->dt; h->precision = d->precision;
-    h->kvp = roundUp(d->kv, 16);
-    h->ksp = roundUp(d->ks, 16);
+    // padded sizes: whole ring rounds of operator groups (16 k per FP64
+    // group, 32 per BF16x3 group)
+    const uint32_t kq = (d->precision == SHAW_ROM_FP64 ? 16 : 32) * kRing;
+    h->kvp = roundUp(d->kv, kq);
+    h->ksp = roundUp(d->ks, kq);
     h->rp = roundUp(std::max<uint64_t>(d->r, 1), 16);
     h->maxCols = roundUp(d->max_cols, kTileCols);
     h->maxSamples = d->max_samples;
+    SHAW_REQUIRE(h->kvp <= 512 && h->ksp <= 512, SHAW_ERR_ARG, "ROM size must be at most 512 modes per dof");
+    h->tpw = (int)((std::max(h->kvp, h->ksp) / 16 + kWaves - 1) / kWaves);
     const size_t lds = (size_t)(h->kvp + h->ksp) * kTileCols * sizeof(double);
     SHAW_REQUIRE(lds <= 160 * 1024, SHAW_ERR_ARG, "ROM size too large for the LDS-resident state tile");
     SHAW_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -806,20 +869,16 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
     SHAW_HIP(hipEventCreate(&h->ev1));
     std::vector<double> cPad((size_t)h->rp * d->kv, 0.0);
     if (d->r) std::memcpy(cPad.data(), d->c, d->r * d->kv * sizeof(double));
-    if (d->precision == SHAW_ROM_FP64) {
-      std::vector<double> p;
-      packF64(d->ar_v, d->kv, d->ks, h->kvp, h->ksp, p); h->avF64.alloc(p.size()); h->avF64.upload(p.data(), p.size());
-      packF64(d->ar_s, d->ks, d->kv, h->ksp, h->kvp, p); h->asF64.alloc(p.size()); h->asF64.upload(p.data(), p.size());
-      packF64(cPad.data(), h->rp, d->kv, h->rp, h->kvp, p); h->cF64.alloc(p.size()); h->cF64.upload(p.data(), p.size());
-    } else {
-      std::vector<unsigned short> hi, lo;
-      packBf16(d->ar_v, d->kv, d->ks, h->kvp, h->ksp, hi, lo);
-      h->avHi.alloc(hi.size()); h->avHi.upload(hi.data(), hi.size()); h->avLo.alloc(lo.size()); h->avLo.upload(lo.data(), lo.size());
-      packBf16(d->ar_s, d->ks, d->kv, h->ksp, h->kvp, hi, lo);
-      h->asHi.alloc(hi.size()); h->asHi.upload(hi.data(), hi.size()); h->asLo.alloc(lo.size()); h->asLo.upload(lo.data(), lo.size());
-      packBf16(cPad.data(), h->rp, d->kv, h->rp, h->kvp, hi, lo);
-      h->cHi.alloc(hi.size()); h->cHi.upload(hi.data(), hi.size()); h->cLo.alloc(lo.size()); h->cLo.upload(lo.data(), lo.size());
-    }
+    auto pack = [&](const double* A, uint64_t rows, uint64_t cols, uint32_t rowsP, uint32_t colsP,
+                    shaw::DevBuf<double>& dst) {
+      const std::vector<double> p = d->precision == SHAW_ROM_FP64 ? packF64(A, rows, cols, rowsP, colsP)
+                                                                  : packBf16(A, rows, cols, rowsP, colsP);
+      dst.alloc(p.size());
+      dst.upload(p.data(), p.size());
+    };
+    pack(d->ar_v, d->kv, d->ks, h->kvp, h->ksp, h->av);
+    pack(d->ar_s, d->ks, d->kv, h->ksp, h->kvp, h->as);
+    pack(cPad.data(), h->rp, d->kv, h->rp, h->kvp, h->c);
     const uint64_t C = h->maxCols;
     h->qv.alloc(C * h->kvp); h->qs.alloc(C * h->ksp); h->b.alloc(C * h->kvp);
     SHAW_HIP(hipMemset(h->qv.p, 0, h->qv.n * 8)); SHAW_HIP(hipMemset(h->qs.p, 0, h->qs.n * 8));
@@ -843,6 +902,7 @@ int shaw_rom_admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, c
     SHAW_REQUIRE(free.size() == n, SHAW_ERR_MEMORY, "not enough free ROM column slots");
     for (uint64_t j = 0; j < n; ++j) {
       SHAW_REQUIRE(horizons[j] <= nsteps, SHAW_ERR_ARG, "horizon longer than the forcing table");
+      SHAW_REQUIRE(horizons[j] < (1ull << 31), SHAW_ERR_ARG, "horizon must be below 2^31 steps");
       if (h->R) SHAW_REQUIRE(horizons[j] / every <= h->maxSamples, SHAW_ERR_ARG, "horizon exceeds the seismogram capacity");
     }
     const uint64_t need = std::max<uint64_t>(h->maxSteps, nsteps);

@@ -115,6 +115,35 @@ def test_rom_fp64_vs_oracle_and_bf16x3(pod21, K):
     assert err <= 1e-4
 
 
+@pytest.mark.parametrize("kv,ks", [(128, 128), (256, 256), (512, 512), (96, 200), (33, 17)])
+def test_rom_sizes_vs_oracle(kv, ks):
+    """Sizes that exercise 1..4 row tiles per wave, padding of the operator
+    groups, kv != ks and receivers spanning several tiles: FP64 within 1e-11
+    of the oracle, BF16x3 within 1e-4 (error printed)."""
+    say(f"[gpu] ROM sizes kv={kv} ks={ks}")
+    rng = np.random.default_rng(kv * 1000 + ks)
+    A = rng.standard_normal((kv, ks)) * (0.5 / np.sqrt(max(kv, ks)))
+    arv, ars = A, -A.T.copy()  # skew pair: the reduced leapfrog stays bounded
+    M, n, dt, every = 37, 120, 0.5, 7
+    B = rng.standard_normal((kv, M))
+    ftab = rng.standard_normal((n, M))
+    C = rng.standard_normal((19, kv))
+    ref = oracle.rom_run(arv, ars, B, ftab, dt, n, C, every)
+    got = shaw_amd.rom_run(arv, ars, B, ftab, dt, n, C, every, shaw_amd.ROM_FP64)
+    for key in ("qv", "qs", "seismogram"):
+        assert rel_l2(got[key], ref[key]) <= 1e-11, key
+    bf = shaw_amd.rom_run(arv, ars, B, ftab, dt, n, C, every, shaw_amd.ROM_BF16X3)
+    err = max(rel_l2(bf["seismogram"][m], ref["seismogram"][m]) for m in range(M))
+    say(f"[gpu] ROM kv={kv} ks={ks}: BF16x3 seismogram max rel L2 error vs FP64 = {err:.3e}")
+    assert err <= 1e-4
+
+
+def test_rom_size_limit():
+    A = np.zeros((513, 8))
+    with pytest.raises(shaw_amd.ShawError, match="512"):
+        shaw_amd.Rom(A, A.T.copy(), 0.5, shaw_amd.ROM_FP64, None, max_cols=16)
+
+
 def test_full_basis_rom_equals_fom():
     """Phi = I on the 4 x 5 mesh: the reduced leapfrog is the FOM leapfrog."""
     mesh = os.path.join(MESHES, "fullMesh4x5")
