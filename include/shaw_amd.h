@@ -170,6 +170,12 @@ int shaw_fom_config(shaw_fom_t h, int* kernel, int* cols_per_thread, int* strip,
  * all samples: LayoutLeft [n x (T * ncols)]) without a host round trip, as
  * shaw_pod_basis; the snapshot buffer is consumed. FP64 FOM only.         */
 int shaw_fom_pod(shaw_fom_t h, int dof, uint64_t k, double* phi, double* sigma);
+/* POD in the inner product diag(w) (w [n] > 0): phi are w-orthonormal modes,
+ * the left singular vectors of diag(sqrt w) S unscaled. No counterpart in the
+ * reference (its POD, computeThinSVD, is Euclidean); used with
+ * shaw_rom_project_energy and shaw_app energy weights.                     */
+int shaw_fom_pod_weighted(shaw_fom_t h, int dof, uint64_t k, const double* w, double* phi,
+                          double* sigma);
 void shaw_fom_destroy(shaw_fom_t h);
 
 /* ------------------------------------------------------------------ ROM --
@@ -180,17 +186,31 @@ typedef struct shaw_rom* shaw_rom_t;
 
 #define SHAW_ROM_FP64 0
 #define SHAW_ROM_BF16X3 1
+/* BF16x4: the operator in three bf16 parts (hi, lo, residual), products
+ * hi*q_hi + hi*q_lo + lo*q_hi + residual*q_hi with FP32 accumulation; no
+ * counterpart in the reference (its ROM is FP64).                          */
+#define SHAW_ROM_BF16X4 2
 
 /* Ar_v = Phi_v^T (J_vp Phi_s) [kv x ks], Ar_s = Phi_s^T (J_sp Phi_v) [ks x kv],
  * row-major outputs; phi_v LayoutLeft [nvp x kv], phi_s LayoutLeft [nsp x ks]
  * (the lsv_* file layout).                                                */
 int shaw_rom_project(shaw_app_t app, const double* phi_v, uint64_t kv, const double* phi_s,
                      uint64_t ks, double* ar_v, double* ar_s);
+/* Energy-consistent projection (no counterpart in the reference, whose
+ * Galerkin operators rom_compute_jacobians.hpp:58-109 are Euclidean):
+ * Ar_v = Phi_v^T diag(mv) J_vp Phi_s and Ar_s = -Ar_v^T, for bases that are
+ * orthonormal in the FOM energy weights (mv on velocity points; stress bases
+ * orthonormal in the stress weights). The FOM satisfies
+ * Mv J_vp = -(Ms J_sp)^T on every live point, so Ar_s is the Galerkin
+ * operator itself; stated exactly, the reduced leapfrog conserves a reduced
+ * energy and operator rounding (BF16x3) cannot excite growing modes.       */
+int shaw_rom_project_energy(shaw_app_t app, const double* phi_v, uint64_t kv, const double* phi_s,
+                            uint64_t ks, const double* mv, double* ar_v, double* ar_s);
 
 typedef struct {
   uint64_t kv, ks;
   double dt;
-  int precision;         /* SHAW_ROM_FP64 | SHAW_ROM_BF16X3 */
+  int precision;         /* SHAW_ROM_FP64 | SHAW_ROM_BF16X3 | SHAW_ROM_BF16X4 */
   const double* ar_v;    /* [kv][ks] */
   const double* ar_s;    /* [ks][kv] */
   uint64_t r;            /* receivers */

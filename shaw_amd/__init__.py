@@ -27,7 +27,7 @@ EXE_PATH = os.path.join(HERE, "shawExe")
 
 FP64, FP32 = 0, 1
 KERNEL_AUTO, KERNEL_SPLIT, KERNEL_FUSED = 0, 1, 2
-ROM_FP64, ROM_BF16X3 = 0, 1
+ROM_FP64, ROM_BF16X3, ROM_BF16X4 = 0, 1, 2
 SIGNAL_KINDS = {"ricker": 0, "sinusoid": 1, "gaussDer": 2}
 
 
@@ -105,9 +105,11 @@ def lib():
         "shaw_fom_config": (i32, [vp, P(i32), P(i32), P(i32), P(i32)]),
         "shaw_fom_destroy": (None, [vp]),
         "shaw_fom_pod": (i32, [vp, i32, u64, vp, vp]),
+        "shaw_fom_pod_weighted": (i32, [vp, i32, u64, vp, vp, vp]),
     }
     optional = {
         "shaw_rom_project": (i32, [vp, vp, u64, vp, u64, vp, vp]),
+        "shaw_rom_project_energy": (i32, [vp, vp, u64, vp, u64, vp, vp, vp]),
         "shaw_rom_create": (i32, [P(_RomDesc), P(vp)]),
         "shaw_rom_admit": (i32, [vp, u64, vp, u64, vp, vp, u64, vp]),
         "shaw_rom_advance": (i32, [vp, u64]),
@@ -356,12 +358,20 @@ class Fom:
         _check(lib().shaw_fom_bytes_per_step(self._h, C.byref(k), C.byref(m)))
         return k.value, m.value
 
-    def pod(self, dof: str, k: int, nsnap: int):
-        """(Phi [n x k], sigma) of this run's snapshots of `dof` (consumes them)."""
+    def pod(self, dof: str, k: int, nsnap: int, weights=None):
+        """(Phi [n x k], sigma) of this run's snapshots of `dof` (consumes them);
+        with `weights` [n] the POD is in the inner product diag(weights)."""
         n = self.app.nVp if dof == "vp" else self.app.nSp
         phi = np.zeros((k, n))
         sig = np.zeros(nsnap)
-        _check(lib().shaw_fom_pod(self._h, 0 if dof == "vp" else 1, k, _p(phi), _p(sig)))
+        d = 0 if dof == "vp" else 1
+        if weights is None:
+            _check(lib().shaw_fom_pod(self._h, d, k, _p(phi), _p(sig)))
+        else:
+            w = _f64(weights).ravel()
+            if w.size != n:
+                raise ShawError(f"POD weights: expected {n} values, got {w.size}")
+            _check(lib().shaw_fom_pod_weighted(self._h, d, k, _p(w), _p(phi), _p(sig)))
         return phi.T, sig
 
     def config(self):
@@ -413,6 +423,22 @@ def rom_project(app: App, phi_v: np.ndarray, phi_s: np.ndarray):
     arv = np.zeros((kv, ks))
     ars = np.zeros((ks, kv))
     _check(lib().shaw_rom_project(app.handle, _p(pv), kv, _p(ps), ks, _p(arv), _p(ars)))
+    return arv, ars
+
+
+def rom_project_energy(app: App, phi_v: np.ndarray, phi_s: np.ndarray, mv: np.ndarray):
+    """Energy-consistent operators Ar_v = Phi_v^T diag(mv) J_vp Phi_s and
+    Ar_s = -Ar_v^T (FP64, on the GPU), for bases orthonormal in the FOM energy
+    weights (an addition to the reference's Euclidean projection)."""
+    kv, ks = phi_v.shape[1], phi_s.shape[1]
+    pv = np.asfortranarray(phi_v, dtype=np.float64)
+    ps = np.asfortranarray(phi_s, dtype=np.float64)
+    w = _f64(mv).ravel()
+    if w.size != app.nVp:
+        raise ShawError(f"energy weights: expected {app.nVp} values, got {w.size}")
+    arv = np.zeros((kv, ks))
+    ars = np.zeros((ks, kv))
+    _check(lib().shaw_rom_project_energy(app.handle, _p(pv), kv, _p(ps), ks, _p(w), _p(arv), _p(ars)))
     return arv, ars
 
 

@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -73,6 +74,7 @@ struct StepArgs {
   const T* srcRhoInv;   // [nb*MC]
   const double* frow;   // forcing values of this step, [M]
   const int32_t* recv;  // [nVp] receiver index or -1; nullptr: no sample this step
+  int64_t recvRow;      // the one grid row holding every receiver, or -1
   double* seis;         // LayoutLeft [R x Tcap x M]
   uint64_t sample, R, Tcap;
   uint32_t nr, nth, M, MC;
@@ -86,20 +88,34 @@ __device__ __forceinline__ uint64_t stpIdx(uint64_t i, uint64_t j, uint64_t rs, 
   return j + 1 < nr ? j * rs + 2 * i + 1 : (nr - 1) * rs + i;
 }
 
-// forcing and receiver epilogue of the velocity update for CPT columns
+// Forcing of the velocity update for CPT columns. The per-column source gid
+// and increment are loop invariants, loaded once per thread: the increment
+// is (dt * rhoInv) * f for the rank-1 form `v = v + dt*rhoInv*f`
+// (KokkosBlas::mult) and (rhoInv * f) * dt for the rank-2 form
+// `v += rhoInv*f*dt` (AddForcingRank2), the same operations in the same
+// order, so v + increment is bit-identical to either form.
 template <class T, int CPT>
-__device__ __forceinline__ void velocityEpilogue(const StepArgs<T>& a, T (&vn)[CPT], uint64_t p,
-                                                 uint32_t colBase) {
+struct ColumnForcing {
+  int64_t gid[CPT];
+  T inc[CPT];
+  __device__ __forceinline__ void load(const StepArgs<T>& a, uint32_t colBase) {
 #pragma unroll
-  for (int k = 0; k < CPT; ++k) {
-    const uint32_t col = colBase + k;
-    if (col < a.M && a.src[col] == (int64_t)p) {
-      const T f = (T)a.frow[col];
-      if (a.rank1) vn[k] = vn[k] + a.dt * a.srcRhoInv[col] * f;   // KokkosBlas::mult
-      else vn[k] += a.srcRhoInv[col] * f * a.dt;                  // AddForcingRank2
+    for (int k = 0; k < CPT; ++k) {
+      const uint32_t col = colBase + k;
+      gid[k] = col < a.M ? a.src[col] : -1;
+      inc[k] = T(0);
+      if (gid[k] >= 0) {
+        const T f = (T)a.frow[col];
+        inc[k] = a.rank1 ? a.dt * a.srcRhoInv[col] * f : a.srcRhoInv[col] * f * a.dt;
+      }
     }
   }
-}
+  __device__ __forceinline__ void apply(T (&vn)[CPT], uint64_t p) const {
+#pragma unroll
+    for (int k = 0; k < CPT; ++k)
+      if (gid[k] == (int64_t)p) vn[k] = vn[k] + inc[k];
+  }
+};
 
 template <class T, int CPT>
 __device__ __forceinline__ void recordReceivers(const StepArgs<T>& a, const T (&vn)[CPT], uint64_t p,
@@ -145,7 +161,9 @@ __global__ void __launch_bounds__(256) k_velocity(StepArgs<T> a) {
     sum = sum + c[3] * xE[k];
     vn[k] = vo[k] + a.dt * sum;
   }
-  velocityEpilogue<T, CPT>(a, vn, p, cb * a.MC + m0);
+  ColumnForcing<T, CPT> frc;
+  frc.load(a, cb * a.MC + m0);
+  frc.apply(vn, p);
   store(v, vn);
   recordReceivers<T, CPT>(a, vn, p, cb * a.MC + m0);
 }
@@ -182,6 +200,16 @@ __global__ void __launch_bounds__(256) k_stress(StepArgs<T> a) {
 }
 
 // ------------------------------------------------------------ fused step
+// Old values of one grid row r for one thread, loaded one row ahead: the
+// srp (north of r) and stp (east) stress, the velocity, the J_vp row, the
+// west stp (used by the strip's first point only) and the J_sp rows of the
+// stress points of row r - 1, which are updated while row r is processed.
+template <class T, int CPT>
+struct RowIn {
+  T N[CPT], E[CPT], vo[CPT], W[CPT];
+  T c[4], bS[2], bT[2];
+};
+
 template <class T, int CPT>
 __global__ void __launch_bounds__(512) k_fused(StepArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
@@ -210,84 +238,92 @@ __global__ void __launch_bounds__(512) k_fused(StepArgs<T> a) {
   const T* sIn = a.sIn + (uint64_t)cb * nSp * a.MC + m0;
   T* sOut = a.sOut + (uint64_t)cb * nSp * a.MC + m0;
 
+  // Loads are issued for every lane and every row with clamped indices (a
+  // lane past the grid edge, a row past the segment or the grid): no branch
+  // separates a load from its use, so the loads of row r + 1 stay in flight
+  // while row r is computed.
+  const uint64_t ic = active ? i : nth - 1;
+  const uint64_t iE = ic < nth - 2 ? ic : nth - 2;        // east stp column
+  const uint64_t iW = pl == 0 ? (ic > 0 ? ic - 1 : 0) : iE;  // west stp column (strip start only)
+  auto fetch = [&](RowIn<T, CPT>& d, uint64_t r) {
+    const uint64_t rc = r < nr ? r : nr - 1;
+    const uint64_t p = rc * nth + ic;
+    load(d.N, sIn + srpIdx(ic, rc < nr - 1 ? rc : nr - 2, rs) * a.MC);
+    load(d.E, sIn + stpIdx(iE, rc, rs, nr) * a.MC);
+    load(d.vo, vIn + p * a.MC);
+    load(d.W, sIn + stpIdx(iW, rc, rs, nr) * a.MC);
+    load(d.c, a.cv + 4 * p);
+    // stress row r - 1, from the unclamped r (r = nr still has stress row nr - 1)
+    const uint64_t rb = r == 0 ? 0 : (r - 1 < nr ? r - 1 : nr - 1);
+    load(d.bS, a.cs + 2 * srpIdx(ic, rb < nr - 1 ? rb : nr - 2, rs));
+    load(d.bT, a.cs + 2 * stpIdx(iE, rb, rs, nr));
+  };
+
+  ColumnForcing<T, CPT> frc;
+  frc.load(a, colBase);
   T S[CPT], vj[CPT], Eold[CPT];
 #pragma unroll
-  for (int k = 0; k < CPT; ++k) { vj[k] = T(0); Eold[k] = T(0); S[k] = T(0); }
-  if (active) load(S, sIn + srpIdx(i, j0 > 0 ? j0 - 1 : 0, rs) * a.MC);
+  for (int k = 0; k < CPT; ++k) { vj[k] = T(0); Eold[k] = T(0); }
+  load(S, sIn + srpIdx(ic, j0 > 0 ? (j0 - 1 < nr - 1 ? j0 - 1 : nr - 2) : 0, rs) * a.MC);
 
-  // software pipeline: the old values of row r + 1 are loaded while row r is
-  // computed (nN/nE/nVo/nC/nW hold the next row)
-  T nN[CPT], nE[CPT], nVo[CPT], nW[CPT], nC[4];
-  auto fetch = [&](uint64_t r) {
-    const uint64_t p = r * nth + i;
-    load(nN, sIn + srpIdx(i, r < nr - 1 ? r : nr - 2, rs) * a.MC);
-    load(nE, sIn + stpIdx(i < nth - 2 ? i : nth - 2, r, rs, nr) * a.MC);
-    load(nVo, vIn + p * a.MC);
-    load(nC, a.cv + 4 * p);
-    if (pl == 0) load(nW, sIn + stpIdx(i > 0 ? i - 1 : 0, r, rs, nr) * a.MC);
-  };
-  if (active && j0 < nr) fetch(j0);
-
-  for (int64_t jj = (int64_t)j0 - 1; jj < (int64_t)j1; ++jj) {
+  // one row: new v of row r = jj + 1 (all strip points incl. the halo), new
+  // stress of row jj (owned points); `cur` holds row r, `nxt` receives r + 1
+  auto row = [&](const RowIn<T, CPT>& cur, RowIn<T, CPT>& nxt, int64_t jj) {
     const uint64_t r = (uint64_t)(jj + 1);
     const bool haveRow = r < nr;
-    T N[CPT], vo[CPT], E[CPT], vn[CPT];
-    T c[4];
-    if (active && haveRow) {
-#pragma unroll
-      for (int k = 0; k < CPT; ++k) { N[k] = nN[k]; E[k] = nE[k]; vo[k] = nVo[k]; }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) c[k] = nC[k];
-      store(stpRow + (pl + 1) * a.MC + m0, E);
-      if (pl == 0) store(stpRow + m0, nW);
-      if (r + 1 < nr && r + 1 <= j1) fetch(r + 1);
-    }
+    store(stpRow + (pl + 1) * a.MC + m0, cur.E);
+    if (pl == 0) store(stpRow + m0, cur.W);
+    fetch(nxt, r + 1);
     __syncthreads();
+    T vn[CPT];
     if (active && haveRow) {
       const uint64_t p = r * nth + i;
       T W[CPT];
       load(W, stpRow + pl * a.MC + m0);
 #pragma unroll
       for (int k = 0; k < CPT; ++k) {
-        T sum = c[0] * N[k] + c[1] * S[k];
-        sum = sum + c[2] * W[k];
-        sum = sum + c[3] * E[k];
-        vn[k] = vo[k] + a.dt * sum;
+        T sum = cur.c[0] * cur.N[k] + cur.c[1] * S[k];
+        sum = sum + cur.c[2] * W[k];
+        sum = sum + cur.c[3] * cur.E[k];
+        vn[k] = cur.vo[k] + a.dt * sum;
       }
-      velocityEpilogue<T, CPT>(a, vn, p, colBase);
+      frc.apply(vn, p);
       if (owner && r < j1) {
         store(vOut + p * a.MC, vn);
-        recordReceivers<T, CPT>(a, vn, p, colBase);
+        if (a.recvRow < 0 || (int64_t)r == a.recvRow) recordReceivers<T, CPT>(a, vn, p, colBase);
       }
     }
-    // stress rows jj (owned rows only): srp(i, jj) needs v_new rows jj and jj+1,
+    // stress rows jj (owned points): srp(i, jj) needs v_new rows jj and jj+1,
     // stp(i, jj) needs v_new(i, jj) and v_new(i+1, jj) (from LDS)
     if (owner && jj >= (int64_t)j0) {
       const uint64_t jr = (uint64_t)jj;
       if (jr + 1 < nr) {
-        const uint64_t q = srpIdx(i, jr, rs);
-        T b[2], so[CPT];
-        load(b, a.cs + 2 * q);
+        T so[CPT];
 #pragma unroll
-        for (int k = 0; k < CPT; ++k) so[k] = S[k] + a.dt * (b[0] * vn[k] + b[1] * vj[k]);
-        store(sOut + q * a.MC, so);
+        for (int k = 0; k < CPT; ++k) so[k] = S[k] + a.dt * (cur.bS[0] * vn[k] + cur.bS[1] * vj[k]);
+        store(sOut + srpIdx(i, jr, rs) * a.MC, so);
       }
       if (i + 1 < nth) {
-        const uint64_t q = stpIdx(i, jr, rs, nr);
-        T b[2], vE[CPT], so[CPT];
-        load(b, a.cs + 2 * q);
+        T vE[CPT], so[CPT];
         load(vE, vRow + (pl + 1) * a.MC + m0);
 #pragma unroll
-        for (int k = 0; k < CPT; ++k) so[k] = Eold[k] + a.dt * (b[0] * vj[k] + b[1] * vE[k]);
-        store(sOut + q * a.MC, so);
+        for (int k = 0; k < CPT; ++k) so[k] = Eold[k] + a.dt * (cur.bT[0] * vj[k] + cur.bT[1] * vE[k]);
+        store(sOut + stpIdx(i, jr, rs, nr) * a.MC, so);
       }
     }
     __syncthreads();
     if (active && haveRow) {
       store(vRow + pl * a.MC + m0, vn);
 #pragma unroll
-      for (int k = 0; k < CPT; ++k) { vj[k] = vn[k]; S[k] = N[k]; Eold[k] = E[k]; }
+      for (int k = 0; k < CPT; ++k) { vj[k] = vn[k]; S[k] = cur.N[k]; Eold[k] = cur.E[k]; }
     }
+  };
+
+  RowIn<T, CPT> A, B;
+  fetch(A, j0);
+  for (int64_t jj = (int64_t)j0 - 1; jj < (int64_t)j1; jj += 2) {
+    row(A, B, jj);
+    if (jj + 1 < (int64_t)j1) row(B, A, jj + 1);
   }
 }
 
@@ -300,6 +336,14 @@ __global__ void k_snapshot(const T* __restrict__ x, uint64_t n, uint32_t MC, con
   if (g >= n || c >= ncols) return;
   const uint32_t col = cols[c];
   out[g + n * (sample + Tcap * c)] = (double)x[((uint64_t)(col / MC) * n + g) * MC + col % MC];
+}
+
+// x[r + n*c] *= w[r] for every column c (grid-stride over columns)
+__global__ void k_scale_rows(double* __restrict__ x, const double* __restrict__ w, uint64_t n, uint64_t cols) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double f = w[r];
+  for (uint64_t c = blockIdx.y; c < cols; c += gridDim.y) x[r + n * c] *= f;
 }
 
 // plain streaming copy (16 B per lane per iteration), for bandwidth and
@@ -349,7 +393,7 @@ struct shaw_fom {
   virtual void time(uint64_t nsteps, double* total, double* perStep) = 0;
   virtual void bytes(double* kernel, double* model) const = 0;
   virtual void config(int* kernel, int* cpt, int* strip, int* seg) const = 0;
-  virtual void pod(int dof, uint64_t k, double* phi, double* sigma) = 0;
+  virtual void pod(int dof, uint64_t k, const double* w, double* phi, double* sigma) = 0;
 };
 
 namespace shaw {
@@ -375,14 +419,16 @@ class FomEngine final : public shaw_fom {
     while (MC_ < M && MC_ < cap) MC_ *= 2;
     nb_ = (uint32_t)((M + MC_ - 1) / MC_);
     kernel_ = o.kernel == SHAW_KERNEL_SPLIT ? SHAW_KERNEL_SPLIT : SHAW_KERNEL_FUSED;
-    cpt_ = o.cols_per_thread > 0 ? o.cols_per_thread : (int)std::min<uint32_t>(MC_, sizeof(T) == 8 ? 4 : 8);
+    // defaults from a sweep of the fused kernel on MI355X at 1024 x 4096, M = 32
+    // (FP64: 2 columns per thread, 32-point strips, 32-row segments)
+    cpt_ = o.cols_per_thread > 0 ? o.cols_per_thread : (int)std::min<uint32_t>(MC_, sizeof(T) == 8 ? 2 : 4);
     SHAW_REQUIRE(cpt_ == 1 || cpt_ == 2 || cpt_ == 4 || cpt_ == 8, SHAW_ERR_ARG, "cols_per_thread must be 1, 2, 4 or 8");
     SHAW_REQUIRE(MC_ % cpt_ == 0, SHAW_ERR_ARG, "cols_per_thread must divide the column block width");
     const uint32_t tpp = MC_ / cpt_;
     if (kernel_ == SHAW_KERNEL_FUSED) {
       P_ = o.strip > 0 ? (uint32_t)o.strip : std::max<uint32_t>(2, std::min<uint32_t>(64, 512 / tpp));
       SHAW_REQUIRE(P_ >= 2 && P_ * tpp <= 512, SHAW_ERR_ARG, "strip too wide for a 512-thread block");
-      SEG_ = o.segment > 0 ? (uint32_t)o.segment : 64;
+      SEG_ = o.segment > 0 ? (uint32_t)o.segment : 32;
       nStrips_ = (uint32_t)((nth_ + (P_ - 1) - 1) / (P_ - 1));
       nSegs_ = (uint32_t)((nr_ + SEG_ - 1) / SEG_);
     }
@@ -444,6 +490,11 @@ class FomEngine final : public shaw_fom {
     }
     recv_.alloc(R ? nVp_ : 0);
     if (R) SHAW_HIP(hipMemcpy(recv_.p, idx.data(), nVp_ * 4, hipMemcpyHostToDevice));
+    // receivers sit on the surface row in every reference input; the fused
+    // kernel then looks the table up on that row only
+    recvRow_ = R ? (int64_t)(gid[0] / nth_) : -1;
+    for (uint64_t r = 0; r < R; ++r)
+      if ((int64_t)(gid[r] / nth_) != recvRow_) recvRow_ = -1;
     R_ = R;
     every_ = every;
     Tcap_ = maxSamples;
@@ -541,14 +592,33 @@ class FomEngine final : public shaw_fom {
     }
     if (model) *model = s * (double)M_ * 3.0 * (double)(nVp_ + nSp_) + 8.0 * (4.0 * nVp_ + 2.0 * nSp_);
   }
-  void pod(int dof, uint64_t k, double* phi, double* sigma) override {
+  void pod(int dof, uint64_t k, const double* w, double* phi, double* sigma) override {
     SHAW_REQUIRE(sizeof(T) == 8, SHAW_ERR_ARG, "POD of snapshots needs the FP64 FOM");
     Snap& s = snap_[dof];
     SHAW_REQUIRE(s.ncols > 0, SHAW_ERR_STATE, "no snapshots were configured for this dof");
     const uint64_t T_ = step_ / s.every, n = dof == 0 ? nVp_ : nSp_;
     SHAW_REQUIRE(T_ == s.Tcap, SHAW_ERR_STATE, "POD needs a full snapshot buffer (advance to max_samples)");
     sync();
+    std::vector<double> sw;
+    if (w) {  // POD in the inner product diag(w): modes of diag(sqrt w) S, then unscaled
+      sw.resize(n);
+      for (uint64_t i = 0; i < n; ++i) {
+        SHAW_REQUIRE(w[i] > 0 && std::isfinite(w[i]), SHAW_ERR_ARG, "POD weights must be positive and finite");
+        sw[i] = std::sqrt(w[i]);
+      }
+      DevBuf<double> dw;
+      dw.alloc(n);
+      SHAW_HIP(hipMemcpy(dw.p, sw.data(), n * sizeof(double), hipMemcpyHostToDevice));
+      const uint64_t cols = T_ * s.ncols;
+      hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((n + 255) / 256), (unsigned)std::min<uint64_t>(cols, 65535)),
+                         dim3(256), 0, 0, s.buf.p, dw.p, n, cols);
+      SHAW_HIP(hipGetLastError());
+      SHAW_HIP(hipDeviceSynchronize());
+    }
     podDevice(s.buf.p, n, T_ * s.ncols, k, phi, sigma);
+    if (w)
+      for (uint64_t c = 0; c < k; ++c)
+        for (uint64_t i = 0; i < n; ++i) phi[i + n * c] /= sw[i];
     s.ncols = 0;  // the buffer was consumed
   }
   void config(int* kernel, int* cpt, int* strip, int* seg) const override {
@@ -574,6 +644,7 @@ class FomEngine final : public shaw_fom {
     a.frow = ftab_.p + (stepNo - 1) * M_;
     const bool rec = R_ && stepNo % every_ == 0;
     a.recv = rec ? recv_.p : nullptr;
+    a.recvRow = recvRow_;
     a.seis = seis_.p;
     a.sample = rec ? stepNo / every_ - 1 : 0;
     a.R = R_;
@@ -663,6 +734,7 @@ class FomEngine final : public shaw_fom {
   DevBuf<int64_t> src_;
   DevBuf<double> ftab_, seis_;
   DevBuf<int32_t> recv_;
+  int64_t recvRow_ = -1;
   std::vector<double> rhoInv_;
   std::vector<std::pair<uint32_t, uint32_t>> dupRecv_;
   uint64_t ftabSteps_ = 0, R_ = 0, every_ = 1, Tcap_ = 0, step_ = 0, stepForSnap_ = 0;
@@ -757,9 +829,12 @@ int shaw_fom_config(shaw_fom_t h, int* kernel, int* cpt, int* strip, int* seg) {
   return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->config(kernel, cpt, strip, seg); });
 }
 int shaw_fom_pod(shaw_fom_t h, int dof, uint64_t k, double* phi, double* sigma) {
+  return shaw_fom_pod_weighted(h, dof, k, nullptr, phi, sigma);
+}
+int shaw_fom_pod_weighted(shaw_fom_t h, int dof, uint64_t k, const double* w, double* phi, double* sigma) {
   return guarded([&] {
     SHAW_REQUIRE(h && phi && (dof == 0 || dof == 1), SHAW_ERR_ARG, "bad argument");
-    h->pod(dof, k, phi, sigma);
+    h->pod(dof, k, w, phi, sigma);
   });
 }
 int shaw_debug_copy(uint64_t bytes, int reps, double* ms_per_copy) {

@@ -20,6 +20,11 @@
 //   BF16x3: Ar = Ar_hi + Ar_lo and q = q_hi + q_lo in bf16; the product uses
 //           Ar_hi q_hi + Ar_hi q_lo + Ar_lo q_hi on v_mfma_f32_16x16x16bf16_1k
 //           with FP32 accumulation; the state update q += dt * acc is FP64.
+//   BF16x4: as BF16x3 plus Ar_res q_hi, Ar_res = bf16(Ar - Ar_hi - Ar_lo):
+//           the operator is represented to ~2^-24 instead of ~2^-17. Over
+//           thousands of steps the operator's representation error acts as
+//           a fixed frequency shift (a phase error that grows with time),
+//           while the state's split error is new every step.
 // Each column has its own step counter and horizon; a column past its
 // horizon is frozen (serving with uneven horizons).
 #include <hip/hip_runtime.h>
@@ -252,15 +257,22 @@ void symmetricEigen(std::vector<double>& A, int n, std::vector<double>& w, std::
   // QL with implicit shifts on the tridiagonal (d, e)
   for (int i = 1; i < n; ++i) e[i - 1] = e[i];
   if (n > 0) e[n - 1] = 0.0;
+  // An off-diagonal entry is negligible relative to its neighbours, or
+  // below 1e-3 eps ||T|| (an absolute eigenvalue error the POD never sees:
+  // each pass uses only eigenvalues above 1e-6 of the largest). The second
+  // test ends the iteration on clusters of noise-level eigenvalues.
+  double tnorm = 0.0;
+  for (int i = 0; i < n; ++i) tnorm = std::max(tnorm, std::abs(d[i]) + std::abs(e[i]));
+  const double eps = std::numeric_limits<double>::epsilon(), floorAbs = 1e-3 * eps * tnorm;
   for (int l = 0; l < n; ++l) {
     int iter = 0, m;
     do {
       for (m = l; m < n - 1; ++m) {
         const double dd = std::abs(d[m]) + std::abs(d[m + 1]);
-        if (std::abs(e[m]) <= std::numeric_limits<double>::epsilon() * dd) break;
+        if (std::abs(e[m]) <= eps * dd || std::abs(e[m]) <= floorAbs) break;
       }
       if (m != l) {
-        SHAW_REQUIRE(iter++ < 60, SHAW_ERR_ARG, "eigensolver did not converge");
+        SHAW_REQUIRE(iter++ < 100, SHAW_ERR_ARG, "eigensolver did not converge");
         double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
         double r = std::hypot(g, 1.0);
         g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
@@ -332,18 +344,19 @@ __global__ void k_mfma_probe(int kind, const double* __restrict__ a, const doubl
 }
 
 // ------------------------------------------------------------ reduced leapfrog
-// Operators are packed for the MFMA A operand in 32-byte lane items, laid out
-// [rowTile][group][lane] so that one wave reads 2 KB contiguous per group:
-//   FP64:   group g of row tile t covers k-steps 4g..4g+3 of 16 x 16 x 4;
-//           lane l holds A[16t + l%16][16g + 4s + l/16], s = 0..3.
-//   BF16x3: group g covers k-steps 2g, 2g+1 of 16 x 16 x 16; lane l holds,
-//           for s = 0, 1, the hi then the lo bf16 parts of
-//           A[16t + l%16][32g + 16s + 4(l/16) + e], e = 0..3.
+// Operators are packed for the MFMA A operand in lane items, laid out
+// [rowTile][group][lane] so that one wave reads contiguous memory per group:
+//   FP64:   32-byte items; group g of row tile t covers k-steps 4g..4g+3 of
+//           16 x 16 x 4; lane l holds A[16t + l%16][16g + 4s + l/16], s < 4.
+//   BF16x3 / BF16x4: 2 / 3 parts of 16 bytes (hi, lo, residual bf16 parts);
+//           group g covers k-steps 2g, 2g+1 of 16 x 16 x 16; part p of lane
+//           l holds that part of A[16t + l%16][32g + 16s + 4(l/16) + e] at
+//           position 4s + e, s < 2, e < 4.
 // Padded sizes are multiples of kRing groups (64 for FP64, 128 for BF16x3).
 struct RomArgs {
-  const dbl4* av;       // Ar_v: rows Kvp, k-dim Ksp
-  const dbl4* as;       // Ar_s: rows Ksp, k-dim Kvp
-  const dbl4* c;        // C:    rows Rp,  k-dim Kvp
+  const void* av;       // Ar_v: rows Kvp, k-dim Ksp
+  const void* as;       // Ar_s: rows Ksp, k-dim Kvp
+  const void* c;        // C:    rows Rp,  k-dim Kvp
   double* qv;           // [slot][Kvp]
   double* qs;           // [slot][Ksp]
   const double* b;      // [slot][Kvp]
@@ -359,24 +372,30 @@ struct RomArgs {
 
 constexpr int kRing = 4;  // operator groups kept in flight per row tile
 
-template <bool BF16>
-struct Acc;
+// PARTS = 0: FP64 operators; 2: bf16 hi + lo (BF16x3); 3: bf16 hi + lo +
+// residual (BF16x4). Ring items: a dbl4 (FP64) or PARTS 16-byte parts, part p
+// holding that part's 4 bf16 values of k-steps 2g and 2g+1 (32 B or 48 B).
+typedef int int4v __attribute__((ext_vector_type(4)));
+typedef short shrt8 __attribute__((ext_vector_type(8)));
+template <int PARTS>
+struct BfItem {
+  int4v p[PARTS];
+};
+template <int PARTS>
+struct Prec {
+  using Item = BfItem<PARTS>;
+  using Acc = flt4;
+};
 template <>
-struct Acc<false> { using T = dbl4; };
-template <>
-struct Acc<true> { using T = flt4; };
+struct Prec<0> {
+  using Item = dbl4;
+  using Acc = dbl4;
+};
 
-// acc[t] = A[rows of tile rt[t]] X for the wave's TPW row tiles, X the 16
-// state columns in LDS ([K][16] FP64). The operator groups stream through a
-// register ring kRing deep; the B fragment of each k-step (and, for BF16x3,
-// its hi/lo split) is built once and used by all TPW tiles. kGroups is a
-// multiple of kRing (the padded sizes guarantee it), so the loop body has no
-// branches and the loads of the next kRing - 1 groups stay in flight while
-// a group's MFMAs run.
-template <bool BF16, int TPW>
-__device__ __forceinline__ void groupProducts(const dbl4 (&ag)[TPW], int g, const double* X, int lane,
-                                              typename Acc<BF16>::T (&acc)[TPW]) {
-  if constexpr (!BF16) {
+template <int PARTS, int TPW>
+__device__ __forceinline__ void groupProducts(const typename Prec<PARTS>::Item (&ag)[TPW], int g, const double* X, int lane,
+                                              typename Prec<PARTS>::Acc (&acc)[TPW]) {
+  if constexpr (PARTS == 0) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const double xb = X[((4 * g + s) * 4 + lane / 16) * kTileCols + lane % 16];
@@ -396,55 +415,66 @@ __device__ __forceinline__ void groupProducts(const dbl4 (&ag)[TPW], int g, cons
       }
 #pragma unroll
       for (int t = 0; t < TPW; ++t) {
-        const shrt16 w = __builtin_bit_cast(shrt16, ag[t]);
-        const shrt4 ah = {w[8 * s + 0], w[8 * s + 1], w[8 * s + 2], w[8 * s + 3]};
-        const shrt4 al = {w[8 * s + 4], w[8 * s + 5], w[8 * s + 6], w[8 * s + 7]};
+        const shrt8 w0 = __builtin_bit_cast(shrt8, ag[t].p[0]);
+        const shrt8 w1 = __builtin_bit_cast(shrt8, ag[t].p[1]);
+        const shrt4 ah = {w0[4 * s + 0], w0[4 * s + 1], w0[4 * s + 2], w0[4 * s + 3]};
+        const shrt4 al = {w1[4 * s + 0], w1[4 * s + 1], w1[4 * s + 2], w1[4 * s + 3]};
         acc[t] = mfmaBf16(ah, xh, acc[t]);
         acc[t] = mfmaBf16(ah, xl, acc[t]);
         acc[t] = mfmaBf16(al, xh, acc[t]);
+        if constexpr (PARTS == 3) {
+          const shrt8 w2 = __builtin_bit_cast(shrt8, ag[t].p[2]);
+          const shrt4 ar = {w2[4 * s + 0], w2[4 * s + 1], w2[4 * s + 2], w2[4 * s + 3]};
+          acc[t] = mfmaBf16(ar, xh, acc[t]);
+        }
       }
     }
   }
 }
 
-template <bool BF16, int TPW>
-__device__ __forceinline__ void tileProducts(const dbl4* __restrict__ A, int kGroups, const int (&rt)[TPW],
-                                             const double* X, int lane, typename Acc<BF16>::T (&acc)[TPW]) {
-  dbl4 ring[kRing][TPW];
-  const dbl4* row[TPW];
+template <int PARTS, int TPW>
+__device__ __forceinline__ void tileProducts(const void* __restrict__ Araw, int kGroups, const int (&rt)[TPW],
+                                             const double* X, int lane, typename Prec<PARTS>::Acc (&acc)[TPW]) {
+  using Item = typename Prec<PARTS>::Item;
+  // BF16x4 with 4 tiles per wave: a 2-deep ring keeps the registers within
+  // the 256 of a 2-waves-per-SIMD workgroup (kGroups stays a multiple of it)
+  constexpr int depth = PARTS * TPW > 9 ? 2 : kRing;
+  const Item* A = reinterpret_cast<const Item*>(Araw);
+  Item ring[depth][TPW];
+  const Item* row[TPW];
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
-    acc[t] = typename Acc<BF16>::T{0, 0, 0, 0};
+    acc[t] = typename Prec<PARTS>::Acc{0, 0, 0, 0};
     row[t] = A + (size_t)rt[t] * kGroups * 64 + lane;
   }
 #pragma unroll
-  for (int d = 0; d < kRing; ++d) {
+  for (int d = 0; d < depth; ++d) {
 #pragma unroll
     for (int t = 0; t < TPW; ++t) ring[d][t] = row[t][d * 64];
     __builtin_amdgcn_sched_barrier(0);  // issue order = consumption order
   }
   int g0 = 0;
-  for (; g0 + kRing < kGroups; g0 += kRing) {
+  for (; g0 + depth < kGroups; g0 += depth) {
 #pragma unroll
-    for (int d = 0; d < kRing; ++d) {
-      groupProducts<BF16, TPW>(ring[d], g0 + d, X, lane, acc);
+    for (int d = 0; d < depth; ++d) {
+      groupProducts<PARTS, TPW>(ring[d], g0 + d, X, lane, acc);
 #pragma unroll
-      for (int t = 0; t < TPW; ++t) ring[d][t] = row[t][(g0 + d + kRing) * 64];
+      for (int t = 0; t < TPW; ++t) ring[d][t] = row[t][(g0 + d + depth) * 64];
       // keep this group's refill here: the scheduler would otherwise sink all
       // refills of the round to its end, shortening the prefetch distance
       __builtin_amdgcn_sched_barrier(0);
     }
   }
 #pragma unroll
-  for (int d = 0; d < kRing; ++d) groupProducts<BF16, TPW>(ring[d], g0 + d, X, lane, acc);
+  for (int d = 0; d < depth; ++d) groupProducts<PARTS, TPW>(ring[d], g0 + d, X, lane, acc);
 }
 
-__device__ __forceinline__ int accRow(bool bf16, int lane, int r) { return bf16 ? rowBf16(lane, r) : rowF64(lane, r); }
+__device__ __forceinline__ int accRow(int parts, int lane, int r) { return parts ? rowBf16(lane, r) : rowF64(lane, r); }
 
 // Y[rows][cols] += dt * (A X) (+ (f dt) b for the velocity half), active
 // columns only. Row tiles rt0 + t * kWaves, t < TPW, belong to this wave.
-template <bool BF16, int TPW>
-__device__ __forceinline__ void halfStep(const RomArgs& a, const dbl4* A, int rowTiles, int kGroups, const double* Xs,
+template <int PARTS, int TPW>
+__device__ __forceinline__ void halfStep(const RomArgs& a, const void* A, int rowTiles, int kGroups, const double* Xs,
                                          double* Ys, int wave, int lane, bool active, double fdt, const double* bcol,
                                          int slot0) {
   const int col = lane % 16;
@@ -457,15 +487,15 @@ __device__ __forceinline__ void halfStep(const RomArgs& a, const dbl4* A, int ro
 #pragma unroll
       for (int t = 0; t < TPW; ++t)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) bv[t][r] = bcol[(uint64_t)(slot0 + col) * a.Kvp + rt[t] * 16 + accRow(BF16, lane, r)];
-    typename Acc<BF16>::T acc[TPW];
-    tileProducts<BF16, TPW>(A, kGroups, rt, Xs, lane, acc);
+        for (int r = 0; r < 4; ++r) bv[t][r] = bcol[(uint64_t)(slot0 + col) * a.Kvp + rt[t] * 16 + accRow(PARTS, lane, r)];
+    typename Prec<PARTS>::Acc acc[TPW];
+    tileProducts<PARTS, TPW>(A, kGroups, rt, Xs, lane, acc);
 #pragma unroll
     for (int t = 0; t < TPW; ++t)
       if (rt0 + t * kWaves < rowTiles && active)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = rt[t] * 16 + accRow(BF16, lane, r);
+          const int row = rt[t] * 16 + accRow(PARTS, lane, r);
           double y = Ys[row * kTileCols + col] + a.dt * (double)acc[t][r];
           if (bcol) y = y + fdt * bv[t][r];
           Ys[row * kTileCols + col] = y;
@@ -473,23 +503,23 @@ __device__ __forceinline__ void halfStep(const RomArgs& a, const dbl4* A, int ro
   }
 }
 
-template <bool BF16, int TPW>
+template <int PARTS, int TPW>
 __device__ __forceinline__ void receivers(const RomArgs& a, const double* Qv, int wave, int lane, bool sample,
                                           uint32_t sampleIdx, int slot0) {
-  const int rowTiles = a.Rp / 16, kGroups = BF16 ? a.Kvp / 32 : a.Kvp / 16;
+  const int rowTiles = a.Rp / 16, kGroups = PARTS ? a.Kvp / 32 : a.Kvp / 16;
   for (int rt0 = wave; rt0 < rowTiles; rt0 += kWaves * TPW) {
     int rt[TPW];
 #pragma unroll
     for (int t = 0; t < TPW; ++t) rt[t] = rt0 + t * kWaves < rowTiles ? rt0 + t * kWaves : rt0;
-    typename Acc<BF16>::T acc[TPW];
-    tileProducts<BF16, TPW>(a.c, kGroups, rt, Qv, lane, acc);
+    typename Prec<PARTS>::Acc acc[TPW];
+    tileProducts<PARTS, TPW>(a.c, kGroups, rt, Qv, lane, acc);
     const int col = lane % 16;
 #pragma unroll
     for (int t = 0; t < TPW; ++t)
       if (rt0 + t * kWaves < rowTiles && sample)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = rt[t] * 16 + accRow(BF16, lane, r);
+          const int row = rt[t] * 16 + accRow(PARTS, lane, r);
           // per slot LayoutLeft [R x Tcap]
           if (row < (int)a.R)
             a.seis[(uint64_t)(slot0 + col) * a.R * a.Tcap + (uint64_t)a.R * sampleIdx + row] = (double)acc[t][r];
@@ -505,7 +535,7 @@ __device__ __forceinline__ void receivers(const RomArgs& a, const double* Qv, in
 // step s); the block-uniform decisions (stop, sample) come from values every
 // wave computes identically, so a step needs only the two barriers between
 // the half-steps.
-template <bool BF16, int TPW>
+template <int PARTS, int TPW>
 __global__ void __launch_bounds__(kWaves * 64) k_rom_steps(RomArgs a) {
   extern __shared__ double sm[];
   double* Qv = sm;                                   // [Kvp][16]
@@ -532,7 +562,7 @@ __global__ void __launch_bounds__(kWaves * 64) k_rom_steps(RomArgs a) {
   for (int c = 0; c < kTileCols; ++c) steps = remain[c] > steps ? remain[c] : steps;
   if ((uint64_t)steps > a.nsteps) steps = (uint32_t)a.nsteps;
   const int kvTiles = a.Kvp / 16, ksTiles = a.Ksp / 16;
-  const int kGroupsS = BF16 ? a.Ksp / 32 : a.Ksp / 16, kGroupsV = BF16 ? a.Kvp / 32 : a.Kvp / 16;
+  const int kGroupsS = PARTS ? a.Ksp / 32 : a.Ksp / 16, kGroupsV = PARTS ? a.Kvp / 32 : a.Kvp / 16;
   double fcur = s0 < hz ? fcol[s0] : 0.0;
   uint32_t phase = ev ? (s0 + 1) % ev : 0;     // (s + 1) mod every
   uint32_t nextIdx = ev ? (s0 + 1) / ev : 0;   // index of the next sample
@@ -545,11 +575,11 @@ __global__ void __launch_bounds__(kWaves * 64) k_rom_steps(RomArgs a) {
     // lanes 0..15 of every wave hold columns 0..15: the same answer in all waves
     const bool anySample = (__ballot(smp) & 0xffffull) != 0;
     // velocity: q_v += dt Ar_v q_s ; q_v += (f dt) b
-    halfStep<BF16, TPW>(a, a.av, kvTiles, kGroupsS, Qs, Qv, wave, lane, act, fdt, a.b, slot0);
+    halfStep<PARTS, TPW>(a, a.av, kvTiles, kGroupsS, Qs, Qv, wave, lane, act, fdt, a.b, slot0);
     __syncthreads();
-    if (anySample) receivers<BF16, TPW>(a, Qv, wave, lane, smp, nextIdx - 1, slot0);
+    if (anySample) receivers<PARTS, TPW>(a, Qv, wave, lane, smp, nextIdx - 1, slot0);
     // stress: q_s += dt Ar_s q_v
-    halfStep<BF16, TPW>(a, a.as, ksTiles, kGroupsV, Qv, Qs, wave, lane, act, 0.0, nullptr, slot0);
+    halfStep<PARTS, TPW>(a, a.as, ksTiles, kGroupsV, Qv, Qs, wave, lane, act, 0.0, nullptr, slot0);
     __syncthreads();
     fcur = fnext;
     if (ev && act) {
@@ -607,9 +637,9 @@ std::vector<double> packF64(const double* A, uint64_t rows, uint64_t cols, uint3
         }
   return out;
 }
-std::vector<double> packBf16(const double* A, uint64_t rows, uint64_t cols, uint32_t rowsP, uint32_t colsP) {
+std::vector<double> packBf16(const double* A, uint64_t rows, uint64_t cols, uint32_t rowsP, uint32_t colsP, int parts) {
   const uint32_t rt = rowsP / 16, G = colsP / 32;
-  std::vector<double> out((size_t)rt * G * 64 * 4, 0.0);
+  std::vector<double> out((size_t)rt * G * 64 * 2 * parts, 0.0);  // parts x 16 B per lane item
   unsigned short* o16 = reinterpret_cast<unsigned short*>(out.data());
   for (uint32_t t = 0; t < rt; ++t)
     for (uint32_t g = 0; g < G; ++g)
@@ -619,24 +649,27 @@ std::vector<double> packBf16(const double* A, uint64_t rows, uint64_t cols, uint
             const uint64_t r = t * 16 + l % 16, c = g * 32 + 16 * s + 4 * (l / 16) + e;
             const double v = (r < rows && c < cols) ? A[r * cols + c] : 0.0;
             const unsigned short h = f32ToBf16((float)v);
-            const size_t o = (((size_t)t * G + g) * 64 + l) * 16 + 8 * s + e;
+            const double rest = v - (double)bf16ToF32(h);
+            const unsigned short lo = f32ToBf16((float)rest);
+            const size_t o = (((size_t)t * G + g) * 64 + l) * (8 * parts) + 4 * s + e;
             o16[o] = h;
-            o16[o + 4] = f32ToBf16((float)(v - (double)bf16ToF32(h)));
+            o16[o + 8] = lo;
+            if (parts == 3) o16[o + 16] = f32ToBf16((float)(rest - (double)bf16ToF32(lo)));
           }
   return out;
 }
 
-template <bool BF16>
+template <int PARTS>
 void launchRomSteps(int tpw, dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a) {
   auto go = [&](auto kernel) {
     (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(kernel, grid, dim3(kWaves * 64), lds, stream, a);
   };
   switch (tpw) {
-    case 1: go(k_rom_steps<BF16, 1>); break;
-    case 2: go(k_rom_steps<BF16, 2>); break;
-    case 3: go(k_rom_steps<BF16, 3>); break;
-    default: go(k_rom_steps<BF16, 4>); break;
+    case 1: go(k_rom_steps<PARTS, 1>); break;
+    case 2: go(k_rom_steps<PARTS, 2>); break;
+    case 3: go(k_rom_steps<PARTS, 3>); break;
+    default: go(k_rom_steps<PARTS, 4>); break;
   }
 }
 
@@ -788,8 +821,9 @@ struct shaw_rom {
     a.maxSteps = maxSteps; a.Tcap = maxSamples; a.nsteps = nsteps; a.dt = dt;
     const dim3 grid((unsigned)(maxCols / kTileCols));
     const size_t lds = (size_t)(kvp + ksp) * kTileCols * sizeof(double);
-    if (precision == SHAW_ROM_FP64) launchRomSteps<false>(tpw, grid, lds, stream, a);
-    else launchRomSteps<true>(tpw, grid, lds, stream, a);
+    if (precision == SHAW_ROM_FP64) launchRomSteps<0>(tpw, grid, lds, stream, a);
+    else if (precision == SHAW_ROM_BF16X3) launchRomSteps<2>(tpw, grid, lds, stream, a);
+    else launchRomSteps<3>(tpw, grid, lds, stream, a);
     SHAW_HIP(hipGetLastError());
     for (uint64_t c = 0; c < maxCols; ++c) {
       const uint64_t s = hStep[c] + nsteps;
@@ -839,12 +873,46 @@ int shaw_rom_project(shaw_app_t app, const double* phiV, uint64_t kv, const doub
   });
 }
 
+int shaw_rom_project_energy(shaw_app_t app, const double* phiV, uint64_t kv, const double* phiS, uint64_t ks,
+                            const double* mv, double* arV, double* arS) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(app && phiV && phiS && mv && arV && arS && kv > 0 && ks > 0, SHAW_ERR_ARG, "bad argument");
+    const Grid& g = app->ops.grid;
+    const uint64_t nVp = g.nVp(), nSp = g.nSp();
+    DevBuf<double> cv, pv, ps, tmp, part, out;
+    cv.alloc(app->ops.coefVp.size()); cv.upload(app->ops.coefVp.data(), cv.n);
+    // left factor diag(Mv) Phi_v, scaled on the host (exact products)
+    std::vector<double> left(nVp * kv);
+    for (uint64_t c = 0; c < kv; ++c)
+      for (uint64_t i = 0; i < nVp; ++i) left[i + nVp * c] = mv[i] * phiV[i + nVp * c];
+    pv.alloc(nVp * kv); pv.upload(left.data(), pv.n);
+    ps.alloc(nSp * ks); ps.upload(phiS, ps.n);
+    tmp.alloc(nVp * ks);
+    hipLaunchKernelGGL(k_apply_jvp, dim3((unsigned)((nVp + 255) / 256), (unsigned)ks), dim3(256), 0, 0, cv.p, ps.p, tmp.p,
+                       (uint32_t)g.nr, (uint32_t)g.nth, ks);
+    const uint64_t chunk = std::max<uint64_t>(4096, (nVp + 255) / 256 / GT * GT + GT);
+    const uint32_t nch = (uint32_t)((nVp + chunk - 1) / chunk);
+    part.alloc((size_t)nch * kv * ks);
+    out.alloc((size_t)kv * ks);
+    hipLaunchKernelGGL(k_gemm_tn_partial, dim3((unsigned)((kv + GT - 1) / GT), (unsigned)((ks + GT - 1) / GT), nch), dim3(256),
+                       0, 0, pv.p, tmp.p, part.p, nVp, (uint32_t)kv, (uint32_t)ks, chunk);
+    const uint64_t ne = kv * ks;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, 0, part.p, out.p, ne, nch);
+    SHAW_HIP(hipGetLastError());
+    SHAW_HIP(hipMemcpy(arV, out.p, ne * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint64_t r = 0; r < ks; ++r)
+      for (uint64_t c = 0; c < kv; ++c) arS[r * kv + c] = -arV[c * ks + r];
+  });
+}
+
 int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
   using namespace shaw;
   return guarded([&] {
     SHAW_REQUIRE(d && out && d->ar_v && d->ar_s, SHAW_ERR_ARG, "null argument");
     SHAW_REQUIRE(d->kv > 0 && d->ks > 0 && d->kv <= 512 && d->ks <= 512, SHAW_ERR_ARG, "ROM size must be in 1..512 modes per dof");
-    SHAW_REQUIRE(d->precision == SHAW_ROM_FP64 || d->precision == SHAW_ROM_BF16X3, SHAW_ERR_ARG, "unknown ROM precision");
+    SHAW_REQUIRE(d->precision == SHAW_ROM_FP64 || d->precision == SHAW_ROM_BF16X3 || d->precision == SHAW_ROM_BF16X4,
+                 SHAW_ERR_ARG, "unknown ROM precision");
     SHAW_REQUIRE(d->max_cols > 0 && d->dt > 0, SHAW_ERR_ARG, "max_cols and dt must be > 0");
     SHAW_REQUIRE(d->r == 0 || d->c, SHAW_ERR_ARG, "receiver rows missing");
     int ndev = 0;
@@ -871,8 +939,9 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
     if (d->r) std::memcpy(cPad.data(), d->c, d->r * d->kv * sizeof(double));
     auto pack = [&](const double* A, uint64_t rows, uint64_t cols, uint32_t rowsP, uint32_t colsP,
                     shaw::DevBuf<double>& dst) {
-      const std::vector<double> p = d->precision == SHAW_ROM_FP64 ? packF64(A, rows, cols, rowsP, colsP)
-                                                                  : packBf16(A, rows, cols, rowsP, colsP);
+      const std::vector<double> p = d->precision == SHAW_ROM_FP64
+                                        ? packF64(A, rows, cols, rowsP, colsP)
+                                        : packBf16(A, rows, cols, rowsP, colsP, d->precision == SHAW_ROM_BF16X4 ? 3 : 2);
       dst.alloc(p.size());
       dst.upload(p.data(), p.size());
     };

@@ -134,8 +134,11 @@ def test_rom_sizes_vs_oracle(kv, ks):
         assert rel_l2(got[key], ref[key]) <= 1e-11, key
     bf = shaw_amd.rom_run(arv, ars, B, ftab, dt, n, C, every, shaw_amd.ROM_BF16X3)
     err = max(rel_l2(bf["seismogram"][m], ref["seismogram"][m]) for m in range(M))
-    say(f"[gpu] ROM kv={kv} ks={ks}: BF16x3 seismogram max rel L2 error vs FP64 = {err:.3e}")
+    b4 = shaw_amd.rom_run(arv, ars, B, ftab, dt, n, C, every, shaw_amd.ROM_BF16X4)
+    err4 = max(rel_l2(b4["seismogram"][m], ref["seismogram"][m]) for m in range(M))
+    say(f"[gpu] ROM kv={kv} ks={ks}: seismogram max rel L2 error vs FP64: BF16x3 {err:.3e}, BF16x4 {err4:.3e}")
     assert err <= 1e-4
+    assert err4 <= err and err4 <= 2e-5
 
 
 def test_rom_size_limit():

@@ -183,3 +183,19 @@ def test_pod_eigensolver_against_numpy(n):
     assert np.abs(np.sort(w) - ref).max() <= 1e-13 * scale
     assert np.abs(V.T @ V - np.eye(n)).max() <= 1e-13
     assert np.abs(G @ V - V * w).max() <= 1e-13 * scale
+
+
+def test_pod_eigensolver_rank_deficient_gram():
+    """A Gram matrix of rank 40 (plus rounding-level noise) of size 600: the
+    cluster of noise eigenvalues must not stall the QL iteration, and the
+    resolved eigenvalues stay accurate to ~eps ||G||."""
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((700, 40)) @ rng.standard_normal((40, 600))
+    X[:, 300:] *= 1e-7
+    G = X.T @ X
+    w, V = shaw_amd.debug_symeig(G)
+    ref = np.linalg.eigvalsh(G)
+    scale = np.abs(ref).max()
+    assert np.abs(np.sort(w) - ref).max() <= 1e-12 * scale
+    assert np.abs(G @ V - V * w).max() <= 1e-12 * scale
+
