@@ -44,7 +44,8 @@ def measured_traffic(kernel_cfg: dict):
     except OSError:
         return None
     kv = {k.strip(): v.strip() for k, v in kv.items()}
-    want = f"cols_per_thread={kernel_cfg['cols_per_thread']} strip={kernel_cfg['strip']} segment={kernel_cfg['segment']}"
+    want = (f"cols_per_thread={kernel_cfg['cols_per_thread']} strip={kernel_cfg['strip']} "
+            f"segment={kernel_cfg['segment']} prefetch_rows={kernel_cfg['prefetch_rows']}")
     if kv.get("kernel_config") != want:
         return None
     return float(kv["traffic_bytes_per_launch"])
@@ -138,9 +139,9 @@ def mfma_tflops(k: int, rates: dict) -> dict:
 
 def rom_err_section(app, w, fom_value, args):
     """The ROM at err <= 1e-3: the energy-consistent Galerkin ROM for one
-    source depth (640 km, theta = 0), trained on 4 periods (20, 35, 50, 70 s)
-    with snapshots every 100 steps (200 per forcing, 800 per dof), POD in the
-    FOM energy weights to 512 modes (nested), K = 384. M = 1024 forcings with
+    source depth (640 km, theta = 0), trained on 5 periods (20, 25, 35, 50,
+    70 s) with snapshots every 100 steps (200 per forcing, 1000 per dof), POD
+    in the FOM energy weights to 512 modes (nested), K = 384. M = 1024 forcings with
     seeded periods in [20, 70] s, 20000 steps, in FP64, BF16x3 and BF16x4;
     errors against the FP64 FOM on 16 spot-checked forcings. Returns the
     section and the basis (reused by the serving section)."""
@@ -149,7 +150,7 @@ def rom_err_section(app, w, fom_value, args):
 
     n, K, depth = args.rom_steps, 384, 640.0
     t0 = time.time()
-    train = [(depth, 0.0, p, workloads.DELAY_S) for p in (20.0, 35.0, 50.0, 70.0)]
+    train = [(depth, 0.0, p, workloads.DELAY_S) for p in (20.0, 25.0, 35.0, 50.0, 70.0)]
     basis = rw.train_basis(app, train, w.dt, n, max(1, n // 200), 512, energy=True)
     arv, ars = rw.operators(app, basis, K)
     periods = np.random.default_rng(640).uniform(20.0, 70.0, 1024)
@@ -169,10 +170,10 @@ def rom_err_section(app, w, fom_value, args):
     ok = {k: bool(max(v) <= 1e-3) for k, v in err_fom.items()}
     log(f"energy ROM errors vs FP64 FOM: { {k: summary(v) for k, v in err_fom.items()} }")
     out = {"workload": f"energy-consistent Galerkin ROM K={K}, one source depth ({depth:g} km), POD in the FOM "
-                       "energy weights of 400+400 snapshots... see setup; M=1024 forcings with seeded periods in "
+                       "energy weights of 1000+1000 snapshots; M=1024 forcings with seeded periods in "
                        f"[20, 70] s, {n} steps, 180 receivers every 10 steps",
-           "training": "4 forcings (periods 20, 35, 50, 70 s) at the same depth, snapshots every 100 steps "
-                       "(200 per forcing, 800 per dof)",
+           "training": "5 forcings (periods 20, 25, 35, 50, 70 s) at the same depth, snapshots every 100 steps "
+                       "(200 per forcing, 1000 per dof)",
            "forcing_steps_per_s": rates,
            "speedup_vs_fp64_fom": {k: v / fom_value for k, v in rates.items()},
            "mfma_tflops": mfma_tflops(K, rates),
@@ -182,7 +183,6 @@ def rom_err_section(app, w, fom_value, args):
            "pod_sigma_ratio": {"384": float(basis.sigma_v[383] / basis.sigma_v[0]),
                                "512": float(basis.sigma_v[511] / basis.sigma_v[0])},
            "setup_s": setup}
-    out["workload"] = out["workload"].replace("of 400+400 snapshots... see setup", "of 800+800 snapshots")
     return out, basis
 
 
@@ -258,6 +258,7 @@ def main() -> int:
     ap.add_argument("--cpt", type=int, default=0)
     ap.add_argument("--strip", type=int, default=0)
     ap.add_argument("--segment", type=int, default=0)
+    ap.add_argument("--prefetch", type=int, default=0)
     ap.add_argument("--copy-bytes", type=int, default=4 << 30)
     ap.add_argument("--no-rom", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -285,7 +286,7 @@ def main() -> int:
                                  w.dt, n_total)
     kern = shaw_amd.KERNEL_FUSED if args.kernel == "fused" else shaw_amd.KERNEL_SPLIT
     fom = shaw_amd.Fom(app, w.m, w.dt, kernel=kern, rank1_forcing=False, cols_per_thread=args.cpt,
-                       strip=args.strip, segment=args.segment)
+                       strip=args.strip, segment=args.segment, prefetch_rows=args.prefetch)
     fom.set_sources(gids)
     fom.set_forcing(tab)
     fom.set_receivers([app.map_receiver(a) for a in workloads.RECEIVERS], w.seis_every,

@@ -126,6 +126,7 @@ typedef struct {
   int cols_per_thread;  /* 0 = default; else 1, 2, 4 or 8 (must divide the column block) */
   int strip;            /* 0 = default; fused kernel: vp points per block row (incl. halo) */
   int segment;          /* 0 = default; fused kernel: radial rows per block */
+  int prefetch_rows;    /* 0 = default; fused kernel: rows loaded ahead, 1 or 2 */
 } shaw_fom_options;
 
 /* app: operators; m: forcing columns; dt: time step. opts may be NULL. */
@@ -165,7 +166,8 @@ int shaw_fom_time(shaw_fom_t h, uint64_t nsteps, double* total_ms, double* ms_pe
  * the reference model s*m*3*(nvp+nsp) + 8*(4*nvp + 2*nsp).               */
 int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel_bytes, double* model_bytes);
 /* Kernel configuration actually used: kernel, cols_per_thread, strip, segment. */
-int shaw_fom_config(shaw_fom_t h, int* kernel, int* cols_per_thread, int* strip, int* segment);
+int shaw_fom_config(shaw_fom_t h, int* kernel, int* cols_per_thread, int* strip, int* segment,
+                    int* prefetch_rows);
 /* POD basis of the device snapshot buffer of `dof` (all configured columns,
  * all samples: LayoutLeft [n x (T * ncols)]) without a host round trip, as
  * shaw_pod_basis; the snapshot buffer is consumed. FP64 FOM only.         */

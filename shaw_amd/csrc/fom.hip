@@ -210,7 +210,7 @@ struct RowIn {
   T c[4], bS[2], bT[2];
 };
 
-template <class T, int CPT>
+template <class T, int CPT, int PF>
 __global__ void __launch_bounds__(512) k_fused(StepArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   T* stpRow = reinterpret_cast<T*>(smem_raw);    // [P+1][MC]: old stp(i0-1 .. i0+P-1) of row r
@@ -267,13 +267,14 @@ __global__ void __launch_bounds__(512) k_fused(StepArgs<T> a) {
   load(S, sIn + srpIdx(ic, j0 > 0 ? (j0 - 1 < nr - 1 ? j0 - 1 : nr - 2) : 0, rs) * a.MC);
 
   // one row: new v of row r = jj + 1 (all strip points incl. the halo), new
-  // stress of row jj (owned points); `cur` holds row r, `nxt` receives r + 1
+  // stress of row jj (owned points); `cur` holds row r, `nxt` receives row
+  // r + PF (the set of row r - 1, consumed by the previous row)
   auto row = [&](const RowIn<T, CPT>& cur, RowIn<T, CPT>& nxt, int64_t jj) {
     const uint64_t r = (uint64_t)(jj + 1);
     const bool haveRow = r < nr;
     store(stpRow + (pl + 1) * a.MC + m0, cur.E);
     if (pl == 0) store(stpRow + m0, cur.W);
-    fetch(nxt, r + 1);
+    fetch(nxt, r + PF);
     __syncthreads();
     T vn[CPT];
     if (active && haveRow) {
@@ -319,11 +320,23 @@ __global__ void __launch_bounds__(512) k_fused(StepArgs<T> a) {
     }
   };
 
-  RowIn<T, CPT> A, B;
-  fetch(A, j0);
-  for (int64_t jj = (int64_t)j0 - 1; jj < (int64_t)j1; jj += 2) {
-    row(A, B, jj);
-    if (jj + 1 < (int64_t)j1) row(B, A, jj + 1);
+  // PF + 1 row sets rotate through an unrolled loop (no register copies)
+  if constexpr (PF == 1) {
+    RowIn<T, CPT> A, B;
+    fetch(A, j0);
+    for (int64_t jj = (int64_t)j0 - 1; jj < (int64_t)j1; jj += 2) {
+      row(A, B, jj);
+      if (jj + 1 < (int64_t)j1) row(B, A, jj + 1);
+    }
+  } else {
+    RowIn<T, CPT> A, B, C;
+    fetch(A, j0);
+    fetch(B, j0 + 1);
+    for (int64_t jj = (int64_t)j0 - 1; jj < (int64_t)j1; jj += 3) {
+      row(A, C, jj);
+      if (jj + 1 < (int64_t)j1) row(B, A, jj + 1);
+      if (jj + 2 < (int64_t)j1) row(C, B, jj + 2);
+    }
   }
 }
 
@@ -392,7 +405,7 @@ struct shaw_fom {
   virtual void setState(const double* vp, const double* sp) = 0;
   virtual void time(uint64_t nsteps, double* total, double* perStep) = 0;
   virtual void bytes(double* kernel, double* model) const = 0;
-  virtual void config(int* kernel, int* cpt, int* strip, int* seg) const = 0;
+  virtual void config(int* kernel, int* cpt, int* strip, int* seg, int* pf) const = 0;
   virtual void pod(int dof, uint64_t k, const double* w, double* phi, double* sigma) = 0;
 };
 
@@ -429,6 +442,8 @@ class FomEngine final : public shaw_fom {
       P_ = o.strip > 0 ? (uint32_t)o.strip : std::max<uint32_t>(2, std::min<uint32_t>(64, 512 / tpp));
       SHAW_REQUIRE(P_ >= 2 && P_ * tpp <= 512, SHAW_ERR_ARG, "strip too wide for a 512-thread block");
       SEG_ = o.segment > 0 ? (uint32_t)o.segment : 32;
+      pf_ = o.prefetch_rows > 0 ? o.prefetch_rows : 1;
+      SHAW_REQUIRE(pf_ == 1 || pf_ == 2, SHAW_ERR_ARG, "prefetch_rows must be 1 or 2");
       nStrips_ = (uint32_t)((nth_ + (P_ - 1) - 1) / (P_ - 1));
       nSegs_ = (uint32_t)((nr_ + SEG_ - 1) / SEG_);
     }
@@ -621,11 +636,12 @@ class FomEngine final : public shaw_fom {
         for (uint64_t i = 0; i < n; ++i) phi[i + n * c] /= sw[i];
     s.ncols = 0;  // the buffer was consumed
   }
-  void config(int* kernel, int* cpt, int* strip, int* seg) const override {
+  void config(int* kernel, int* cpt, int* strip, int* seg, int* pf) const override {
     if (kernel) *kernel = kernel_;
     if (cpt) *cpt = cpt_;
     if (strip) *strip = (int)P_;
     if (seg) *seg = (int)SEG_;
+    if (pf) *pf = pf_;
   }
 
  private:
@@ -683,7 +699,8 @@ class FomEngine final : public shaw_fom {
     a.sOut = s_[cur_ ^ 1].p;
     const unsigned blocks = nStrips_ * nSegs_ * nb_;
     const size_t lds = (size_t)(2 * P_ + 1) * MC_ * sizeof(T);
-    hipLaunchKernelGGL((k_fused<T, CPT>), dim3(blocks), dim3(P_ * tpp), lds, stream_, a);
+    if (pf_ == 2) hipLaunchKernelGGL((k_fused<T, CPT, 2>), dim3(blocks), dim3(P_ * tpp), lds, stream_, a);
+    else hipLaunchKernelGGL((k_fused<T, CPT, 1>), dim3(blocks), dim3(P_ * tpp), lds, stream_, a);
     cur_ ^= 1;
     snapshot(0, v_[cur_].p);
     snapshot(1, s_[cur_].p);
@@ -726,7 +743,7 @@ class FomEngine final : public shaw_fom {
 
   uint64_t nr_ = 0, nth_ = 0, nVp_ = 0, nSp_ = 0, M_ = 0;
   uint32_t MC_ = 1, nb_ = 1, P_ = 0, SEG_ = 0, nStrips_ = 0, nSegs_ = 0;
-  int kernel_ = SHAW_KERNEL_FUSED, cpt_ = 1, rank1_ = 0;
+  int kernel_ = SHAW_KERNEL_FUSED, cpt_ = 1, rank1_ = 0, pf_ = 1;
   double dt_ = 0;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
@@ -825,8 +842,8 @@ int shaw_fom_time(shaw_fom_t h, uint64_t nsteps, double* total, double* perStep)
 int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel, double* model) {
   return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->bytes(kernel, model); });
 }
-int shaw_fom_config(shaw_fom_t h, int* kernel, int* cpt, int* strip, int* seg) {
-  return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->config(kernel, cpt, strip, seg); });
+int shaw_fom_config(shaw_fom_t h, int* kernel, int* cpt, int* strip, int* seg, int* pf) {
+  return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->config(kernel, cpt, strip, seg, pf); });
 }
 int shaw_fom_pod(shaw_fom_t h, int dof, uint64_t k, double* phi, double* sigma) {
   return shaw_fom_pod_weighted(h, dof, k, nullptr, phi, sigma);

@@ -176,7 +176,8 @@ void InputParser::parseAll(const std::string& inputFile) {
       const std::string p = r["precision"].str();
       if (p == "fp64") romPrecision = 0;
       else if (p == "bf16x3") romPrecision = 1;
-      else throw std::runtime_error("rom precision must be fp64 or bf16x3");
+      else if (p == "bf16x4") romPrecision = 2;
+      else throw std::runtime_error("rom precision must be fp64, bf16x3 or bf16x4");
     }
     if (vpBasisFile == "empty" || spBasisFile == "empty") {
       std::cout << "You did not set the basisfiles so I am using dummy basis" << std::endl;

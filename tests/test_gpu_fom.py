@@ -26,7 +26,8 @@ LAYERED = {"kind": "layered", "radius": [3480.0, 4100.0, 4700.0, 5200.0, 5600.0,
 
 KERNELS = [dict(kernel=shaw_amd.KERNEL_SPLIT), dict(kernel=shaw_amd.KERNEL_FUSED),
            dict(kernel=shaw_amd.KERNEL_FUSED, strip=8, segment=3),
-           dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=1, strip=5, segment=1)]
+           dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=1, strip=5, segment=1),
+           dict(kernel=shaw_amd.KERNEL_FUSED, prefetch_rows=2, strip=6, segment=4)]
 
 
 def say(msg):
@@ -54,7 +55,7 @@ def run_both(test, **kw):
         yield rid, len(s), fs is not None, a, b
 
 
-@pytest.mark.parametrize("kw", KERNELS, ids=["split", "fused", "fused-narrow", "fused-cpt1"])
+@pytest.mark.parametrize("kw", KERNELS, ids=["split", "fused", "fused-narrow", "fused-cpt1", "fused-pf2"])
 @pytest.mark.parametrize("test", FOM_TESTS)
 def test_regression_inputs_bit_exact(test, kw):
     say(f"[gpu] {test} {kw}")
@@ -86,6 +87,9 @@ def mesh64x256(tmp_path_factory):
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=8, strip=32, segment=7)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=2, strip=16, segment=64)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, prefetch_rows=2, strip=32, segment=32)),
+                                  (40, dict(kernel=shaw_amd.KERNEL_FUSED, prefetch_rows=2, cols_per_thread=1,
+                                            strip=16, segment=10)),
                                   (7, dict(kernel=shaw_amd.KERNEL_FUSED)),
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED)),
                                   (40, dict(kernel=shaw_amd.KERNEL_SPLIT, cols_per_thread=2))])
