@@ -225,7 +225,8 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out);
 /* Admit n columns into free slots: b [kv][n] (rho_inv[g] Phi_v[g,:] per
  * column), forcing values [nsteps][n], horizons (steps per column), sample
  * period `every`. slots_out[n] receives the slot ids. Fails with
- * SHAW_ERR_MEMORY when not enough slots are free.                          */
+ * SHAW_ERR_MEMORY when not enough slots are free. Waits for the engine's
+ * launches in flight (they write back every slot of their workgroups).    */
 int shaw_rom_admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const double* ftab,
                    const uint64_t* horizons, uint64_t every, uint64_t* slots_out);
 /* Advance every active column by up to nsteps (columns stop at their horizon). */
@@ -235,6 +236,12 @@ int shaw_rom_sync(shaw_rom_t h);
  * (any output may be NULL), then free the slot.                            */
 int shaw_rom_retire(shaw_rom_t h, uint64_t slot, double* seis, double* qv, double* qs,
                     uint64_t* samples);
+/* Retire n slots at once (one stream sync and one staged copy): seismograms
+ * into seis [n][r x max_samples] (each LayoutLeft [r x T_j], T_j samples
+ * stored, the rest untouched), T_j into samples (may be NULL). No counterpart
+ * in the reference; for serving, where a request retires many columns.     */
+int shaw_rom_retire_batch(shaw_rom_t h, uint64_t n, const uint64_t* slots, uint64_t max_samples,
+                          double* seis, uint64_t* samples);
 /* Current reduced states of an admitted column (kv and ks values) and its
  * step count; any output may be NULL.                                      */
 int shaw_rom_state(shaw_rom_t h, uint64_t slot, double* qv, double* qs, uint64_t* step);

@@ -116,6 +116,7 @@ def lib():
         "shaw_rom_advance": (i32, [vp, u64]),
         "shaw_rom_sync": (i32, [vp]),
         "shaw_rom_retire": (i32, [vp, u64, vp, vp, vp, P(u64)]),
+        "shaw_rom_retire_batch": (i32, [vp, u64, vp, u64, vp, vp]),
         "shaw_rom_free_slots": (i32, [vp, P(u64)]),
         "shaw_rom_state": (i32, [vp, u64, vp, vp, P(u64)]),
         "shaw_debug_mfma": (i32, [i32, vp, vp, vp]),
@@ -499,6 +500,22 @@ class Rom:
         _check(lib().shaw_rom_retire(self._h, slot, _p(seis), _p(qv), _p(qs), C.byref(t)))
         T = int(t.value)
         return seis[: self.R * T].reshape(T, self.R), qv, qs
+
+    def retire_many(self, slots, max_samples: int):
+        """Seismograms [T_j][R] of several slots, retired with one sync and one
+        staged copy (states are not returned)."""
+        sl = np.ascontiguousarray(slots, dtype=np.uint64)
+        n = sl.size
+        seis = np.empty(max(1, n * self.R * max(1, max_samples)))
+        t = np.zeros(max(1, n), dtype=np.uint64)
+        _check(lib().shaw_rom_retire_batch(self._h, n, sl.ctypes.data_as(C.c_void_p), max_samples, _p(seis),
+                                           t.ctypes.data_as(C.c_void_p)))
+        out = []
+        for j in range(n):
+            T = int(t[j])
+            blk = seis[j * self.R * max_samples: j * self.R * max_samples + self.R * T]
+            out.append(blk.reshape(T, self.R))
+        return out
 
     def free_slots(self) -> int:
         n = C.c_uint64()

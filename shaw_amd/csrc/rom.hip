@@ -34,6 +34,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "rom_internal.hpp"
@@ -602,6 +603,25 @@ __global__ void __launch_bounds__(kWaves * 64) k_rom_steps(RomArgs a) {
   }
 }
 
+// admission: column j of the staged inputs goes to slot slots[j]; its states
+// start at zero
+__global__ void k_admit_scatter(const uint64_t* __restrict__ slots, uint64_t n, const double* __restrict__ bStage,
+                                const double* __restrict__ fStage, uint32_t kvp, uint32_t ksp, uint64_t nsteps,
+                                uint64_t maxSteps, double* __restrict__ b, double* __restrict__ ftab,
+                                double* __restrict__ qv, double* __restrict__ qs) {
+  const uint64_t j = blockIdx.y;
+  if (j >= n) return;
+  const uint64_t s = slots[j];
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nsteps; e += (uint64_t)gridDim.x * blockDim.x)
+    ftab[s * maxSteps + e] = fStage[j * nsteps + e];
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < kvp; e += (uint64_t)gridDim.x * blockDim.x) {
+    b[s * kvp + e] = bStage[j * kvp + e];
+    qv[s * kvp + e] = 0.0;
+  }
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ksp; e += (uint64_t)gridDim.x * blockDim.x)
+    qs[s * ksp + e] = 0.0;
+}
+
 template <class T>
 struct DevBuf {
   T* p = nullptr;
@@ -791,6 +811,27 @@ void podDevice(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* phi, do
 }
 }  // namespace shaw
 
+namespace shaw {
+// host copy split over up to 8 threads (large retire copies are bound by a
+// single core's memcpy rate otherwise)
+void parallelCopy(void* dst, const void* src, size_t bytes) {
+  const size_t nt = bytes < (16u << 20) ? 1 : std::min<size_t>(8, bytes / (8u << 20));
+  if (nt <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t part = (bytes / nt + 4095) / 4096 * 4096;
+  for (size_t t = 0; t < nt; ++t) {
+    const size_t b0 = t * part;
+    if (b0 >= bytes) break;
+    const size_t len = std::min(part, bytes - b0);
+    th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + b0, static_cast<const char*>(src) + b0, len); });
+  }
+  for (auto& x : th) x.join();
+}
+}  // namespace shaw
+
 struct shaw_rom {
   uint64_t kv = 0, ks = 0, R = 0, maxCols = 0, maxSamples = 0, maxSteps = 0;
   uint32_t kvp = 0, ksp = 0, rp = 0;
@@ -803,8 +844,28 @@ struct shaw_rom {
   shaw::DevBuf<uint64_t> stepCol, horizon, every;
   std::vector<uint64_t> hStep, hHorizon, hEvery;
   std::vector<uint8_t> used;
+  shaw::DevBuf<double> stage;          // admission inputs, [n][Kvp] then [n][nsteps]
+  shaw::DevBuf<uint64_t> stageSlots;
+  void* pinned = nullptr;              // host staging of retired seismograms
+  size_t pinnedBytes = 0;
+  double* pinnedStage(size_t bytes) {
+    if (bytes > pinnedBytes) {
+      if (pinned) (void)hipHostFree(pinned);
+      pinned = nullptr;
+      pinnedBytes = 0;
+      SHAW_HIP(hipHostMalloc(&pinned, bytes));
+      pinnedBytes = bytes;
+    }
+    return static_cast<double*>(pinned);
+  }
+  void uploadSlotTables() {
+    stepCol.upload(hStep.data(), maxCols);
+    horizon.upload(hHorizon.data(), maxCols);
+    every.upload(hEvery.data(), maxCols);
+  }
   ~shaw_rom() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (pinned) (void)hipHostFree(pinned);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -962,6 +1023,7 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
 
 int shaw_rom_admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const double* ftab, const uint64_t* horizons,
                    uint64_t every, uint64_t* slots) {
+  using namespace shaw;
   return guarded([&] {
     SHAW_REQUIRE(h && b && ftab && horizons && slots, SHAW_ERR_ARG, "null argument");
     SHAW_REQUIRE(h->R == 0 || every > 0, SHAW_ERR_ARG, "sampling period must be > 0 when receivers are set");
@@ -974,9 +1036,13 @@ int shaw_rom_admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, c
       SHAW_REQUIRE(horizons[j] < (1ull << 31), SHAW_ERR_ARG, "horizon must be below 2^31 steps");
       if (h->R) SHAW_REQUIRE(horizons[j] / every <= h->maxSamples, SHAW_ERR_ARG, "horizon exceeds the seismogram capacity");
     }
+    if (n == 0) return;
+    // a launch still in flight reads and writes back every slot of its
+    // workgroups: admit only on an idle engine
+    SHAW_HIP(hipStreamSynchronize(h->stream));
     const uint64_t need = std::max<uint64_t>(h->maxSteps, nsteps);
     if (need > h->maxSteps) {  // grow the per-slot forcing store, keeping admitted columns
-      shaw::DevBuf<double> nb;
+      DevBuf<double> nb;
       nb.alloc(h->maxCols * need);
       SHAW_HIP(hipMemset(nb.p, 0, nb.n * 8));
       if (h->ftab.p && h->maxSteps)
@@ -985,25 +1051,30 @@ int shaw_rom_admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, c
       std::swap(nb.n, h->ftab.n);
       h->maxSteps = need;
     }
-    std::vector<double> col(h->kvp), f(nsteps);
-    const std::vector<double> zv(std::max(h->kvp, h->ksp), 0.0);
+    // stage [n][Kvp] forcing columns and [n][nsteps] forcing series, one copy, one scatter
+    std::vector<double> st(n * (h->kvp + nsteps), 0.0);
+    for (uint64_t j = 0; j < n; ++j) {
+      for (uint64_t k = 0; k < h->kv; ++k) st[j * h->kvp + k] = b[k * n + j];
+      double* f = st.data() + n * h->kvp + j * nsteps;
+      for (uint64_t t = 0; t < nsteps; ++t) f[t] = ftab[t * n + j];
+    }
+    if (h->stage.n < st.size()) h->stage.alloc(st.size());
+    h->stage.upload(st.data(), st.size());
+    if (h->stageSlots.n < n) h->stageSlots.alloc(n);
+    h->stageSlots.upload(free.data(), n);
+    hipLaunchKernelGGL(k_admit_scatter, dim3(64, (unsigned)n), dim3(256), 0, h->stream, h->stageSlots.p, n, h->stage.p,
+                       h->stage.p + n * h->kvp, h->kvp, h->ksp, nsteps, h->maxSteps, h->b.p, h->ftab.p, h->qv.p, h->qs.p);
+    SHAW_HIP(hipGetLastError());
     for (uint64_t j = 0; j < n; ++j) {
       const uint64_t s = free[j];
-      for (uint64_t k = 0; k < h->kvp; ++k) col[k] = k < h->kv ? b[k * n + j] : 0.0;
-      h->b.upload(col.data(), h->kvp, s * h->kvp);
-      for (uint64_t t = 0; t < nsteps; ++t) f[t] = ftab[t * n + j];
-      h->ftab.upload(f.data(), nsteps, s * h->maxSteps);
-      h->qv.upload(zv.data(), h->kvp, s * h->kvp);
-      h->qs.upload(zv.data(), h->ksp, s * h->ksp);
       h->hStep[s] = 0;
       h->hHorizon[s] = horizons[j];
       h->hEvery[s] = h->R ? every : 0;
       h->used[s] = 1;
       slots[j] = s;
     }
-    h->stepCol.upload(h->hStep.data(), h->maxCols);
-    h->horizon.upload(h->hHorizon.data(), h->maxCols);
-    h->every.upload(h->hEvery.data(), h->maxCols);
+    SHAW_HIP(hipStreamSynchronize(h->stream));
+    h->uploadSlotTables();
   });
 }
 
@@ -1028,6 +1099,44 @@ int shaw_rom_state(shaw_rom_t h, uint64_t slot, double* qv, double* qs, uint64_t
     if (qv) SHAW_HIP(hipMemcpy(qv, h->qv.p + slot * h->kvp, h->kv * 8, hipMemcpyDeviceToHost));
     if (qs) SHAW_HIP(hipMemcpy(qs, h->qs.p + slot * h->ksp, h->ks * 8, hipMemcpyDeviceToHost));
     if (step) *step = h->hStep[slot];
+  });
+}
+
+int shaw_rom_retire_batch(shaw_rom_t h, uint64_t n, const uint64_t* slots, uint64_t max_samples, double* seis,
+                          uint64_t* samples) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && (n == 0 || slots), SHAW_ERR_ARG, "null argument");
+    for (uint64_t j = 0; j < n; ++j)
+      SHAW_REQUIRE(slots[j] < h->maxCols && h->used[slots[j]], SHAW_ERR_ARG, "bad slot");
+    SHAW_HIP(hipStreamSynchronize(h->stream));
+    if (seis && h->R && max_samples) {
+      // chunks of slots through a pinned buffer of at most 256 MB: async
+      // copies of a chunk, one sync, then a threaded host copy
+      const uint64_t per = h->R * max_samples;
+      const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(n, (256ull << 20) / (per * sizeof(double))));
+      double* st = h->pinnedStage(chunk * per * sizeof(double));
+      for (uint64_t j0 = 0; j0 < n; j0 += chunk) {
+        const uint64_t j1 = std::min(n, j0 + chunk);
+        for (uint64_t j = j0; j < j1; ++j) {
+          const uint64_t s = slots[j];
+          const uint64_t T = std::min<uint64_t>(h->hEvery[s] ? h->hStep[s] / h->hEvery[s] : 0, max_samples);
+          if (T)
+            SHAW_HIP(hipMemcpyAsync(st + (j - j0) * per, h->seis.p + s * h->R * h->maxSamples, h->R * T * 8,
+                                    hipMemcpyDeviceToHost, h->stream));
+        }
+        SHAW_HIP(hipStreamSynchronize(h->stream));
+        shaw::parallelCopy(seis + j0 * per, st, (j1 - j0) * per * sizeof(double));
+      }
+    }
+    for (uint64_t j = 0; j < n; ++j) {
+      const uint64_t s = slots[j];
+      if (samples) samples[j] = h->hEvery[s] ? std::min<uint64_t>(h->hStep[s] / h->hEvery[s], max_samples) : 0;
+      h->used[s] = 0;
+      h->hHorizon[s] = 0;
+      h->hStep[s] = 0;
+      h->hEvery[s] = 0;
+    }
+    h->uploadSlotTables();
   });
 }
 

@@ -63,7 +63,7 @@ def column_bytes(kv: int, ks: int, horizon: int, every: int, r: int) -> int:
 class Server:
     def __init__(self, engines: dict, hbm_budget: float, chunk: int = 500, clock=time.perf_counter):
         """engines: K -> engine with .admit(b, ftab, horizons, every) -> slots,
-        .advance(n), .sync(), .retire(slot, max_samples) -> (seis, qv, qs),
+        .advance(n), .sync(), .retire_many(slots, max_samples) -> [seis],
         .free_slots(), .kv, .ks, .R."""
         self.engines = engines
         self.budget = hbm_budget
@@ -131,7 +131,7 @@ class Server:
                 continue
             e = self.engines[req.k]
             T = req.horizon // req.every if req.every else 0
-            req.seismograms = [e.retire(s, T)[0] for s in req.slots]
+            req.seismograms = e.retire_many(req.slots, T)
             req.done = self.now()
             self.resident_bytes -= self.footprint(req)
             self.finished.append(req)
@@ -210,6 +210,9 @@ class GpuEngine:
     def retire(self, slot, max_samples):
         return self.rom.retire(slot, max_samples)
 
+    def retire_many(self, slots, max_samples):
+        return self.rom.retire_many(slots, max_samples)
+
     def free_slots(self):
         return self.rom.free_slots()
 
@@ -258,3 +261,6 @@ class NumpyEngine:
         c = self.cols.pop(slot)
         seis = np.array(c["seis"]).reshape(-1, self.R) if self.R else np.zeros((0, 0))
         return seis, c["qv"], c["qs"]
+
+    def retire_many(self, slots, max_samples):
+        return [self.retire(s, max_samples)[0] for s in slots]
