@@ -247,6 +247,59 @@ def serving_section(app, w, basis, args) -> dict:
     return rep
 
 
+def config5_section(args) -> dict:
+    """Config 5: the large FOM, 2048 x 8192, M = 64 seeded forcings,
+    dt = 0.025 s. FP64 and FP32 states, each with the autotuned tile shape,
+    timed over --steps steps after the warmup (states in HBM; the full run is
+    40000 steps); the FP32 state's seismogram error against FP64 over the
+    first --c5-err-steps steps (180 receivers every 10 steps)."""
+    import shaw_amd
+    from shaw_amd import workloads
+
+    w = workloads.config5()
+    t0 = time.time()
+    app = shaw_amd.App.structured(w.nr, w.nth, workloads.layered_material())
+    gids = [app.map_source(d, a) for d, a, p, dl in w.sources]
+    rec = [app.map_receiver(a) for a in workloads.RECEIVERS]
+    n_err = args.c5_err_steps
+    n_tab = max(n_err, args.warmup + args.steps + 1)
+    tab = shaw_amd.forcing_table(["ricker"] * w.m, [s[3] for s in w.sources], [s[2] for s in w.sources], w.dt, n_tab)
+    log(f"config5 setup {time.time() - t0:.1f} s")
+    out, seis = {}, {}
+    for prec, name in ((shaw_amd.FP64, "fp64"), (shaw_amd.FP32, "fp32")):
+        fom = shaw_amd.Fom(app, w.m, w.dt, precision=prec, rank1_forcing=False)
+        fom.set_sources(gids)
+        fom.set_forcing(tab)
+        fom.set_receivers(rec, w.seis_every, n_tab // w.seis_every)
+        tuned = fom.autotune(3)
+        cfg = fom.config()
+        fom.advance(args.warmup)
+        fom.sync()
+        total, per = fom.time(args.steps)
+        kb, mb = fom.bytes_per_step()
+        out[name] = {"ms_per_step": per, "forcing_steps_per_s": w.m / (per / 1e3),
+                     "algorithmic_GB_s": kb / (per / 1e3) / 1e9, "hbm_frac": kb / (per / 1e3) / 1e9 / HBM_PEAK_GBS,
+                     "bytes_per_step": kb, "autotuned": {"config": cfg, "trial_ms_per_step": tuned}}
+        log(f"config5 {name}: {per:.3f} ms/step, {kb / (per / 1e3) / 1e9:.0f} GB/s, config {cfg}")
+        fom.close()
+        # error run from zero state over the first n_err steps
+        fom = shaw_amd.Fom(app, w.m, w.dt, precision=prec, rank1_forcing=False, cols_per_thread=cfg["cols_per_thread"],
+                           strip=cfg["strip"], segment=cfg["segment"], prefetch_rows=cfg["prefetch_rows"])
+        fom.set_sources(gids)
+        fom.set_forcing(tab)
+        fom.set_receivers(rec, w.seis_every, n_err // w.seis_every)
+        fom.advance(n_err)
+        fom.sync()
+        seis[name] = fom.seismogram()
+        fom.close()
+    err = [float(np.linalg.norm(seis["fp32"][m] - seis["fp64"][m]) / np.linalg.norm(seis["fp64"][m]))
+           for m in range(w.m)]
+    out["fp32_vs_fp64_seismogram_rel_l2"] = {"max": max(err), "median": float(np.median(err)),
+                                             "steps": n_err, "sim_time_s": n_err * w.dt}
+    out["workload"] = w.describe() + ", 180 receivers every 10 steps"
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +317,8 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--rom-steps", type=int, default=20000)
     ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--config5", action="store_true", help="also measure the large FOM (2048 x 8192, M = 64)")
+    ap.add_argument("--c5-err-steps", type=int, default=8000)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -340,6 +395,8 @@ def main() -> int:
         out["rom_at_err_1e-3"], basis = rom_err_section(app, w, value, args)
     if not args.no_serve:
         out["serving"] = serving_section(app, w, basis, args)
+    if args.config5:
+        out["fom_config5"] = config5_section(args)
     if not args.no_cpu:
         log(f"cpu baseline: oracle, {args.cpu_steps} steps")
         out["cpu_baseline"] = cpu_baseline(w, material, args.cpu_steps)

@@ -168,6 +168,12 @@ int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel_bytes, double* model_by
 /* Kernel configuration actually used: kernel, cols_per_thread, strip, segment. */
 int shaw_fom_config(shaw_fom_t h, int* kernel, int* cols_per_thread, int* strip, int* segment,
                     int* prefetch_rows);
+/* Tile autotuner of the fused kernel (no counterpart in the reference, whose
+ * launch shapes are fixed Kokkos policies): times every valid (columns per
+ * thread, strip, segment, rows prefetched) for trial_steps steps on the
+ * current state, keeps the fastest (read it back with shaw_fom_config) and
+ * restores the state. Needs the forcing table to cover the next step.     */
+int shaw_fom_autotune(shaw_fom_t h, uint64_t trial_steps, double* best_ms_per_step);
 /* POD basis of the device snapshot buffer of `dof` (all configured columns,
  * all samples: LayoutLeft [n x (T * ncols)]) without a host round trip, as
  * shaw_pod_basis; the snapshot buffer is consumed. FP64 FOM only.         */

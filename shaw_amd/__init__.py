@@ -104,6 +104,7 @@ def lib():
         "shaw_fom_time": (i32, [vp, u64, P(dbl), P(dbl)]),
         "shaw_fom_bytes_per_step": (i32, [vp, P(dbl), P(dbl)]),
         "shaw_fom_config": (i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i32)]),
+        "shaw_fom_autotune": (i32, [vp, u64, P(C.c_double)]),
         "shaw_fom_destroy": (None, [vp]),
         "shaw_fom_pod": (i32, [vp, i32, u64, vp, vp]),
         "shaw_fom_pod_weighted": (i32, [vp, i32, u64, vp, vp, vp]),
@@ -376,6 +377,13 @@ class Fom:
             _check(lib().shaw_fom_pod_weighted(self._h, d, k, _p(w), _p(phi), _p(sig)))
         return phi.T, sig
 
+    def autotune(self, trial_steps: int = 3) -> float:
+        """Pick the fastest fused-kernel tile shape (state unchanged); returns
+        its ms per step."""
+        ms = C.c_double()
+        _check(lib().shaw_fom_autotune(self._h, trial_steps, C.byref(ms)))
+        return ms.value
+
     def config(self):
         v = [C.c_int() for _ in range(5)]
         _check(lib().shaw_fom_config(self._h, *[C.byref(x) for x in v]))
@@ -392,6 +400,7 @@ def fom_run(app: App, dt: float, nsteps: int, sources, rank2: bool, receivers=()
     if not rank2 and m != 1:
         raise ShawError("rank-1 run needs one source")
     rank1_forcing = fom_kw.pop("rank1_forcing", not rank2)
+    tune = fom_kw.pop("autotune", 0)
     fom = Fom(app, m, dt, rank1_forcing=rank1_forcing, **fom_kw)
     fom.set_sources([app.map_source(d, a) for d, a, p, dl in sources])
     fom.set_forcing(forcing_table([signal] * m, [s[3] for s in sources], [s[2] for s in sources], dt, nsteps))
@@ -402,6 +411,8 @@ def fom_run(app: App, dt: float, nsteps: int, sources, rank2: bool, receivers=()
         fom.set_snapshots("vp", snap_freq_vp, cols, nsteps // snap_freq_vp)
     if snap_freq_sp:
         fom.set_snapshots("sp", snap_freq_sp, cols, nsteps // snap_freq_sp)
+    if tune:
+        fom.autotune(tune)
     fom.advance(nsteps)
     fom.sync()
     vp, sp = fom.state()

@@ -407,6 +407,7 @@ struct shaw_fom {
   virtual void bytes(double* kernel, double* model) const = 0;
   virtual void config(int* kernel, int* cpt, int* strip, int* seg, int* pf) const = 0;
   virtual void pod(int dof, uint64_t k, const double* w, double* phi, double* sigma) = 0;
+  virtual void autotune(uint64_t trialSteps, double* bestMs) = 0;
 };
 
 namespace shaw {
@@ -690,8 +691,71 @@ class FomEngine final : public shaw_fom {
     hipLaunchKernelGGL((k_stress<T, CPT>), gs, dim3(256), 0, stream_, a);
     snapshot(1, a.sOut);
   }
+  // Tile autotuner of the fused kernel: every valid (columns per thread,
+  // strip, segment, rows prefetched) is timed for `trialSteps` steps on the
+  // current state (after one untimed step), the fastest is kept, and the
+  // state is restored: the trials do not advance the run, record receivers
+  // or take snapshots.
+  void autotune(uint64_t trialSteps, double* bestMs) override {
+    SHAW_REQUIRE(kernel_ == SHAW_KERNEL_FUSED, SHAW_ERR_STATE, "the autotuner tunes the fused kernel");
+    SHAW_REQUIRE(trialSteps >= 1 && trialSteps <= 1000, SHAW_ERR_ARG, "trial steps must be in 1..1000");
+    SHAW_REQUIRE(haveSources_ && ftab_.p && step_ + 1 <= ftabSteps_, SHAW_ERR_STATE,
+                 "set sources and a forcing table covering the next step before tuning");
+    sync();
+    const int cur0 = cur_;
+    DevBuf<T> sv, ss;
+    sv.alloc(v_[cur0].n);
+    ss.alloc(s_[cur0].n);
+    SHAW_HIP(hipMemcpy(sv.p, v_[cur0].p, sv.n * sizeof(T), hipMemcpyDeviceToDevice));
+    SHAW_HIP(hipMemcpy(ss.p, s_[cur0].p, ss.n * sizeof(T), hipMemcpyDeviceToDevice));
+    struct Cfg { int cpt; uint32_t P, SEG; int pf; };
+    const Cfg start{cpt_, P_, SEG_, pf_};
+    Cfg best = start;
+    double bestT = 1e300;
+    auto apply = [&](const Cfg& c) {
+      cpt_ = c.cpt; P_ = c.P; SEG_ = c.SEG; pf_ = c.pf;
+      nStrips_ = (uint32_t)((nth_ + (P_ - 1) - 1) / (P_ - 1));
+      nSegs_ = (uint32_t)((nr_ + SEG_ - 1) / SEG_);
+    };
+    for (int cpt : {1, 2, 4, 8}) {
+      if (MC_ % cpt) continue;
+      const uint32_t tpp = MC_ / cpt;
+      for (uint32_t P : {8u, 16u, 32u, 64u}) {
+        if (P * tpp > 512 || P * tpp < 64) continue;
+        for (uint32_t SEG : {16u, 32u, 64u})
+          for (int pf : {1, 2}) {
+            const Cfg c{cpt, P, SEG, pf};
+            apply(c);
+            trialStep();
+            SHAW_HIP(hipEventRecord(ev0_, stream_));
+            for (uint64_t k = 0; k < trialSteps; ++k) trialStep();
+            SHAW_HIP(hipEventRecord(ev1_, stream_));
+            SHAW_HIP(hipEventSynchronize(ev1_));
+            float ms = 0;
+            SHAW_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+            if (ms / trialSteps < bestT) { bestT = ms / trialSteps; best = c; }
+          }
+      }
+    }
+    apply(best);
+    SHAW_HIP(hipMemcpy(v_[cur0].p, sv.p, sv.n * sizeof(T), hipMemcpyDeviceToDevice));
+    SHAW_HIP(hipMemcpy(s_[cur0].p, ss.p, ss.n * sizeof(T), hipMemcpyDeviceToDevice));
+    cur_ = cur0;
+    if (bestMs) *bestMs = bestT;
+  }
+  void trialStep() {
+    StepArgs<T> a = args(step_ + 1);
+    a.recv = nullptr;
+    switch (cpt_) {
+      case 1: launchFused<1>(a, false); break;
+      case 2: launchFused<2>(a, false); break;
+      case 4: launchFused<4>(a, false); break;
+      default: launchFused<8>(a, false); break;
+    }
+    SHAW_HIP(hipGetLastError());
+  }
   template <int CPT>
-  void launchFused(StepArgs<T>& a) {
+  void launchFused(StepArgs<T>& a, bool observe = true) {
     const uint32_t tpp = MC_ / CPT;
     a.vIn = v_[cur_].p;
     a.sIn = s_[cur_].p;
@@ -702,8 +766,10 @@ class FomEngine final : public shaw_fom {
     if (pf_ == 2) hipLaunchKernelGGL((k_fused<T, CPT, 2>), dim3(blocks), dim3(P_ * tpp), lds, stream_, a);
     else hipLaunchKernelGGL((k_fused<T, CPT, 1>), dim3(blocks), dim3(P_ * tpp), lds, stream_, a);
     cur_ ^= 1;
-    snapshot(0, v_[cur_].p);
-    snapshot(1, s_[cur_].p);
+    if (observe) {
+      snapshot(0, v_[cur_].p);
+      snapshot(1, s_[cur_].p);
+    }
   }
   void launchStep() {
     const uint64_t stepNo = step_ + 1;
@@ -844,6 +910,12 @@ int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel, double* model) {
 }
 int shaw_fom_config(shaw_fom_t h, int* kernel, int* cpt, int* strip, int* seg, int* pf) {
   return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->config(kernel, cpt, strip, seg, pf); });
+}
+int shaw_fom_autotune(shaw_fom_t h, uint64_t trial_steps, double* best_ms_per_step) {
+  return guarded([&] {
+    SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle");
+    h->autotune(trial_steps, best_ms_per_step);
+  });
 }
 int shaw_fom_pod(shaw_fom_t h, int dof, uint64_t k, double* phi, double* sigma) {
   return shaw_fom_pod_weighted(h, dof, k, nullptr, phi, sigma);

@@ -108,6 +108,35 @@ def test_random_rank2_bit_exact(mesh64x256, M, kw):
     assert np.abs(a["seismogram"]).max() > 0
 
 
+def test_autotune_keeps_state_and_parity(mesh64x256):
+    """The tile autotuner times candidate shapes on the live state and must
+    leave it unchanged; the run it tunes stays bit-identical to the oracle."""
+    say("[gpu] autotune")
+    rng = np.random.default_rng(77)
+    app = shaw_amd.App.from_mesh(mesh64x256, LAYERED)
+    ora = oracle.App(mesh64x256, LAYERED)
+    src = random_sources(rng, 24)
+    dt, n = 0.5, 90
+    fom = shaw_amd.Fom(app, 24, dt)
+    fom.set_sources([app.map_source(d, a) for d, a, p, dl in src])
+    fom.set_forcing(shaw_amd.forcing_table(["ricker"] * 24, [s[3] for s in src], [s[2] for s in src], dt, n))
+    fom.advance(40)
+    before = fom.state()
+    ms = fom.autotune(2)
+    after = fom.state()
+    for x, y in zip(before, after):
+        np.testing.assert_array_equal(x, y)
+    cfg = fom.config()
+    assert ms > 0 and cfg["cols_per_thread"] in (1, 2, 4, 8) and cfg["prefetch_rows"] in (1, 2)
+    assert cfg["segment"] in (16, 32, 64) and cfg["strip"] in (8, 16, 32, 64)
+    fom.close()
+    rec = list(np.arange(0, 180, 7.5))
+    a = shaw_amd.fom_run(app, dt, n, src, True, rec, 3, 40, 30, autotune=2)
+    b = ora.fom_run(dt, n, src, True, rec, 3, 40, 30)
+    for key in b:
+        np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+
+
 def test_config1_shape_rank1_bit_exact(tmp_path):
     """The config-1 workload shape (200 x 1000, layered mantle, Ricker 640 km /
     45 s at theta=0, dt=0.25, 180 surface receivers every step) for a
