@@ -127,6 +127,9 @@ typedef struct {
   int strip;            /* 0 = default; fused kernel: vp points per block row (incl. halo) */
   int segment;          /* 0 = default; fused kernel: radial rows per block */
   int prefetch_rows;    /* 0 = default; fused kernel: rows loaded ahead, 1 or 2 */
+  int steps_per_launch; /* 0 = default (1); fused kernel: 1, or 2 (two leapfrog steps per
+                           launch, level n+1 kept on chip; snapshots at the intermediate
+                           step fall back to single steps)                          */
 } shaw_fom_options;
 
 /* app: operators; m: forcing columns; dt: time step. opts may be NULL. */
@@ -167,10 +170,11 @@ int shaw_fom_time(shaw_fom_t h, uint64_t nsteps, double* total_ms, double* ms_pe
 int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel_bytes, double* model_bytes);
 /* Kernel configuration actually used: kernel, cols_per_thread, strip, segment. */
 int shaw_fom_config(shaw_fom_t h, int* kernel, int* cols_per_thread, int* strip, int* segment,
-                    int* prefetch_rows);
+                    int* prefetch_rows, int* steps_per_launch);
 /* Tile autotuner of the fused kernel (no counterpart in the reference, whose
  * launch shapes are fixed Kokkos policies): times every valid (columns per
- * thread, strip, segment, rows prefetched) for trial_steps steps on the
+ * thread, strip, segment, rows prefetched, steps per launch) for
+ * trial_steps launches on the
  * current state, keeps the fastest (read it back with shaw_fom_config) and
  * restores the state. Needs the forcing table to cover the next step.     */
 int shaw_fom_autotune(shaw_fom_t h, uint64_t trial_steps, double* best_ms_per_step);

@@ -340,6 +340,204 @@ __global__ void __launch_bounds__(512) k_fused(StepArgs<T> a) {
   }
 }
 
+// ------------------------------------------------------------ two fused steps
+// One launch advances two leapfrog steps, n -> n+1 -> n+2. Level n+1 lives
+// only in registers and LDS, so the states are read and written once per two
+// steps. A block owns P-3 theta points of a strip (lanes: 1 west halo point,
+// P-3 owned points, 2 east halo points) and SEG radial rows. Marching up
+// input row r it computes, in order:
+//   A  v1(r)       = v^n(r) + dt J_vp s^n + f(step n+1)
+//   B  s1(row r-1) = s^n + dt J_sp v1          (srp: rows r-1, r of v1; stp: lanes i, i+1)
+//   C  v2(r-1)     = v1(r-1) + dt J_vp s1 + f(step n+2)
+//   D  s2(row r-2) = s1 + dt J_sp v2
+// Every value is the expression the one-step kernel evaluates, in the same
+// order, so two fused steps are bit-identical to two single steps. Rows
+// j0-1 .. j1+1 of level n+1 and row j1 of level n+2 are recomputed halos.
+template <class T>
+struct StepArgs2 {
+  StepArgs<T> s1, s2;  // the two steps' arguments (forcing rows, receiver sampling)
+};
+
+template <class T, int CPT>
+__global__ void __launch_bounds__(512) k_fused2(StepArgs2<T> aa) {
+  const StepArgs<T>& a = aa.s1;
+  extern __shared__ unsigned char smem_raw[];
+  T* stp0Row = reinterpret_cast<T*>(smem_raw);    // [P+1][MC]: s^n stp of row r (lane pl at pl+1, west load at 0)
+  T* v1Row = stp0Row + (a.P + 1) * a.MC;          // [P][MC]: v1 of row r-1 (then r)
+  T* stp1Row = v1Row + a.P * a.MC;                // [P+1][MC]: s1 stp of row r-1 (lane pl at pl+1)
+  T* v2Row = stp1Row + (a.P + 1) * a.MC;          // [P][MC]: v2 of row r-2 (then r-1)
+
+  const unsigned lb = xcdLogicalBlock(blockIdx.x, gridDim.x);
+  const uint32_t strip = lb % a.nStrips;
+  const uint32_t seg = (lb / a.nStrips) % a.nSegs;
+  const uint32_t cb = lb / (a.nStrips * a.nSegs);
+
+  const uint32_t tpp = a.MC / CPT;
+  const uint32_t pl = threadIdx.x / tpp;
+  const uint32_t m0 = (threadIdx.x % tpp) * CPT;
+  const uint64_t nr = a.nr, nth = a.nth, rs = 2 * nth - 1;
+  const uint64_t nVp = nr * nth, nSp = (nr - 1) * nth + nr * (nth - 1);
+  const uint32_t Q = a.P - 3;  // owned points per strip
+  const int64_t ii = (int64_t)strip * Q - 1 + pl;
+  const bool active = ii >= 0 && ii < (int64_t)nth;
+  const uint64_t i = active ? (uint64_t)ii : 0;
+  const bool owner = active && pl >= 1 && pl <= Q;
+  const bool doStp1 = active && pl + 1 < a.P && i + 1 < nth;  // s1 stp(i) needs v1(i+1)
+  const bool doV2 = active && pl >= 1 && pl + 1 < a.P;        // needs s1 stp(i-1) of lane pl-1
+  const uint64_t j0 = (uint64_t)seg * a.SEG;
+  const uint64_t j1 = j0 + a.SEG < nr ? j0 + a.SEG : nr;
+  const uint32_t colBase = cb * a.MC + m0;
+
+  const T* vIn = a.vIn + (uint64_t)cb * nVp * a.MC + m0;
+  T* vOut = a.vOut + (uint64_t)cb * nVp * a.MC + m0;
+  const T* sIn = a.sIn + (uint64_t)cb * nSp * a.MC + m0;
+  T* sOut = a.sOut + (uint64_t)cb * nSp * a.MC + m0;
+
+  // clamped, branch-free loads as in the one-step kernel
+  const uint64_t ic = active ? i : (ii < 0 ? 0 : nth - 1);
+  const uint64_t iE = ic < nth - 2 ? ic : nth - 2;
+  const uint64_t iW = pl == 0 ? (ic > 0 ? ic - 1 : 0) : iE;
+  auto fetch = [&](RowIn<T, CPT>& d, uint64_t r) {
+    const uint64_t rc = r < nr ? r : nr - 1;
+    const uint64_t p = rc * nth + ic;
+    load(d.N, sIn + srpIdx(ic, rc < nr - 1 ? rc : nr - 2, rs) * a.MC);
+    load(d.E, sIn + stpIdx(iE, rc, rs, nr) * a.MC);
+    load(d.vo, vIn + p * a.MC);
+    load(d.W, sIn + stpIdx(iW, rc, rs, nr) * a.MC);
+    load(d.c, a.cv + 4 * p);
+    const uint64_t rb = r == 0 ? 0 : (r - 1 < nr ? r - 1 : nr - 1);
+    load(d.bS, a.cs + 2 * srpIdx(ic, rb < nr - 1 ? rb : nr - 2, rs));
+    load(d.bT, a.cs + 2 * stpIdx(iE, rb, rs, nr));
+  };
+
+  ColumnForcing<T, CPT> frc1, frc2;
+  frc1.load(aa.s1, colBase);
+  frc2.load(aa.s2, colBase);
+  // carried values: level n at rows r-1 (S0: srp, E0: stp), v1 at row r-1,
+  // level n+1 at row r-2 (S1: srp, E1: stp), v2 at row r-2, and the J_vp row
+  // r-1 and J_sp rows r-2 the second step needs
+  T S0[CPT], E0[CPT], vj1[CPT], S1[CPT], E1[CPT], vj2[CPT];
+  T cPrev[4] = {T(0), T(0), T(0), T(0)}, bSPrev[2] = {T(0), T(0)}, bTPrev[2] = {T(0), T(0)};
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) { E0[k] = vj1[k] = S1[k] = E1[k] = vj2[k] = T(0); }
+  // the first input row is j0 - 1 (or 0): S0 is srp of the row below it
+  const uint64_t rFirst = j0 > 0 ? j0 - 1 : 0;
+  load(S0, sIn + srpIdx(ic, rFirst > 0 ? (rFirst - 1 < nr - 1 ? rFirst - 1 : nr - 2) : 0, rs) * a.MC);
+  if (pl == 0) {
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) stp1Row[m0 + k] = T(0);
+  }
+
+  auto row = [&](const RowIn<T, CPT>& cur, RowIn<T, CPT>& nxt, uint64_t r) {
+    const bool haveRow = r < nr;
+    store(stp0Row + (pl + 1) * a.MC + m0, cur.E);
+    if (pl == 0) store(stp0Row + m0, cur.W);
+    fetch(nxt, r + 1);
+    __syncthreads();  // (1) s^n stp of row r visible
+    // A: v1(r), every lane (halo points included)
+    T v1[CPT];
+    if (haveRow) {
+      T W[CPT];
+      load(W, stp0Row + pl * a.MC + m0);
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        T sum = cur.c[0] * cur.N[k] + cur.c[1] * S0[k];
+        sum = sum + cur.c[2] * W[k];
+        sum = sum + cur.c[3] * cur.E[k];
+        v1[k] = cur.vo[k] + a.dt * sum;
+      }
+      frc1.apply(v1, r * nth + i);
+      if (owner && r >= j0 && r < j1 && (a.recvRow < 0 || (int64_t)r == a.recvRow))
+        recordReceivers<T, CPT>(aa.s1, v1, r * nth + i, colBase);
+    } else {
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) v1[k] = T(0);
+    }
+    // B: s1 of row r-1 (srp needs v1 rows r-1 and r; stp needs v1(i+1, r-1) from LDS)
+    T srp1[CPT], stp1[CPT];
+    const bool rowB = r >= 1 && r - 1 >= rFirst && r - 1 < nr;
+    const uint64_t rB = r - 1;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) { srp1[k] = S1[k]; stp1[k] = T(0); }
+    if (rowB) {
+      if (rB + 1 < nr) {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) srp1[k] = S0[k] + a.dt * (cur.bS[0] * v1[k] + cur.bS[1] * vj1[k]);
+      }
+      if (doStp1) {
+        T vE[CPT];
+        load(vE, v1Row + (pl + 1) * a.MC + m0);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) stp1[k] = E0[k] + a.dt * (cur.bT[0] * vj1[k] + cur.bT[1] * vE[k]);
+      }
+    }
+    store(stp1Row + (pl + 1) * a.MC + m0, stp1);
+    __syncthreads();  // (2) s1 stp of row r-1 visible
+    // C: v2(r-1), lanes with both stp neighbours
+    T v2[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) v2[k] = T(0);
+    const bool rowC = r >= 1 && r - 1 >= j0 && r - 1 <= j1 && r - 1 < nr;
+    if (rowC && doV2) {
+      const uint64_t rc = r - 1;
+      T W[CPT];
+      load(W, stp1Row + pl * a.MC + m0);
+      // north srp of row nr-1 is srp(nr-2) (the merged surface slot), which is S1
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) {
+        const T Nk = rc + 1 < nr ? srp1[k] : S1[k];
+        T sum = cPrev[0] * Nk + cPrev[1] * S1[k];
+        sum = sum + cPrev[2] * W[k];
+        sum = sum + cPrev[3] * stp1[k];
+        v2[k] = vj1[k] + a.dt * sum;
+      }
+      frc2.apply(v2, rc * nth + i);
+      if (owner && rc < j1) {
+        store(vOut + (rc * nth + i) * a.MC, v2);
+        if (a.recvRow < 0 || (int64_t)rc == a.recvRow) recordReceivers<T, CPT>(aa.s2, v2, rc * nth + i, colBase);
+      }
+    }
+    // D: s2 of row r-2, owned points (srp needs v2 rows r-2, r-1; stp needs v2(i+1, r-2) from LDS)
+    if (owner && r >= 2 && r - 2 >= j0 && r - 2 < j1) {
+      const uint64_t rd = r - 2;
+      if (rd + 1 < nr) {
+        T so[CPT];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) so[k] = S1[k] + a.dt * (bSPrev[0] * v2[k] + bSPrev[1] * vj2[k]);
+        store(sOut + srpIdx(i, rd, rs) * a.MC, so);
+      }
+      if (i + 1 < nth) {
+        T vE[CPT], so[CPT];
+        load(vE, v2Row + (pl + 1) * a.MC + m0);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) so[k] = E1[k] + a.dt * (bTPrev[0] * vj2[k] + bTPrev[1] * vE[k]);
+        store(sOut + stpIdx(i, rd, rs, nr) * a.MC, so);
+      }
+    }
+    __syncthreads();  // (3) readers of v1Row / v2Row done
+    store(v1Row + pl * a.MC + m0, v1);
+    store(v2Row + pl * a.MC + m0, v2);
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      vj1[k] = v1[k]; S0[k] = cur.N[k]; E0[k] = cur.E[k];
+      vj2[k] = v2[k]; S1[k] = srp1[k]; E1[k] = stp1[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cPrev[k] = cur.c[k];
+    bSPrev[0] = cur.bS[0]; bSPrev[1] = cur.bS[1];
+    bTPrev[0] = cur.bT[0]; bTPrev[1] = cur.bT[1];
+  };
+
+  // input rows rFirst .. j1 + 1 (D of row j1 - 1 happens at r = j1 + 1)
+  const uint64_t rLast = j1 + 1;
+  RowIn<T, CPT> A, B;
+  fetch(A, rFirst);
+  for (uint64_t r = rFirst; r <= rLast; r += 2) {
+    row(A, B, r);
+    if (r + 1 <= rLast) row(B, A, r + 1);
+  }
+}
+
 // ------------------------------------------------------------ gathers
 template <class T>
 __global__ void k_snapshot(const T* __restrict__ x, uint64_t n, uint32_t MC, const uint32_t* __restrict__ cols,
@@ -405,7 +603,7 @@ struct shaw_fom {
   virtual void setState(const double* vp, const double* sp) = 0;
   virtual void time(uint64_t nsteps, double* total, double* perStep) = 0;
   virtual void bytes(double* kernel, double* model) const = 0;
-  virtual void config(int* kernel, int* cpt, int* strip, int* seg, int* pf) const = 0;
+  virtual void config(int* kernel, int* cpt, int* strip, int* seg, int* pf, int* spl) const = 0;
   virtual void pod(int dof, uint64_t k, const double* w, double* phi, double* sigma) = 0;
   virtual void autotune(uint64_t trialSteps, double* bestMs) = 0;
 };
@@ -433,8 +631,9 @@ class FomEngine final : public shaw_fom {
     while (MC_ < M && MC_ < cap) MC_ *= 2;
     nb_ = (uint32_t)((M + MC_ - 1) / MC_);
     kernel_ = o.kernel == SHAW_KERNEL_SPLIT ? SHAW_KERNEL_SPLIT : SHAW_KERNEL_FUSED;
-    // defaults from a sweep of the fused kernel on MI355X at 1024 x 4096, M = 32
-    // (FP64: 2 columns per thread, 32-point strips, 32-row segments)
+    // defaults from sweeps of the fused kernels on MI355X at 1024 x 4096, M = 32:
+    // 2 columns per thread, 32-point strips, 64-row segments, two steps per
+    // launch (one step per launch: 32-row segments)
     cpt_ = o.cols_per_thread > 0 ? o.cols_per_thread : (int)std::min<uint32_t>(MC_, sizeof(T) == 8 ? 2 : 4);
     SHAW_REQUIRE(cpt_ == 1 || cpt_ == 2 || cpt_ == 4 || cpt_ == 8, SHAW_ERR_ARG, "cols_per_thread must be 1, 2, 4 or 8");
     SHAW_REQUIRE(MC_ % cpt_ == 0, SHAW_ERR_ARG, "cols_per_thread must divide the column block width");
@@ -442,10 +641,14 @@ class FomEngine final : public shaw_fom {
     if (kernel_ == SHAW_KERNEL_FUSED) {
       P_ = o.strip > 0 ? (uint32_t)o.strip : std::max<uint32_t>(2, std::min<uint32_t>(64, 512 / tpp));
       SHAW_REQUIRE(P_ >= 2 && P_ * tpp <= 512, SHAW_ERR_ARG, "strip too wide for a 512-thread block");
-      SEG_ = o.segment > 0 ? (uint32_t)o.segment : 32;
       pf_ = o.prefetch_rows > 0 ? o.prefetch_rows : 1;
       SHAW_REQUIRE(pf_ == 1 || pf_ == 2, SHAW_ERR_ARG, "prefetch_rows must be 1 or 2");
+      spl_ = o.steps_per_launch > 0 ? o.steps_per_launch : (pf_ == 1 && P_ >= 4 ? 2 : 1);
+      SEG_ = o.segment > 0 ? (uint32_t)o.segment : (spl_ == 2 ? 64 : 32);
+      SHAW_REQUIRE(spl_ == 1 || spl_ == 2, SHAW_ERR_ARG, "steps_per_launch must be 1 or 2");
+      SHAW_REQUIRE(spl_ == 1 || P_ >= 4, SHAW_ERR_ARG, "two steps per launch need a strip of at least 4 points");
       nStrips_ = (uint32_t)((nth_ + (P_ - 1) - 1) / (P_ - 1));
+      nStrips2_ = P_ >= 4 ? (uint32_t)((nth_ + (P_ - 3) - 1) / (P_ - 3)) : 0;
       nSegs_ = (uint32_t)((nr_ + SEG_ - 1) / SEG_);
     }
     SHAW_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -552,8 +755,22 @@ class FomEngine final : public shaw_fom {
     if (R_) SHAW_REQUIRE(last / every_ <= Tcap_, SHAW_ERR_STATE, "seismogram buffer too small for the requested steps");
     for (int d = 0; d < 2; ++d)
       if (snap_[d].ncols) SHAW_REQUIRE(last / snap_[d].every <= snap_[d].Tcap, SHAW_ERR_STATE, "snapshot buffer too small");
-    for (uint64_t k = 0; k < nsteps; ++k) launchStep();
+    for (uint64_t k = 0; k < nsteps;) {
+      if (spl_ == 2 && k + 2 <= nsteps && !snapDue(step_ + 1)) {
+        launchPair();
+        k += 2;
+      } else {
+        launchStep();
+        k += 1;
+      }
+    }
     SHAW_HIP(hipGetLastError());
+  }
+  // a snapshot at the intermediate step of a pair needs that level in HBM
+  bool snapDue(uint64_t stepNo) const {
+    for (int d = 0; d < 2; ++d)
+      if (snap_[d].ncols && stepNo % snap_[d].every == 0) return true;
+    return false;
   }
   void sync() override { SHAW_HIP(hipStreamSynchronize(stream_)); }
   uint64_t step() const override { return step_; }
@@ -603,8 +820,9 @@ class FomEngine final : public shaw_fom {
     const double coef = (4.0 * nVp_ + 2.0 * nSp_) * s * nb_;
     if (kernel) {
       // fused: read + write each state once; split: v (r+w) + s (r) / v (r) + s (r+w)
+      // (per step: a launch of the two-step kernel moves these bytes for two steps)
       const double passes = kernel_ == SHAW_KERNEL_FUSED ? 2.0 : 3.0;
-      *kernel = s * cols * passes * (double)(nVp_ + nSp_) + coef;
+      *kernel = (s * cols * passes * (double)(nVp_ + nSp_) + coef) / (kernel_ == SHAW_KERNEL_FUSED ? spl_ : 1);
     }
     if (model) *model = s * (double)M_ * 3.0 * (double)(nVp_ + nSp_) + 8.0 * (4.0 * nVp_ + 2.0 * nSp_);
   }
@@ -637,12 +855,13 @@ class FomEngine final : public shaw_fom {
         for (uint64_t i = 0; i < n; ++i) phi[i + n * c] /= sw[i];
     s.ncols = 0;  // the buffer was consumed
   }
-  void config(int* kernel, int* cpt, int* strip, int* seg, int* pf) const override {
+  void config(int* kernel, int* cpt, int* strip, int* seg, int* pf, int* spl) const override {
     if (kernel) *kernel = kernel_;
     if (cpt) *cpt = cpt_;
     if (strip) *strip = (int)P_;
     if (seg) *seg = (int)SEG_;
     if (pf) *pf = pf_;
+    if (spl) *spl = kernel_ == SHAW_KERNEL_FUSED ? spl_ : 1;
   }
 
  private:
@@ -699,8 +918,8 @@ class FomEngine final : public shaw_fom {
   void autotune(uint64_t trialSteps, double* bestMs) override {
     SHAW_REQUIRE(kernel_ == SHAW_KERNEL_FUSED, SHAW_ERR_STATE, "the autotuner tunes the fused kernel");
     SHAW_REQUIRE(trialSteps >= 1 && trialSteps <= 1000, SHAW_ERR_ARG, "trial steps must be in 1..1000");
-    SHAW_REQUIRE(haveSources_ && ftab_.p && step_ + 1 <= ftabSteps_, SHAW_ERR_STATE,
-                 "set sources and a forcing table covering the next step before tuning");
+    SHAW_REQUIRE(haveSources_ && ftab_.p && step_ + 2 <= ftabSteps_, SHAW_ERR_STATE,
+                 "set sources and a forcing table covering the next two steps before tuning");
     sync();
     const int cur0 = cur_;
     DevBuf<T> sv, ss;
@@ -708,13 +927,13 @@ class FomEngine final : public shaw_fom {
     ss.alloc(s_[cur0].n);
     SHAW_HIP(hipMemcpy(sv.p, v_[cur0].p, sv.n * sizeof(T), hipMemcpyDeviceToDevice));
     SHAW_HIP(hipMemcpy(ss.p, s_[cur0].p, ss.n * sizeof(T), hipMemcpyDeviceToDevice));
-    struct Cfg { int cpt; uint32_t P, SEG; int pf; };
-    const Cfg start{cpt_, P_, SEG_, pf_};
-    Cfg best = start;
+    struct Cfg { int cpt; uint32_t P, SEG; int pf, spl; };
+    Cfg best{cpt_, P_, SEG_, pf_, spl_};
     double bestT = 1e300;
     auto apply = [&](const Cfg& c) {
-      cpt_ = c.cpt; P_ = c.P; SEG_ = c.SEG; pf_ = c.pf;
+      cpt_ = c.cpt; P_ = c.P; SEG_ = c.SEG; pf_ = c.pf; spl_ = c.spl;
       nStrips_ = (uint32_t)((nth_ + (P_ - 1) - 1) / (P_ - 1));
+      nStrips2_ = P_ >= 4 ? (uint32_t)((nth_ + (P_ - 3) - 1) / (P_ - 3)) : 0;
       nSegs_ = (uint32_t)((nr_ + SEG_ - 1) / SEG_);
     };
     for (int cpt : {1, 2, 4, 8}) {
@@ -723,18 +942,21 @@ class FomEngine final : public shaw_fom {
       for (uint32_t P : {8u, 16u, 32u, 64u}) {
         if (P * tpp > 512 || P * tpp < 64) continue;
         for (uint32_t SEG : {16u, 32u, 64u})
-          for (int pf : {1, 2}) {
-            const Cfg c{cpt, P, SEG, pf};
-            apply(c);
-            trialStep();
-            SHAW_HIP(hipEventRecord(ev0_, stream_));
-            for (uint64_t k = 0; k < trialSteps; ++k) trialStep();
-            SHAW_HIP(hipEventRecord(ev1_, stream_));
-            SHAW_HIP(hipEventSynchronize(ev1_));
-            float ms = 0;
-            SHAW_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-            if (ms / trialSteps < bestT) { bestT = ms / trialSteps; best = c; }
-          }
+          for (int pf : {1, 2})
+            for (int spl : {1, 2}) {
+              if (spl == 2 && pf == 2) continue;  // the two-step kernel loads one row ahead
+              const Cfg c{cpt, P, SEG, pf, spl};
+              apply(c);
+              trialLaunch();
+              SHAW_HIP(hipEventRecord(ev0_, stream_));
+              for (uint64_t k = 0; k < trialSteps; ++k) trialLaunch();
+              SHAW_HIP(hipEventRecord(ev1_, stream_));
+              SHAW_HIP(hipEventSynchronize(ev1_));
+              float ms = 0;
+              SHAW_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+              const double perStep = ms / (double)(trialSteps * spl);
+              if (perStep < bestT) { bestT = perStep; best = c; }
+            }
       }
     }
     apply(best);
@@ -742,6 +964,24 @@ class FomEngine final : public shaw_fom {
     SHAW_HIP(hipMemcpy(s_[cur0].p, ss.p, ss.n * sizeof(T), hipMemcpyDeviceToDevice));
     cur_ = cur0;
     if (bestMs) *bestMs = bestT;
+  }
+  // one launch of the current shape (one step or a pair) that records
+  // nothing and does not advance the run
+  void trialLaunch() {
+    if (spl_ == 2) {
+      StepArgs2<T> aa;
+      pairArgs(aa);
+      aa.s1.recv = aa.s2.recv = nullptr;
+      switch (cpt_) {
+        case 1: launchFused2<1>(aa, false); break;
+        case 2: launchFused2<2>(aa, false); break;
+        case 4: launchFused2<4>(aa, false); break;
+        default: launchFused2<8>(aa, false); break;
+      }
+      SHAW_HIP(hipGetLastError());
+    } else {
+      trialStep();
+    }
   }
   void trialStep() {
     StepArgs<T> a = args(step_ + 1);
@@ -770,6 +1010,41 @@ class FomEngine final : public shaw_fom {
       snapshot(0, v_[cur_].p);
       snapshot(1, s_[cur_].p);
     }
+  }
+  template <int CPT>
+  void launchFused2(StepArgs2<T>& aa, bool observe = true) {
+    const uint32_t tpp = MC_ / CPT;
+    for (StepArgs<T>* a : {&aa.s1, &aa.s2}) {
+      a->vIn = v_[cur_].p;
+      a->sIn = s_[cur_].p;
+      a->vOut = v_[cur_ ^ 1].p;
+      a->sOut = s_[cur_ ^ 1].p;
+      a->nStrips = nStrips2_;
+    }
+    const unsigned blocks = nStrips2_ * nSegs_ * nb_;
+    const size_t lds = (size_t)(4 * P_ + 2) * MC_ * sizeof(T);
+    hipLaunchKernelGGL((k_fused2<T, CPT>), dim3(blocks), dim3(P_ * tpp), lds, stream_, aa);
+    cur_ ^= 1;
+    if (observe) {
+      snapshot(0, v_[cur_].p);
+      snapshot(1, s_[cur_].p);
+    }
+  }
+  void pairArgs(StepArgs2<T>& aa) {
+    aa.s1 = args(step_ + 1);
+    aa.s2 = args(step_ + 2);
+  }
+  void launchPair() {
+    StepArgs2<T> aa;
+    pairArgs(aa);
+    stepForSnap_ = step_ + 2;
+    switch (cpt_) {
+      case 1: launchFused2<1>(aa); break;
+      case 2: launchFused2<2>(aa); break;
+      case 4: launchFused2<4>(aa); break;
+      default: launchFused2<8>(aa); break;
+    }
+    step_ += 2;
   }
   void launchStep() {
     const uint64_t stepNo = step_ + 1;
@@ -808,8 +1083,8 @@ class FomEngine final : public shaw_fom {
   }
 
   uint64_t nr_ = 0, nth_ = 0, nVp_ = 0, nSp_ = 0, M_ = 0;
-  uint32_t MC_ = 1, nb_ = 1, P_ = 0, SEG_ = 0, nStrips_ = 0, nSegs_ = 0;
-  int kernel_ = SHAW_KERNEL_FUSED, cpt_ = 1, rank1_ = 0, pf_ = 1;
+  uint32_t MC_ = 1, nb_ = 1, P_ = 0, SEG_ = 0, nStrips_ = 0, nStrips2_ = 0, nSegs_ = 0;
+  int kernel_ = SHAW_KERNEL_FUSED, cpt_ = 1, rank1_ = 0, pf_ = 1, spl_ = 1;
   double dt_ = 0;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
@@ -908,8 +1183,8 @@ int shaw_fom_time(shaw_fom_t h, uint64_t nsteps, double* total, double* perStep)
 int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel, double* model) {
   return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->bytes(kernel, model); });
 }
-int shaw_fom_config(shaw_fom_t h, int* kernel, int* cpt, int* strip, int* seg, int* pf) {
-  return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->config(kernel, cpt, strip, seg, pf); });
+int shaw_fom_config(shaw_fom_t h, int* kernel, int* cpt, int* strip, int* seg, int* pf, int* spl) {
+  return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->config(kernel, cpt, strip, seg, pf, spl); });
 }
 int shaw_fom_autotune(shaw_fom_t h, uint64_t trial_steps, double* best_ms_per_step) {
   return guarded([&] {

@@ -24,10 +24,12 @@ LAYERED = {"kind": "layered", "radius": [3480.0, 4100.0, 4700.0, 5200.0, 5600.0,
            "density": [5600.0, 5300.0, 4950.0, 4600.0, 4200.0, 3900.0, 3600.0, 3300.0],
            "velocity": [7300.0, 7000.0, 6600.0, 6200.0, 5700.0, 5200.0, 4800.0, 4400.0]}
 
-KERNELS = [dict(kernel=shaw_amd.KERNEL_SPLIT), dict(kernel=shaw_amd.KERNEL_FUSED),
-           dict(kernel=shaw_amd.KERNEL_FUSED, strip=8, segment=3),
-           dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=1, strip=5, segment=1),
-           dict(kernel=shaw_amd.KERNEL_FUSED, prefetch_rows=2, strip=6, segment=4)]
+KERNELS = [dict(kernel=shaw_amd.KERNEL_SPLIT), dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=1),
+           dict(kernel=shaw_amd.KERNEL_FUSED, strip=8, segment=3, steps_per_launch=1),
+           dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=1, strip=5, segment=1, steps_per_launch=1),
+           dict(kernel=shaw_amd.KERNEL_FUSED, prefetch_rows=2, strip=6, segment=4),
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2),
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2, strip=5, segment=3)]
 
 
 def say(msg):
@@ -55,7 +57,8 @@ def run_both(test, **kw):
         yield rid, len(s), fs is not None, a, b
 
 
-@pytest.mark.parametrize("kw", KERNELS, ids=["split", "fused", "fused-narrow", "fused-cpt1", "fused-pf2"])
+@pytest.mark.parametrize("kw", KERNELS, ids=["split", "fused", "fused-narrow", "fused-cpt1", "fused-pf2", "fused2",
+                                             "fused2-narrow"])
 @pytest.mark.parametrize("test", FOM_TESTS)
 def test_regression_inputs_bit_exact(test, kw):
     say(f"[gpu] {test} {kw}")
@@ -85,11 +88,18 @@ def mesh64x256(tmp_path_factory):
 
 @pytest.mark.parametrize("M,kw", [(32, dict(kernel=shaw_amd.KERNEL_SPLIT)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=1)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=8, strip=32, segment=7)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=2, strip=16, segment=64)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, prefetch_rows=2, strip=32, segment=32)),
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED, prefetch_rows=2, cols_per_thread=1,
                                             strip=16, segment=10)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2, cols_per_thread=4,
+                                            strip=12, segment=7)),
+                                  (40, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2, cols_per_thread=1,
+                                            strip=16, segment=64)),
+                                  (7, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2, strip=4, segment=2)),
                                   (7, dict(kernel=shaw_amd.KERNEL_FUSED)),
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED)),
                                   (40, dict(kernel=shaw_amd.KERNEL_SPLIT, cols_per_thread=2))])
@@ -128,6 +138,7 @@ def test_autotune_keeps_state_and_parity(mesh64x256):
         np.testing.assert_array_equal(x, y)
     cfg = fom.config()
     assert ms > 0 and cfg["cols_per_thread"] in (1, 2, 4, 8) and cfg["prefetch_rows"] in (1, 2)
+    assert cfg["steps_per_launch"] in (1, 2)
     assert cfg["segment"] in (16, 32, 64) and cfg["strip"] in (8, 16, 32, 64)
     fom.close()
     rec = list(np.arange(0, 180, 7.5))
@@ -173,7 +184,7 @@ def test_fp32_state_error_bound(mesh64x256):
 
 def test_full_config2_shape_properties():
     """1024 x 4096, M = 32 (config 2's full single-GPU shape), 10 steps:
-    fused == split bit for bit, and doubling the forcing doubles the state
+    fused == split == two steps per launch bit for bit, and doubling the forcing doubles the state
     exactly (linearity; scaling by 2 is exact in binary floating point)."""
     say("[gpu] config-2 shape properties")
     app = shaw_amd.App.structured(1024, 4096, LAYERED)
@@ -184,9 +195,9 @@ def test_full_config2_shape_properties():
     tab = shaw_amd.forcing_table(["ricker"] * M, [s[3] for s in src], [s[2] for s in src], dt, n)
     tab[:] = 1.0 + np.arange(M)  # large constant forcing so the wave reaches many points in 10 steps
     out = {}
-    for name, kern, scale in [("fused", shaw_amd.KERNEL_FUSED, 1.0), ("split", shaw_amd.KERNEL_SPLIT, 1.0),
-                              ("fused2", shaw_amd.KERNEL_FUSED, 2.0)]:
-        f = shaw_amd.Fom(app, M, dt, kernel=kern, rank1_forcing=False)
+    for name, kern, scale, spl in [("fused", shaw_amd.KERNEL_FUSED, 1.0, 1), ("split", shaw_amd.KERNEL_SPLIT, 1.0, 1),
+                                   ("fused2", shaw_amd.KERNEL_FUSED, 2.0, 1), ("pairs", shaw_amd.KERNEL_FUSED, 1.0, 2)]:
+        f = shaw_amd.Fom(app, M, dt, kernel=kern, rank1_forcing=False, steps_per_launch=spl)
         f.set_sources(gids)
         f.set_forcing(tab * scale)
         f.set_receivers([app.map_receiver(a) for a in range(180)], 1, n)
@@ -194,6 +205,8 @@ def test_full_config2_shape_properties():
         out[name] = f.state() + (f.seismogram(),)
         f.close()
     for a, b in zip(out["fused"], out["split"]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(out["fused"], out["pairs"]):  # two steps per launch at the full shape
         np.testing.assert_array_equal(a, b)
     for a, b in zip(out["fused"], out["fused2"]):
         np.testing.assert_array_equal(2.0 * a, b)
