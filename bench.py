@@ -248,6 +248,46 @@ def serving_section(app, w, basis, args) -> dict:
     return rep
 
 
+def config1_section(args) -> dict:
+    """Config 1, the reference's CPU-runnable case: 200 x 1000, one Ricker
+    source (640 km, 45 s) at theta = 0, rank-1 forcing form, dt = 0.25 s,
+    4000 steps, 180 receivers every step. The whole run on the GPU with HIP
+    graph replays (default) and with eager launches, the oracle's whole run
+    on the host cores, and the seismograms compared (bit-identical)."""
+    import oracle
+    import shaw_amd
+    from shaw_amd import workloads
+
+    w = workloads.config1()
+    mat = workloads.layered_material()
+    app = shaw_amd.App.structured(w.nr, w.nth, mat)
+    out, seis = {}, {}
+    for name, gsteps in (("graphs", 0), ("eager", -1)):
+        fom = shaw_amd.Fom(app, 1, w.dt, rank1_forcing=True, graph_steps=gsteps)
+        fom.set_sources([app.map_source(d, a) for d, a, p, dl in w.sources])
+        fom.set_forcing(shaw_amd.forcing_table(["ricker"], [w.sources[0][3]], [w.sources[0][2]], w.dt, w.nsteps))
+        fom.set_receivers([app.map_receiver(a) for a in workloads.RECEIVERS], 1, w.nsteps)
+        fom.advance(16)  # warm (and capture)
+        fom.reset()
+        total, _ = fom.time(w.nsteps)
+        seis[name] = fom.seismogram()
+        out[name] = {"seconds": total / 1e3, "forcing_steps_per_s": w.nsteps / (total / 1e3)}
+        fom.close()
+    t0 = time.time()
+    ora = oracle.App("", mat, structured=(w.nr, w.nth, 0.0, 180.0, 3480.0, 6371.0))
+    t1 = time.time()
+    ref = ora.fom_run(w.dt, w.nsteps, w.sources, False, workloads.RECEIVERS, 1)
+    t2 = time.time()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out["cpu_oracle"] = {"seconds": t2 - t1, "forcing_steps_per_s": w.nsteps / (t2 - t1), "cores": cores,
+                         "setup_s": t1 - t0}
+    out["gpu_equals_oracle"] = bool(np.array_equal(seis["graphs"][0], ref["seismogram"][0]))
+    out["graphs_equal_eager"] = bool(np.array_equal(seis["graphs"], seis["eager"]))
+    out["workload"] = w.describe() + ", 180 receivers every step, rank-1 forcing form"
+    log(f"config1: {out}")
+    return out
+
+
 def config5_section(args) -> dict:
     """Config 5: the large FOM, 2048 x 8192, M = 64 seeded forcings,
     dt = 0.025 s. FP64 and FP32 states, each with the autotuned tile shape,
@@ -320,6 +360,7 @@ def main() -> int:
     ap.add_argument("--rom-steps", type=int, default=20000)
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--config5", action="store_true", help="also measure the large FOM (2048 x 8192, M = 64)")
+    ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--c5-err-steps", type=int, default=8000)
     args = ap.parse_args()
 
@@ -403,6 +444,8 @@ def main() -> int:
         out["rom_at_err_1e-3"], basis = rom_err_section(app, w, value, args)
     if not args.no_serve:
         out["serving"] = serving_section(app, w, basis, args)
+    if not args.no_config1:
+        out["fom_config1"] = config1_section(args)
     if args.config5:
         out["fom_config5"] = config5_section(args)
     if not args.no_cpu:

@@ -127,9 +127,12 @@ typedef struct {
   int strip;            /* 0 = default; fused kernel: vp points per block row (incl. halo) */
   int segment;          /* 0 = default; fused kernel: radial rows per block */
   int prefetch_rows;    /* 0 = default; fused kernel: rows loaded ahead, 1 or 2 */
-  int steps_per_launch; /* 0 = default (1); fused kernel: 1, or 2 (two leapfrog steps per
-                           launch, level n+1 kept on chip; snapshots at the intermediate
-                           step fall back to single steps)                          */
+  int steps_per_launch; /* 0 = default (2 unless prefetch_rows = 2); fused kernel: 1, or 2
+                           (two leapfrog steps per launch, level n+1 kept on chip;
+                           snapshots at the intermediate step fall back to single steps) */
+  int graph_steps;      /* 0 = default (64); < 0: no graphs. Fused kernel without
+                           snapshots: whole chunks of this many steps replay one captured
+                           HIP graph (kernels read their step from a device counter)   */
 } shaw_fom_options;
 
 /* app: operators; m: forcing columns; dt: time step. opts may be NULL. */

@@ -49,7 +49,7 @@ class _AppInfo(C.Structure):
 class _FomOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("kernel", C.c_int), ("rank1_forcing", C.c_int),
                 ("cols_per_thread", C.c_int), ("strip", C.c_int), ("segment", C.c_int),
-                ("prefetch_rows", C.c_int), ("steps_per_launch", C.c_int)]
+                ("prefetch_rows", C.c_int), ("steps_per_launch", C.c_int), ("graph_steps", C.c_int)]
 
 
 class _RomDesc(C.Structure):
@@ -268,11 +268,12 @@ class Fom:
     """Full-order leapfrog for m forcing columns on the GPU."""
 
     def __init__(self, app: App, m: int, dt: float, precision=FP64, kernel=KERNEL_AUTO,
-                 rank1_forcing=None, cols_per_thread=0, strip=0, segment=0, prefetch_rows=0, steps_per_launch=0):
+                 rank1_forcing=None, cols_per_thread=0, strip=0, segment=0, prefetch_rows=0, steps_per_launch=0,
+                 graph_steps=0):
         if rank1_forcing is None:
             rank1_forcing = m == 1
         o = _FomOptions(precision, kernel, int(bool(rank1_forcing)), cols_per_thread, strip, segment, prefetch_rows,
-                        steps_per_launch)
+                        steps_per_launch, graph_steps)
         h = C.c_void_p()
         _check(lib().shaw_fom_create(app.handle, m, dt, C.byref(o), C.byref(h)))
         self._h, self.app, self.m, self.dt = h, app, m, dt

@@ -75,6 +75,13 @@ struct StepArgs {
   const double* frow;   // forcing values of this step, [M]
   const int32_t* recv;  // [nVp] receiver index or -1; nullptr: no sample this step
   int64_t recvRow;      // the one grid row holding every receiver, or -1
+  // graph launches: the step is *stepDev + stepOff (1-based), and frow, recv
+  // and sample follow from it (resolveStep); stepDev == nullptr: host-resolved
+  const uint64_t* stepDev;
+  uint64_t stepOff;
+  const double* ftabBase;
+  const int32_t* recvTable;
+  uint64_t every;
   double* seis;         // LayoutLeft [R x Tcap x M]
   uint64_t sample, R, Tcap;
   uint32_t nr, nth, M, MC;
@@ -82,6 +89,16 @@ struct StepArgs {
   int rank1;
   uint32_t P, SEG, nStrips, nSegs, nb;
 };
+
+template <class T>
+__device__ __forceinline__ void resolveStep(StepArgs<T>& a) {
+  if (a.stepDev == nullptr) return;
+  const uint64_t s = *a.stepDev + a.stepOff;
+  a.frow = a.ftabBase + (s - 1) * a.M;
+  const bool rec = a.R && a.every && s % a.every == 0;
+  a.recv = rec ? a.recvTable : nullptr;
+  a.sample = rec ? s / a.every - 1 : 0;
+}
 
 __device__ __forceinline__ uint64_t srpIdx(uint64_t i, uint64_t j, uint64_t rs) { return j * rs + 2 * i; }
 __device__ __forceinline__ uint64_t stpIdx(uint64_t i, uint64_t j, uint64_t rs, uint64_t nr) {
@@ -212,6 +229,7 @@ struct RowIn {
 
 template <class T, int CPT, int PF>
 __global__ void __launch_bounds__(512) k_fused(StepArgs<T> a) {
+  resolveStep(a);
   extern __shared__ unsigned char smem_raw[];
   T* stpRow = reinterpret_cast<T*>(smem_raw);    // [P+1][MC]: old stp(i0-1 .. i0+P-1) of row r
   T* vRow = stpRow + (a.P + 1) * a.MC;            // [P][MC]: new v of row jj
@@ -360,6 +378,8 @@ struct StepArgs2 {
 
 template <class T, int CPT>
 __global__ void __launch_bounds__(512) k_fused2(StepArgs2<T> aa) {
+  resolveStep(aa.s1);
+  resolveStep(aa.s2);
   const StepArgs<T>& a = aa.s1;
   extern __shared__ unsigned char smem_raw[];
   T* stp0Row = reinterpret_cast<T*>(smem_raw);    // [P+1][MC]: s^n stp of row r (lane pl at pl+1, west load at 0)
@@ -549,6 +569,14 @@ __global__ void k_snapshot(const T* __restrict__ x, uint64_t n, uint32_t MC, con
   out[g + n * (sample + Tcap * c)] = (double)x[((uint64_t)(col / MC) * n + g) * MC + col % MC];
 }
 
+// the device step counter of graph launches (one lane, a vector store)
+__global__ void k_step_set(uint64_t* __restrict__ step, uint64_t value) {
+  if (threadIdx.x == 0) step[threadIdx.x] = value;
+}
+__global__ void k_step_add(uint64_t* __restrict__ step, uint64_t n) {
+  if (threadIdx.x == 0) step[threadIdx.x] = step[threadIdx.x] + n;
+}
+
 // x[r + n*c] *= w[r] for every column c (grid-stride over columns)
 __global__ void k_scale_rows(double* __restrict__ x, const double* __restrict__ w, uint64_t n, uint64_t cols) {
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -644,6 +672,9 @@ class FomEngine final : public shaw_fom {
       pf_ = o.prefetch_rows > 0 ? o.prefetch_rows : 1;
       SHAW_REQUIRE(pf_ == 1 || pf_ == 2, SHAW_ERR_ARG, "prefetch_rows must be 1 or 2");
       spl_ = o.steps_per_launch > 0 ? o.steps_per_launch : (pf_ == 1 && P_ >= 4 ? 2 : 1);
+      graphSteps_ = o.graph_steps < 0 ? 0 : (o.graph_steps > 0 ? o.graph_steps : 64);
+      SHAW_REQUIRE(graphSteps_ % (2 * spl_) == 0, SHAW_ERR_ARG,
+                   "graph_steps must be a multiple of 2 x steps per launch (an even number of launches)");
       SEG_ = o.segment > 0 ? (uint32_t)o.segment : (spl_ == 2 ? 64 : 32);
       SHAW_REQUIRE(spl_ == 1 || spl_ == 2, SHAW_ERR_ARG, "steps_per_launch must be 1 or 2");
       SHAW_REQUIRE(spl_ == 1 || P_ >= 4, SHAW_ERR_ARG, "two steps per launch need a strip of at least 4 points");
@@ -674,6 +705,7 @@ class FomEngine final : public shaw_fom {
   }
   ~FomEngine() override {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    dropGraph();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -693,12 +725,16 @@ class FomEngine final : public shaw_fom {
     haveSources_ = true;
   }
   void setForcing(uint64_t nsteps, const double* tab) override {
+    sync();
+    dropGraph();  // the graph holds the table's address
     ftab_.alloc(nsteps * M_);
     SHAW_HIP(hipMemcpy(ftab_.p, tab, nsteps * M_ * sizeof(double), hipMemcpyHostToDevice));
     ftabSteps_ = nsteps;
   }
   void setReceivers(uint64_t R, const uint64_t* gid, uint64_t every, uint64_t maxSamples) override {
     SHAW_REQUIRE(R == 0 || every > 0, SHAW_ERR_ARG, "receiver sampling period must be > 0");
+    sync();
+    dropGraph();  // the graph holds the receiver table and seismogram addresses
     std::vector<int32_t> idx(nVp_, -1);
     dupRecv_.clear();
     for (uint64_t r = 0; r < R; ++r) {
@@ -755,7 +791,20 @@ class FomEngine final : public shaw_fom {
     if (R_) SHAW_REQUIRE(last / every_ <= Tcap_, SHAW_ERR_STATE, "seismogram buffer too small for the requested steps");
     for (int d = 0; d < 2; ++d)
       if (snap_[d].ncols) SHAW_REQUIRE(last / snap_[d].every <= snap_[d].Tcap, SHAW_ERR_STATE, "snapshot buffer too small");
-    for (uint64_t k = 0; k < nsteps;) {
+    uint64_t k = 0;
+    // whole chunks of graphStep_ steps replay one captured graph (fused
+    // kernels, no snapshots: a snapshot kernel would depend on the step)
+    const bool graphs = graphSteps_ > 0 && kernel_ == SHAW_KERNEL_FUSED && !snap_[0].ncols && !snap_[1].ncols &&
+                        nsteps >= (uint64_t)graphSteps_;
+    if (graphs) {
+      ensureGraph();
+      hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, stream_, stepDev_.p, step_);
+      for (; k + graphSteps_ <= nsteps; k += graphSteps_) {
+        SHAW_HIP(hipGraphLaunch(gexec_, stream_));
+        step_ += graphSteps_;
+      }
+    }
+    for (; k < nsteps;) {
       if (spl_ == 2 && k + 2 <= nsteps && !snapDue(step_ + 1)) {
         launchPair();
         k += 2;
@@ -765,6 +814,68 @@ class FomEngine final : public shaw_fom {
       }
     }
     SHAW_HIP(hipGetLastError());
+  }
+  // A graph of graphSteps_ steps of the current shape, captured from the
+  // current buffer parity; every launch reads its step from stepDev_ (set
+  // before the replays, advanced by the graph's last node), so one graph
+  // serves every chunk. It holds an even number of launches, so the
+  // ping-pong parity is the same after a replay.
+  void ensureGraph() {
+    const int key[7] = {cpt_, (int)P_, (int)SEG_, pf_, spl_, cur_, graphSteps_};
+    if (gexec_ && std::equal(key, key + 7, graphKey_)) return;
+    dropGraph();
+    if (!stepDev_.p) stepDev_.alloc(1);
+    const int cur0 = cur_;
+    SHAW_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < graphSteps_; k += spl_) {
+      if (spl_ == 2) {
+        StepArgs2<T> aa;
+        aa.s1 = deviceStepArgs(k + 1);
+        aa.s2 = deviceStepArgs(k + 2);
+        dispatchPair(aa, false);
+      } else {
+        StepArgs<T> a = deviceStepArgs(k + 1);
+        dispatchStep(a, false);
+      }
+    }
+    hipLaunchKernelGGL(k_step_add, dim3(1), dim3(64), 0, stream_, stepDev_.p, (uint64_t)graphSteps_);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream_, &g);
+    cur_ = cur0;  // capture launched nothing
+    SHAW_HIP(e);
+    const hipError_t ei = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    SHAW_HIP(ei);
+    std::copy(key, key + 7, graphKey_);
+  }
+  void dropGraph() {
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    gexec_ = nullptr;
+  }
+  StepArgs<T> deviceStepArgs(uint64_t off) {
+    StepArgs<T> a = args(1);  // shape fields; the step-dependent ones come from stepDev_
+    a.stepDev = stepDev_.p;
+    a.stepOff = off;
+    a.ftabBase = ftab_.p;
+    a.recvTable = R_ ? recv_.p : nullptr;
+    a.every = R_ ? every_ : 0;
+    return a;
+  }
+  void dispatchStep(StepArgs<T>& a, bool observe) {
+    switch (cpt_) {
+      case 1: launchFused<1>(a, observe); break;
+      case 2: launchFused<2>(a, observe); break;
+      case 4: launchFused<4>(a, observe); break;
+      default: launchFused<8>(a, observe); break;
+    }
+  }
+  void dispatchPair(StepArgs2<T>& aa, bool observe) {
+    switch (cpt_) {
+      case 1: launchFused2<1>(aa, observe); break;
+      case 2: launchFused2<2>(aa, observe); break;
+      case 4: launchFused2<4>(aa, observe); break;
+      default: launchFused2<8>(aa, observe); break;
+    }
   }
   // a snapshot at the intermediate step of a pair needs that level in HBM
   bool snapDue(uint64_t stepNo) const {
@@ -1093,6 +1204,10 @@ class FomEngine final : public shaw_fom {
   DevBuf<double> ftab_, seis_;
   DevBuf<int32_t> recv_;
   int64_t recvRow_ = -1;
+  DevBuf<uint64_t> stepDev_;
+  hipGraphExec_t gexec_ = nullptr;
+  int graphKey_[7] = {0, 0, 0, 0, 0, -1, 0};
+  int graphSteps_ = 64;
   std::vector<double> rhoInv_;
   std::vector<std::pair<uint32_t, uint32_t>> dupRecv_;
   uint64_t ftabSteps_ = 0, R_ = 0, every_ = 1, Tcap_ = 0, step_ = 0, stepForSnap_ = 0;

@@ -148,6 +148,49 @@ def test_autotune_keeps_state_and_parity(mesh64x256):
         np.testing.assert_array_equal(a[key], b[key], err_msg=key)
 
 
+@pytest.mark.parametrize("spl", [1, 2])
+def test_graph_replays_equal_eager(mesh64x256, spl):
+    """Chunks of 8 steps replay one captured HIP graph whose kernels read their
+    step from a device counter. Checked against eager launches after every
+    call: several replays, eager remainders, host state reads and a reset in
+    between, and a forcing table replaced (which drops the graph)."""
+    say(f"[gpu] graph replays spl={spl}")
+    rng = np.random.default_rng(5)
+    app = shaw_amd.App.from_mesh(mesh64x256, LAYERED)
+    src = random_sources(rng, 20)
+    dt, n = 0.5, 200
+    tab = shaw_amd.forcing_table(["ricker"] * 20, [s[3] for s in src], [s[2] for s in src], dt, n)
+    rec = [app.map_receiver(a) for a in np.arange(0, 180, 9.0)]
+
+    def make(graph_steps):
+        f = shaw_amd.Fom(app, 20, dt, steps_per_launch=spl, graph_steps=graph_steps)
+        f.set_sources([app.map_source(d, a) for d, a, p, dl in src])
+        f.set_forcing(tab)
+        f.set_receivers(rec, 3, n // 3)
+        return f
+
+    g, e = make(8), make(-1)
+    for chunk in (30, 17, 8, 41):
+        g.advance(chunk)
+        e.advance(chunk)
+        for x, y in zip(g.state(), e.state()):
+            np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(g.seismogram(), e.seismogram())
+    g.reset()
+    e.reset()
+    g.set_forcing(tab * 0.5)
+    e.set_forcing(tab * 0.5)
+    for chunk in (24, 9):
+        g.advance(chunk)
+        e.advance(chunk)
+        for x, y in zip(g.state(), e.state()):
+            np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(g.seismogram(), e.seismogram())
+    assert np.abs(g.seismogram()).max() > 0
+    g.close()
+    e.close()
+
+
 def test_config1_shape_rank1_bit_exact(tmp_path):
     """The config-1 workload shape (200 x 1000, layered mantle, Ricker 640 km /
     45 s at theta=0, dt=0.25, 180 surface receivers every step) for a
