@@ -659,9 +659,10 @@ class FomEngine final : public shaw_fom {
     while (MC_ < M && MC_ < cap) MC_ *= 2;
     nb_ = (uint32_t)((M + MC_ - 1) / MC_);
     kernel_ = o.kernel == SHAW_KERNEL_SPLIT ? SHAW_KERNEL_SPLIT : SHAW_KERNEL_FUSED;
-    // defaults from sweeps of the fused kernels on MI355X at 1024 x 4096, M = 32:
-    // 2 columns per thread, 32-point strips, 64-row segments, two steps per
-    // launch (one step per launch: 32-row segments)
+    // defaults from sweeps of the fused kernels on MI355X: at 1024 x 4096,
+    // M = 32, 2 columns per thread, 32-point strips, 64-row segments and two
+    // steps per launch (one step per launch: 32-row segments); at 200 x 1000,
+    // M = 1, 4 and 16, the fastest segments give about 1700 wavefronts a launch
     cpt_ = o.cols_per_thread > 0 ? o.cols_per_thread : (int)std::min<uint32_t>(MC_, sizeof(T) == 8 ? 2 : 4);
     SHAW_REQUIRE(cpt_ == 1 || cpt_ == 2 || cpt_ == 4 || cpt_ == 8, SHAW_ERR_ARG, "cols_per_thread must be 1, 2, 4 or 8");
     SHAW_REQUIRE(MC_ % cpt_ == 0, SHAW_ERR_ARG, "cols_per_thread must divide the column block width");
@@ -675,9 +676,17 @@ class FomEngine final : public shaw_fom {
       graphSteps_ = o.graph_steps < 0 ? 0 : (o.graph_steps > 0 ? o.graph_steps : 64);
       SHAW_REQUIRE(graphSteps_ % (2 * spl_) == 0, SHAW_ERR_ARG,
                    "graph_steps must be a multiple of 2 x steps per launch (an even number of launches)");
-      SEG_ = o.segment > 0 ? (uint32_t)o.segment : (spl_ == 2 ? 64 : 32);
       SHAW_REQUIRE(spl_ == 1 || spl_ == 2, SHAW_ERR_ARG, "steps_per_launch must be 1 or 2");
       SHAW_REQUIRE(spl_ == 1 || P_ >= 4, SHAW_ERR_ARG, "two steps per launch need a strip of at least 4 points");
+      // segment: 64 rows (two steps per launch) or 32, halved while a launch
+      // has fewer than kMinWaves wavefronts (small grids and few columns:
+      // 200 x 1000 with M = 1 is fastest with 2-row segments)
+      if (o.segment > 0) {
+        SEG_ = (uint32_t)o.segment;
+      } else {
+        SEG_ = spl_ == 2 ? 64 : 32;
+        while (SEG_ > 2 && launchWaves(cpt_, P_, SEG_, spl_) < kMinWaves) SEG_ /= 2;
+      }
       nStrips_ = (uint32_t)((nth_ + (P_ - 1) - 1) / (P_ - 1));
       nStrips2_ = P_ >= 4 ? (uint32_t)((nth_ + (P_ - 3) - 1) / (P_ - 3)) : 0;
       nSegs_ = (uint32_t)((nr_ + SEG_ - 1) / SEG_);
@@ -1050,12 +1059,14 @@ class FomEngine final : public shaw_fom {
     for (int cpt : {1, 2, 4, 8}) {
       if (MC_ % cpt) continue;
       const uint32_t tpp = MC_ / cpt;
-      for (uint32_t P : {8u, 16u, 32u, 64u}) {
+      for (uint32_t P : {8u, 16u, 32u, 64u, 128u, 256u}) {
         if (P * tpp > 512 || P * tpp < 64) continue;
-        for (uint32_t SEG : {16u, 32u, 64u})
+        for (uint32_t SEG : {2u, 4u, 8u, 16u, 32u, 64u})
           for (int pf : {1, 2})
             for (int spl : {1, 2}) {
               if (spl == 2 && pf == 2) continue;  // the two-step kernel loads one row ahead
+              // short segments only where the grid is too small to fill the chip
+              if (SEG < 16 && launchWaves(cpt, P, SEG, spl) > 16 * kMinWaves) continue;
               const Cfg c{cpt, P, SEG, pf, spl};
               apply(c);
               trialLaunch();
@@ -1075,6 +1086,13 @@ class FomEngine final : public shaw_fom {
     SHAW_HIP(hipMemcpy(s_[cur0].p, ss.p, ss.n * sizeof(T), hipMemcpyDeviceToDevice));
     cur_ = cur0;
     if (bestMs) *bestMs = bestT;
+  }
+  // wavefronts of one fused launch of the given shape
+  uint64_t launchWaves(int cpt, uint32_t P, uint32_t SEG, int spl) const {
+    const uint32_t halo = spl == 2 ? 3 : 1;
+    const uint64_t strips = (nth_ + (P - halo) - 1) / (P - halo), segs = (nr_ + SEG - 1) / SEG;
+    const uint64_t wavesPerBlock = ((uint64_t)P * (MC_ / cpt) + 63) / 64;
+    return strips * segs * nb_ * wavesPerBlock;
   }
   // one launch of the current shape (one step or a pair) that records
   // nothing and does not advance the run
@@ -1208,6 +1226,7 @@ class FomEngine final : public shaw_fom {
   hipGraphExec_t gexec_ = nullptr;
   int graphKey_[7] = {0, 0, 0, 0, 0, -1, 0};
   int graphSteps_ = 64;
+  static constexpr uint64_t kMinWaves = 1536;  // 6 per CU of the 256
   std::vector<double> rhoInv_;
   std::vector<std::pair<uint32_t, uint32_t>> dupRecv_;
   uint64_t ftabSteps_ = 0, R_ = 0, every_ = 1, Tcap_ = 0, step_ = 0, stepForSnap_ = 0;

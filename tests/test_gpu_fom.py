@@ -139,7 +139,7 @@ def test_autotune_keeps_state_and_parity(mesh64x256):
     cfg = fom.config()
     assert ms > 0 and cfg["cols_per_thread"] in (1, 2, 4, 8) and cfg["prefetch_rows"] in (1, 2)
     assert cfg["steps_per_launch"] in (1, 2)
-    assert cfg["segment"] in (16, 32, 64) and cfg["strip"] in (8, 16, 32, 64)
+    assert cfg["segment"] in (2, 4, 8, 16, 32, 64) and cfg["strip"] in (8, 16, 32, 64, 128, 256)
     fom.close()
     rec = list(np.arange(0, 180, 7.5))
     a = shaw_amd.fom_run(app, dt, n, src, True, rec, 3, 40, 30, autotune=2)
@@ -206,6 +206,21 @@ def test_config1_shape_rank1_bit_exact(tmp_path):
     b = ora.fom_run(0.25, 600, src, False, rec, 1)
     for key in b:
         np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+
+
+def test_default_segment_fills_the_chip():
+    """Default tile shapes: segments are halved until a launch has at least
+    1536 wavefronts (6 per CU), so 200 x 1000 with one column runs 2-row
+    segments while 1024 x 4096 with 32 columns keeps 64-row segments."""
+    say("[gpu] default segments")
+    small = shaw_amd.Fom(shaw_amd.App.structured(200, 1000, LAYERED), 1, 0.25)
+    cfg = small.config()
+    assert (cfg["strip"], cfg["segment"], cfg["steps_per_launch"]) == (64, 2, 2)
+    small.close()
+    big = shaw_amd.Fom(shaw_amd.App.structured(1024, 4096, LAYERED), 32, 0.05)
+    cfg = big.config()
+    assert (cfg["cols_per_thread"], cfg["strip"], cfg["segment"], cfg["steps_per_launch"]) == (2, 32, 64, 2)
+    big.close()
 
 
 def test_fp32_state_error_bound(mesh64x256):
