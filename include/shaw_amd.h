@@ -271,6 +271,21 @@ void shaw_rom_destroy(shaw_rom_t h);
  * exceeds the numerical rank. Replaces computeThinSVD
  * (src/tools/main_eigen_svd.cc:113-197). phi: LayoutLeft [n x k].          */
 int shaw_pod_basis(const double* s, uint64_t n, uint64_t ns, uint64_t k, double* phi, double* sigma);
+/* As shaw_pod_basis, but computes up to kmax modes and stops at the numerical
+ * rank (singular values above 1e-14 sigma_1); *k receives the count. This is
+ * what computeThinSVD's thin U (src/tools/main_eigen_svd.cc:129-133) holds
+ * for a full-rank snapshot matrix; for a rank-deficient one only the resolved
+ * columns are written (the trailing columns of a thin SVD there are an
+ * arbitrary orthonormal completion). phi: [n x kmax] capacity.             */
+int shaw_pod_basis_upto(const double* s, uint64_t n, uint64_t ns, uint64_t kmax, double* phi, double* sigma,
+                        uint64_t* k);
+/* out = phi q: the FOM state (or, with the rows of phi at receiver gids, the
+ * seismogram) of reduced coordinates, on the GPU in FP64. phi LayoutLeft
+ * [n x k], q LayoutLeft [k x ncols], out LayoutLeft [n x ncols]; each entry
+ * is the sum over modes in index order without FMA, as the loops of
+ * src/tools/main_reconstruct_seismo.cc:28-38 (reconstructSeismogram) and the
+ * gemv of src/tools/main_reconstruct_fom_state.cc:158-160 (reconstructFomState). */
+int shaw_reconstruct(const double* phi, uint64_t n, uint64_t k, const double* q, uint64_t ncols, double* out);
 
 /* ---------------------------------------------------------- diagnostics --
  * One MFMA instruction on one wavefront with lane-wise operands, for

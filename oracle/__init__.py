@@ -65,6 +65,8 @@ def _load():
     lib.orc_rom_run.restype = C.c_int
     lib.orc_rom_run.argtypes = [C.c_uint64, C.c_uint64, _dp, _dp, _dp, _dp, C.c_double,
                                 C.c_uint64, C.c_uint64, C.c_uint64, _dp, C.c_uint64, vp, vp, vp]
+    lib.orc_reconstruct.restype = C.c_int
+    lib.orc_reconstruct.argtypes = [vp, C.c_uint64, C.c_uint64, vp, C.c_uint64, vp]
     _lib = lib
     return lib
 
@@ -227,3 +229,13 @@ def rom_run(ArV, ArS, B, ftab, dt, nsteps, C=None, seis_freq=0):
                     np.ascontiguousarray(B), np.ascontiguousarray(ftab), dt, nsteps, M, R, Cc,
                     seis_freq if R else 0, _ptr(seis), _ptr(qv), _ptr(qs))
     return {"seismogram": seis, "qv": qv, "qs": qs}
+
+
+def reconstruct(phi: np.ndarray, q: np.ndarray) -> np.ndarray:
+    """phi [n x K] q [K x ncols] as reconstructSeismogram sums it (mode order)."""
+    phi = np.asfortranarray(phi, dtype=np.float64)
+    q = np.asfortranarray(q, dtype=np.float64)
+    n, K = phi.shape
+    out = np.zeros((q.shape[1], n))
+    _load().orc_reconstruct(_ptr(phi), n, K, _ptr(q), q.shape[1], _ptr(out))
+    return out.T

@@ -642,4 +642,20 @@ int orc_rom_run(uint64_t Kv, uint64_t Ks, const double* ArV, const double* ArS, 
   return 0;
 }
 
+// The reconstruction loop of src/tools/main_reconstruct_seismo.cc:28-38:
+// out(i, j) = 0, then out(i, j) += phi(i, k) * q(k, j) for k = 0 .. K-1.
+// reconstructFomState's KokkosBlas::gemv (main_reconstruct_fom_state.cc:
+// 158-160) computes the same sums in an order that library leaves open.
+// phi LayoutLeft [n x K], q LayoutLeft [K x ncols], out LayoutLeft [n x ncols].
+int orc_reconstruct(const double* phi, uint64_t n, uint64_t K, const double* q, uint64_t ncols, double* out) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i)
+    for (uint64_t j = 0; j < ncols; ++j) {
+      double s = 0.0;
+      for (uint64_t k = 0; k < K; ++k) s += phi[i + n * k] * q[k + K * j];
+      out[i + n * j] = s;
+    }
+  return 0;
+}
+
 }  // extern "C"

@@ -122,6 +122,8 @@ def lib():
         "shaw_rom_state": (i32, [vp, u64, vp, vp, P(u64)]),
         "shaw_debug_mfma": (i32, [i32, vp, vp, vp]),
         "shaw_pod_basis": (i32, [vp, u64, u64, u64, vp, vp]),
+        "shaw_pod_basis_upto": (i32, [vp, u64, u64, u64, vp, vp, P(u64)]),
+        "shaw_reconstruct": (i32, [vp, u64, u64, vp, u64, vp]),
         "shaw_debug_copy": (i32, [u64, i32, P(dbl)]),
         "shaw_debug_symeig": (i32, [i32, vp, vp, vp]),
         "shaw_rom_time": (i32, [vp, u64, P(dbl)]),
@@ -578,6 +580,29 @@ def pod_basis(S: np.ndarray, k: int):
     sig = np.zeros(ns)
     _check(lib().shaw_pod_basis(_p(S), n, ns, k, _p(phi), _p(sig)))
     return phi.T, sig
+
+
+def pod_basis_upto(S: np.ndarray, kmax: int):
+    """(Phi [n x k], sigma [ns]) with k <= kmax stopping at the numerical rank."""
+    S = np.asfortranarray(S, dtype=np.float64)
+    n, ns = S.shape
+    kmax = min(kmax, n, ns)
+    phi = np.zeros((kmax, n))
+    sig = np.zeros(ns)
+    k = C.c_uint64()
+    _check(lib().shaw_pod_basis_upto(_p(S), n, ns, kmax, _p(phi), _p(sig), C.byref(k)))
+    return phi[: k.value].T, sig
+
+
+def reconstruct(phi: np.ndarray, q: np.ndarray) -> np.ndarray:
+    """phi [n x k] @ q [k x ncols] on the GPU (FP64, sums in mode order, no FMA)."""
+    phi = np.asfortranarray(phi, dtype=np.float64)
+    q = np.asfortranarray(q, dtype=np.float64)
+    n, k = phi.shape
+    assert q.shape[0] == k
+    out = np.zeros((q.shape[1], n))  # column-major [n x ncols]
+    _check(lib().shaw_reconstruct(_p(phi), n, k, _p(q), q.shape[1], _p(out)))
+    return out.T
 
 
 def debug_copy(nbytes: int, reps: int = 10) -> float:

@@ -969,7 +969,7 @@ class FomEngine final : public shaw_fom {
       SHAW_HIP(hipGetLastError());
       SHAW_HIP(hipDeviceSynchronize());
     }
-    podDevice(s.buf.p, n, T_ * s.ncols, k, phi, sigma);
+    podDevice(s.buf.p, n, T_ * s.ncols, k, phi, sigma, nullptr);
     if (w)
       for (uint64_t c = 0; c < k; ++c)
         for (uint64_t i = 0; i < n; ++i) phi[i + n * c] /= sw[i];

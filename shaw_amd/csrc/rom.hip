@@ -699,7 +699,7 @@ void launchRomSteps(int tpw, dim3 grid, size_t lds, hipStream_t stream, const Ro
 namespace shaw {
 // POD of device snapshots dS [n x ns] column-major (overwritten by the
 // deflation); phi (host, [n x k] column-major) and sigma (host, ns values).
-void podDevice(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* phi, double* sigma) {
+void podDevice(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* phi, double* sigma, uint64_t* kOut) {
     SHAW_REQUIRE(dSp && phi && n > 0 && ns > 0 && k > 0, SHAW_ERR_ARG, "bad argument");
     SHAW_REQUIRE(k <= ns && k <= n, SHAW_ERR_ARG, "POD: number of modes exceeds the number of snapshots");
     SHAW_REQUIRE(ns <= 4096, SHAW_ERR_ARG, "POD: at most 4096 snapshots");
@@ -801,13 +801,15 @@ void podDevice(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* phi, do
       gramTN(dNew.p, (uint32_t)r, dS.p, (uint32_t)ns, Y);
       applyNN(dNew.p, (uint32_t)r, Y, (uint32_t)ns, dS.p, dS.p, -1.0);
     }
-    SHAW_REQUIRE(have == k, SHAW_ERR_ARG,
+    // kOut: up to k modes (stop at the numerical rank); else exactly k
+    SHAW_REQUIRE(kOut ? have > 0 : have == k, SHAW_ERR_ARG,
                  "POD: requested modes exceed the numerical rank of the snapshots (" + std::to_string(have) +
                      " resolved)");
     if (sigma) {
       for (uint64_t i = 0; i < ns; ++i) sigma[i] = i < have ? sv[i] : (i - have < lastTail.size() ? lastTail[i - have] : 0.0);
     }
-    SHAW_HIP(hipMemcpy(phi, dPhi.p, n * k * 8, hipMemcpyDeviceToHost));
+    SHAW_HIP(hipMemcpy(phi, dPhi.p, n * have * 8, hipMemcpyDeviceToHost));
+    if (kOut) *kOut = have;
 }
 }  // namespace shaw
 
@@ -1195,7 +1197,48 @@ int shaw_pod_basis(const double* S, uint64_t n, uint64_t ns, uint64_t k, double*
     DevBuf<double> dS;
     dS.alloc(n * ns);
     dS.upload(S, n * ns);
-    podDevice(dS.p, n, ns, k, phi, sigma);
+    podDevice(dS.p, n, ns, k, phi, sigma, nullptr);
+  });
+}
+
+int shaw_pod_basis_upto(const double* S, uint64_t n, uint64_t ns, uint64_t kmax, double* phi, double* sigma,
+                        uint64_t* k) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(S && phi && k && n > 0 && ns > 0 && kmax > 0, SHAW_ERR_ARG, "bad argument");
+    int ndev = 0;
+    SHAW_HIP(hipGetDeviceCount(&ndev));
+    SHAW_REQUIRE(ndev > 0, SHAW_ERR_HIP, "no HIP device available");
+    DevBuf<double> dS;
+    dS.alloc(n * ns);
+    dS.upload(S, n * ns);
+    podDevice(dS.p, n, ns, std::min(kmax, std::min(n, ns)), phi, sigma, k);
+  });
+}
+
+int shaw_reconstruct(const double* phi, uint64_t n, uint64_t k, const double* q, uint64_t ncols, double* out) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(phi && q && out && n > 0 && k > 0 && ncols > 0, SHAW_ERR_ARG, "bad argument");
+    SHAW_REQUIRE(k <= 0xffffffffull && ncols <= 0xffffffffull, SHAW_ERR_ARG, "too many modes or columns");
+    SHAW_REQUIRE(ncols <= 65535ull * NNC, SHAW_ERR_ARG, "at most 1048560 columns per call");
+    int ndev = 0;
+    SHAW_HIP(hipGetDeviceCount(&ndev));
+    SHAW_REQUIRE(ndev > 0, SHAW_ERR_HIP, "no HIP device available");
+    // q column-major [k x ncols] -> row-major, the layout k_gemm_nn_rows reads
+    std::vector<double> w((size_t)k * ncols);
+    for (uint64_t c = 0; c < ncols; ++c)
+      for (uint64_t j = 0; j < k; ++j) w[j * ncols + c] = q[j + k * c];
+    DevBuf<double> dPhi, dW, dOut;
+    dPhi.alloc(n * k);
+    dPhi.upload(phi, n * k);
+    dW.alloc(w.size());
+    dW.upload(w.data(), w.size());
+    dOut.alloc(n * ncols);
+    hipLaunchKernelGGL(k_gemm_nn_rows, dim3((unsigned)((n + 255) / 256), (unsigned)((ncols + NNC - 1) / NNC)),
+                       dim3(256), 0, 0, dPhi.p, dW.p, dOut.p, nullptr, 1.0, n, (uint32_t)k, (uint32_t)ncols);
+    SHAW_HIP(hipGetLastError());
+    SHAW_HIP(hipMemcpy(out, dOut.p, n * ncols * 8, hipMemcpyDeviceToHost));
   });
 }
 

@@ -5,5 +5,7 @@
 namespace shaw {
 // POD of device-resident snapshots dS [n x ns] column-major (overwritten);
 // phi: host [n x k] column-major, sigma: host [ns] (may be null).
-void podDevice(double* dS, uint64_t n, uint64_t ns, uint64_t k, double* phi, double* sigma);
+// kOut == nullptr: exactly k modes or an error; else up to k modes, stopping at
+// the numerical rank, with the count written to *kOut
+void podDevice(double* dS, uint64_t n, uint64_t ns, uint64_t k, double* phi, double* sigma, uint64_t* kOut);
 }  // namespace shaw

@@ -83,6 +83,11 @@ class TextFile {
     if (k == 0 || *end) bad("number");
     return v;
   }
+  // a number anywhere ahead (line breaks are separators too)
+  double realAcrossLines() {
+    skipBlank();
+    return real();
+  }
   void endLine() {
     skipSpaces();
     if (i_ < n_ && p_[i_] != '\n') bad("end of line");
@@ -718,7 +723,7 @@ std::vector<double> readMatrix(const std::string& file, bool binary, uint64_t& m
   n = (uint64_t)f.integer();
   A.resize(m * n);
   for (uint64_t i = 0; i < m; ++i)
-    for (uint64_t j = 0; j < n; ++j) A[i + m * j] = f.real();
+    for (uint64_t j = 0; j < n; ++j) A[i + m * j] = f.realAcrossLines();
   return A;
 }
 
