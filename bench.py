@@ -46,10 +46,20 @@ def measured_traffic(kernel_cfg: dict):
     kv = {k.strip(): v.strip() for k, v in kv.items()}
     want = (f"cols_per_thread={kernel_cfg['cols_per_thread']} strip={kernel_cfg['strip']} "
             f"segment={kernel_cfg['segment']} prefetch_rows={kernel_cfg['prefetch_rows']} "
-            f"steps_per_launch={kernel_cfg['steps_per_launch']}")
+            f"steps_per_launch={kernel_cfg['steps_per_launch']} col_block={kernel_cfg['col_block']}")
     if kv.get("kernel_config") != want:
         return None
     return float(kv["traffic_bytes_per_launch"])
+
+
+def fused_kernel_name(cfg: dict) -> str:
+    """The FOM kernel a launch of this configuration runs."""
+    spl, cpt = cfg["steps_per_launch"], cfg["cols_per_thread"]
+    if spl == 1:
+        return f"k_fused<double,{cpt},{cfg['prefetch_rows']}>"
+    if spl == 2:
+        return f"k_fused2<double,{cpt}>"
+    return f"k_fusedN<double,{cpt},{spl}>"
 
 
 def log(msg: str) -> None:
@@ -325,7 +335,8 @@ def config5_section(args) -> dict:
         fom.close()
         # error run from zero state over the first n_err steps
         fom = shaw_amd.Fom(app, w.m, w.dt, precision=prec, rank1_forcing=False, cols_per_thread=cfg["cols_per_thread"],
-                           strip=cfg["strip"], segment=cfg["segment"], prefetch_rows=cfg["prefetch_rows"])
+                           strip=cfg["strip"], segment=cfg["segment"], prefetch_rows=cfg["prefetch_rows"],
+                           steps_per_launch=cfg["steps_per_launch"], col_block=cfg["col_block"])
         fom.set_sources(gids)
         fom.set_forcing(tab)
         fom.set_receivers(rec, w.seis_every, n_err // w.seis_every)
@@ -354,6 +365,7 @@ def main() -> int:
     ap.add_argument("--segment", type=int, default=0)
     ap.add_argument("--prefetch", type=int, default=0)
     ap.add_argument("--steps-per-launch", type=int, default=0)
+    ap.add_argument("--col-block", type=int, default=0, help="FOM columns stored together per grid point")
     ap.add_argument("--copy-bytes", type=int, default=4 << 30)
     ap.add_argument("--no-rom", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -385,7 +397,7 @@ def main() -> int:
     kern = shaw_amd.KERNEL_FUSED if args.kernel == "fused" else shaw_amd.KERNEL_SPLIT
     fom = shaw_amd.Fom(app, w.m, w.dt, kernel=kern, rank1_forcing=False, cols_per_thread=args.cpt,
                        strip=args.strip, segment=args.segment, prefetch_rows=args.prefetch,
-                       steps_per_launch=args.steps_per_launch)
+                       steps_per_launch=args.steps_per_launch, col_block=args.col_block)
     fom.set_sources(gids)
     fom.set_forcing(tab)
     fom.set_receivers([app.map_receiver(a) for a in workloads.RECEIVERS], w.seis_every,
@@ -399,7 +411,7 @@ def main() -> int:
     assert np.isfinite(seis).all(), "seismogram is not finite"
     kernel_bytes, model_bytes = fom.bytes_per_step()
     cfg = fom.config()
-    spl = cfg["steps_per_launch"]  # a launch of the two-step kernel advances two steps
+    spl = cfg["steps_per_launch"]  # a launch of the k-step kernel advances k steps
     value = w.m * args.steps / (total_ms / 1000.0)
     achieved = kernel_bytes / (ms_per_step / 1000.0) / 1e9  # = bytes per launch / launch time
     log(f"{total_ms:.1f} ms for {args.steps} steps -> {ms_per_step:.4f} ms/step, {value:.0f} forcing-steps/s, "
@@ -422,9 +434,7 @@ def main() -> int:
                    "kernel": cfg, "fom_variant": "plain FP64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(fom.config()),
-                     "kernel": ((f"k_fused2<double,{cfg['cols_per_thread']}>" if spl == 2 else
-                                 f"k_fused<double,{cfg['cols_per_thread']},{cfg['prefetch_rows']}>")
-                                if args.kernel == "fused" else "k_velocity+k_stress"),
+                     "kernel": (fused_kernel_name(cfg) if args.kernel == "fused" else "k_velocity+k_stress"),
                      "steps_per_launch": spl,
                      "bytes_per_launch": kernel_bytes * spl,
                      "launch_ms": ms_per_step * spl,

@@ -127,12 +127,18 @@ typedef struct {
   int strip;            /* 0 = default; fused kernel: vp points per block row (incl. halo) */
   int segment;          /* 0 = default; fused kernel: radial rows per block */
   int prefetch_rows;    /* 0 = default; fused kernel: rows loaded ahead, 1 or 2 */
-  int steps_per_launch; /* 0 = default (2 unless prefetch_rows = 2); fused kernel: 1, or 2
-                           (two leapfrog steps per launch, level n+1 kept on chip;
-                           snapshots at the intermediate step fall back to single steps) */
-  int graph_steps;      /* 0 = default (64); < 0: no graphs. Fused kernel without
-                           snapshots: whole chunks of this many steps replay one captured
-                           HIP graph (kernels read their step from a device counter)   */
+  int steps_per_launch; /* 0 = default (2 unless prefetch_rows = 2); fused kernel: 1..4
+                           leapfrog steps per launch, the intermediate levels kept on
+                           chip (strip >= 2 x steps); a pack with a snapshot due at an
+                           intermediate step falls back to fewer steps per launch     */
+  int graph_steps;      /* 0 = default (64, rounded up to an even number of launches);
+                           < 0: no graphs. Fused kernel without snapshots: whole chunks
+                           of this many steps replay one captured HIP graph (kernels
+                           read their step from a device counter); a multiple of
+                           2 x steps_per_launch                                       */
+  int col_block;        /* 0 = default; columns stored together per grid point (power of
+                           two, at most 256 bytes of state): smaller blocks give more,
+                           smaller workgroups                                          */
 } shaw_fom_options;
 
 /* app: operators; m: forcing columns; dt: time step. opts may be NULL. */
@@ -168,12 +174,15 @@ int shaw_fom_set_state(shaw_fom_t h, const double* vp, const double* sp);
 /* Time `nsteps` steps with HIP events on the handle's stream: total ms and
  * the average duration of one leapfrog step (all its kernels).            */
 int shaw_fom_time(shaw_fom_t h, uint64_t nsteps, double* total_ms, double* ms_per_step);
-/* Algorithmic HBM bytes of one leapfrog step of the selected kernel, and
- * the reference model s*m*3*(nvp+nsp) + 8*(4*nvp + 2*nsp).               */
+/* Algorithmic HBM bytes of one leapfrog step of the selected kernel (the
+ * state read and written once per launch, the coefficients read once per
+ * launch, divided by the steps per launch), and the reference model
+ * s*m*3*(nvp+nsp) + 8*(4*nvp + 2*nsp).                                    */
 int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel_bytes, double* model_bytes);
-/* Kernel configuration actually used: kernel, cols_per_thread, strip, segment. */
+/* Kernel configuration actually used: kernel, cols_per_thread, strip, segment,
+ * prefetch_rows, steps_per_launch, col_block (any pointer may be null). */
 int shaw_fom_config(shaw_fom_t h, int* kernel, int* cols_per_thread, int* strip, int* segment,
-                    int* prefetch_rows, int* steps_per_launch);
+                    int* prefetch_rows, int* steps_per_launch, int* col_block);
 /* Tile autotuner of the fused kernel (no counterpart in the reference, whose
  * launch shapes are fixed Kokkos policies): times every valid (columns per
  * thread, strip, segment, rows prefetched, steps per launch) for

@@ -49,7 +49,8 @@ class _AppInfo(C.Structure):
 class _FomOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("kernel", C.c_int), ("rank1_forcing", C.c_int),
                 ("cols_per_thread", C.c_int), ("strip", C.c_int), ("segment", C.c_int),
-                ("prefetch_rows", C.c_int), ("steps_per_launch", C.c_int), ("graph_steps", C.c_int)]
+                ("prefetch_rows", C.c_int), ("steps_per_launch", C.c_int), ("graph_steps", C.c_int),
+                ("col_block", C.c_int)]
 
 
 class _RomDesc(C.Structure):
@@ -103,7 +104,7 @@ def lib():
         "shaw_fom_set_state": (i32, [vp, vp, vp]),
         "shaw_fom_time": (i32, [vp, u64, P(dbl), P(dbl)]),
         "shaw_fom_bytes_per_step": (i32, [vp, P(dbl), P(dbl)]),
-        "shaw_fom_config": (i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i32), P(i32)]),
+        "shaw_fom_config": (i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i32), P(i32), P(i32)]),
         "shaw_fom_autotune": (i32, [vp, u64, P(C.c_double)]),
         "shaw_fom_destroy": (None, [vp]),
         "shaw_fom_pod": (i32, [vp, i32, u64, vp, vp]),
@@ -271,11 +272,11 @@ class Fom:
 
     def __init__(self, app: App, m: int, dt: float, precision=FP64, kernel=KERNEL_AUTO,
                  rank1_forcing=None, cols_per_thread=0, strip=0, segment=0, prefetch_rows=0, steps_per_launch=0,
-                 graph_steps=0):
+                 graph_steps=0, col_block=0):
         if rank1_forcing is None:
             rank1_forcing = m == 1
         o = _FomOptions(precision, kernel, int(bool(rank1_forcing)), cols_per_thread, strip, segment, prefetch_rows,
-                        steps_per_launch, graph_steps)
+                        steps_per_launch, graph_steps, col_block)
         h = C.c_void_p()
         _check(lib().shaw_fom_create(app.handle, m, dt, C.byref(o), C.byref(h)))
         self._h, self.app, self.m, self.dt = h, app, m, dt
@@ -389,10 +390,10 @@ class Fom:
         return ms.value
 
     def config(self):
-        v = [C.c_int() for _ in range(6)]
+        v = [C.c_int() for _ in range(7)]
         _check(lib().shaw_fom_config(self._h, *[C.byref(x) for x in v]))
         return {"kernel": v[0].value, "cols_per_thread": v[1].value, "strip": v[2].value, "segment": v[3].value,
-                "prefetch_rows": v[4].value, "steps_per_launch": v[5].value}
+                "prefetch_rows": v[4].value, "steps_per_launch": v[5].value, "col_block": v[6].value}
 
 
 def fom_run(app: App, dt: float, nsteps: int, sources, rank2: bool, receivers=(), seis_freq=0,

@@ -558,6 +558,308 @@ __global__ void __launch_bounds__(512) k_fused2(StepArgs2<T> aa) {
   }
 }
 
+// ------------------------------------------------------------ NS fused steps
+// One launch advances NS leapfrog steps (NS = 3 or 4), n -> n+NS; the
+// intermediate levels live only in registers and LDS, so the states are read
+// and written once per NS steps. A block owns Q = P - 2 NS + 1 theta points
+// of a strip (lanes: NS-1 west halo points, Q owned, NS east halo points)
+// and SEG radial rows. Marching up input row r, level l = 1..NS computes
+//   v_l(r-l+1) = v_{l-1}(r-l+1) + dt J_vp s_{l-1} + f(step n+l)
+//   s_l(r-l)   = s_{l-1}(r-l)   + dt J_sp v_l
+// each as the one-step kernel evaluates it, in the same order, so NS fused
+// steps are bit-identical to NS single steps. Level l is valid on lanes
+// l-1 .. P-l (velocity, srp) and l-1 .. P-l-1 (stp), and from row
+// j0-(NS-l) (recomputed halo rows below the segment). The coefficient rows
+// of the last NS input rows sit in an LDS ring (loaded once per grid point
+// rather than once per column thread: they cost no registers), and the stp
+// rows that consecutive levels exchange alternate between two LDS buffers.
+template <class T, int NS>
+struct StepArgsN {
+  StepArgs<T> s[NS];  // the steps' arguments (forcing rows, receiver sampling)
+};
+
+// ColumnForcing for NS steps. The source of a column is hit when the lane's
+// grid column is the source's and the row matches: a 32-bit row compare per
+// level instead of a 64-bit gid compare. One increment per step, each the
+// one-step kernel's expression.
+template <class T, int CPT, int NS>
+struct ColumnForcingN {
+  uint32_t row[CPT];  // the source's grid row if it lies in this lane's grid column, else ~0u
+  T inc[NS][CPT];
+  __device__ __forceinline__ void load(const StepArgsN<T, NS>& aa, uint32_t colBase, uint32_t i, bool active) {
+    const StepArgs<T>& a = aa.s[0];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const uint32_t col = colBase + k;
+      const int64_t g = col < a.M ? a.src[col] : -1;
+      row[k] = ~0u;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) inc[l][k] = T(0);
+      if (g >= 0) {
+        if (active && (uint32_t)(g % a.nth) == i) row[k] = (uint32_t)(g / a.nth);
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          const T f = (T)aa.s[l].frow[col];
+          inc[l][k] = a.rank1 ? a.dt * a.srcRhoInv[col] * f : a.srcRhoInv[col] * f * a.dt;
+        }
+      }
+    }
+  }
+  __device__ __forceinline__ void apply(int l, T (&vn)[CPT], uint32_t r) const {
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const T added = vn[k] + inc[l][k];
+      vn[k] = row[k] == r ? added : vn[k];
+    }
+  }
+};
+
+// old state of one input row for one thread (coefficients go to the LDS ring)
+template <class T, int CPT>
+struct RowState {
+  T N[CPT], E[CPT], vo[CPT], W[CPT];
+};
+
+// Addressing: every row index is block-uniform and every offset within a
+// column block fits 32 bits (the host checks nSp*MC), so a load address is a
+// scalar row offset plus a per-lane offset fixed for the whole launch.
+template <class T, int CPT, int NS>
+__global__ void __launch_bounds__(512) k_fusedN(StepArgsN<T, NS> aa) {
+#pragma unroll
+  for (int l = 0; l < NS; ++l) resolveStep(aa.s[l]);
+  const StepArgs<T>& a = aa.s[0];
+  const uint32_t P = a.P, MC = a.MC;
+  extern __shared__ unsigned char smem_raw[];
+  // the ring first: its 8-value entries stay aligned for any MC
+  T* ring = reinterpret_cast<T*>(smem_raw);  // [NS][P][8]: J_vp row rho, J_sp rows rho-1 of input row rho
+  T* stpEven = ring + NS * P * 8;            // [P+1][MC]: stp of levels 0, 2 (lane pl at pl+1; level 0's west load at 0)
+  T* stpOdd = stpEven + (P + 1) * MC;        // [P+1][MC]: stp of levels 1, 3
+  T* vRow = stpOdd + (P + 1) * MC;           // [NS][P][MC]: v_l of the previous input row
+
+  const unsigned lb = xcdLogicalBlock(blockIdx.x, gridDim.x);
+  const uint32_t strip = lb % a.nStrips;
+  const uint32_t seg = (lb / a.nStrips) % a.nSegs;
+  const uint32_t cb = lb / (a.nStrips * a.nSegs);
+
+  const uint32_t tpp = MC / CPT;
+  const uint32_t pl = threadIdx.x / tpp;
+  const uint32_t m0 = (threadIdx.x % tpp) * CPT;
+  const uint32_t nr = a.nr, nth = a.nth, rs = 2 * nth - 1;
+  const uint32_t Q = P - (2 * NS - 1);  // owned points per strip
+  const int32_t ii = (int32_t)(strip * Q) - (NS - 1) + (int32_t)pl;
+  const bool active = ii >= 0 && ii < (int32_t)nth;
+  const uint32_t i = active ? (uint32_t)ii : 0;
+  const bool owner = active && pl >= NS - 1 && pl < NS - 1 + Q;
+  const uint32_t j0 = seg * a.SEG;
+  const uint32_t j1 = j0 + a.SEG < nr ? j0 + a.SEG : nr;
+
+  const uint64_t nVp = (uint64_t)nr * nth, nSp = (uint64_t)(nr - 1) * nth + (uint64_t)nr * (nth - 1);
+  const T* vIn = a.vIn + (uint64_t)cb * nVp * MC;
+  T* vOut = a.vOut + (uint64_t)cb * nVp * MC;
+  const T* sIn = a.sIn + (uint64_t)cb * nSp * MC;
+  T* sOut = a.sOut + (uint64_t)cb * nSp * MC;
+
+  // per-lane offsets (clamped, so every lane loads in bounds and no branch
+  // separates a load from its use); a v row is nth*MC values, a stress row
+  // rs*MC, and the surface stress row nr-1 holds stp only
+  const uint32_t ic = active ? i : (ii < 0 ? 0 : nth - 1);
+  const uint32_t iE = ic < nth - 2 ? ic : nth - 2;        // east stp column
+  const uint32_t iW = pl == 0 ? (ic > 0 ? ic - 1 : 0) : iE;  // west stp column (strip start only)
+  const uint32_t vStride = nth * MC, sStride = rs * MC, topS = (nr - 1) * sStride;
+  const uint32_t oV = ic * MC + m0, oSrp = 2 * ic * MC + m0;
+  const uint32_t oStpE = (2 * iE + 1) * MC + m0, oStpETop = iE * MC + m0;
+  const uint32_t oStpW = (2 * iW + 1) * MC + m0, oStpWTop = iW * MC + m0;
+  const uint32_t cvStride = 4 * nth, csStride = 2 * rs, csTop = (nr - 1) * csStride;
+  const uint32_t cV = 4 * ic, cSrp = 4 * ic, cStp = 2 * (2 * iE + 1), cStpTop = 2 * iE;
+
+  T cf[8];  // coefficients of the next input row, on their way to the ring
+  auto fetch = [&](RowState<T, CPT>& d, uint32_t r) {
+    const uint32_t rc = r < nr ? r : nr - 1;
+    const uint32_t rn = rc < nr - 1 ? rc : nr - 2;  // north srp row (the surface's is row nr-2)
+    const bool top = rc + 1 >= nr;
+    load(d.N, sIn + (rn * sStride + oSrp));
+    load(d.E, sIn + (top ? topS + oStpETop : rc * sStride + oStpE));
+    load(d.vo, vIn + (rc * vStride + oV));
+    load(d.W, sIn + (top ? topS + oStpWTop : rc * sStride + oStpW));
+    T c4[4], bS[2], bT[2];
+    load(c4, a.cv + (rc * cvStride + cV));
+    // stress row r - 1, from the unclamped r (r = nr still has stress row nr - 1)
+    const uint32_t rb = r == 0 ? 0 : (r - 1 < nr ? r - 1 : nr - 1);
+    const uint32_t rbn = rb < nr - 1 ? rb : nr - 2;
+    load(bS, a.cs + (rbn * csStride + cSrp));
+    load(bT, a.cs + (rb + 1 >= nr ? csTop + cStpTop : rb * csStride + cStp));
+    cf[0] = c4[0]; cf[1] = c4[1]; cf[2] = c4[2]; cf[3] = c4[3];
+    cf[4] = bS[0]; cf[5] = bS[1]; cf[6] = bT[0]; cf[7] = bT[1];
+  };
+
+  ColumnForcingN<T, CPT, NS> frc;
+  frc.load(aa, cb * MC + m0, i, active);
+  // carries from the previous input row r-1: S[l], E[l] = srp, stp of level
+  // l at row r-1-l (l = 0..NS-1); vj[l-1] = v_l at row r-l (l = 1..NS)
+  T S[NS][CPT], E[NS][CPT], vj[NS][CPT];
+#pragma unroll
+  for (int l = 0; l < NS; ++l)
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) S[l][k] = E[l][k] = vj[l][k] = T(0);
+  // first row of level l: j0 - (NS - l), at least 0; the first input row is lo[1]
+  uint32_t lo[NS + 1];
+#pragma unroll
+  for (int l = 0; l <= NS; ++l) lo[l] = j0 >= (uint32_t)(NS - l) ? j0 - (NS - l) : 0;
+  const uint32_t rFirst = lo[1];
+  {  // S[0] is srp of the row below the first input row
+    const uint32_t rb = rFirst > 0 ? (rFirst - 1 < nr - 1 ? rFirst - 1 : nr - 2) : 0;
+    load(S[0], sIn + (rb * sStride + oSrp));
+  }
+  if (pl == 0) {
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) stpOdd[m0 + k] = T(0);
+  }
+  // receivers, compacted so the row loop keeps few scalars live: every
+  // sampling level reads the same receiver table; a mask bit per level
+  const int64_t recvRow = a.recvRow;
+  const uint32_t colBase = cb * MC + m0;
+  const int32_t* recvTab = nullptr;
+  uint32_t recMask = 0, sample[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    if (aa.s[l].recv) { recvTab = aa.s[l].recv; recMask |= 1u << l; }
+    sample[l] = (uint32_t)aa.s[l].sample;
+  }
+  double* const seis = a.seis;
+  const uint32_t R = (uint32_t)a.R, Tcap = (uint32_t)a.Tcap, M = a.M;
+  auto record = [&](int l, const T (&vn)[CPT], uint32_t p) {
+    const int32_t q = recvTab[p];
+    if (q < 0) return;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const uint32_t col = colBase + k;
+      if (col < M) seis[(uint64_t)q + (uint64_t)R * (sample[l] + (uint64_t)Tcap * col)] = (double)vn[k];
+    }
+  };
+
+  auto row = [&](const RowState<T, CPT>& cur, RowState<T, CPT>& nxt, uint32_t r) {
+    store(stpEven + (pl + 1) * MC + m0, cur.E);
+    if (pl == 0) store(stpEven + m0, cur.W);
+    const uint32_t slot = r % NS;
+    if (m0 == 0) store(ring + (slot * P + pl) * 8, cf);
+    fetch(nxt, r + 1);
+    __syncthreads();  // level-0 stp and the coefficient row of r visible
+    T vNew[NS][CPT], srpN[NS][CPT], stpN[NS][CPT];
+#pragma unroll
+    for (int l = 1; l <= NS; ++l) {
+      const T* stpIn = ((l - 1) & 1) ? stpOdd : stpEven;
+      T* stpOut = (l & 1) ? stpOdd : stpEven;
+      // velocity of level l at row rv (input row rv's coefficients: ring slot (r+1-l) mod NS)
+      const bool past = r + 1 >= (uint32_t)l;
+      const uint32_t rv = r + 1 - l;
+      const bool rowV = past && rv >= lo[l] && rv < nr;
+      const bool doV = l == 1 || (active && pl + 1 >= (uint32_t)l && pl + l <= P);
+      const uint32_t slotL = (slot + NS - (l - 1)) % NS;
+      T cl[8];
+      load(cl, ring + (slotL * P + pl) * 8);
+      // Row conditions are block-uniform branches; lane conditions select
+      // (every lane of a wave executes either way, and an LDS read of an
+      // unused lane stays inside its array).
+      T v[CPT];
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) v[k] = T(0);
+      if (rowV) {
+        T W[CPT];
+        load(W, stpIn + pl * MC + m0);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+          // level 1 reads the loaded row; level l > 1 the level below, this
+          // row (north srp, own stp) and the previous row (south srp, v)
+          const T Nk = l == 1 ? cur.N[k] : srpN[l - 1][k];
+          const T Sk = S[l - 1][k];
+          const T Ek = l == 1 ? cur.E[k] : stpN[l - 1][k];
+          const T vo = l == 1 ? cur.vo[k] : vj[l >= 2 ? l - 2 : 0][k];
+          T sum = cl[0] * Nk + cl[1] * Sk;
+          sum = sum + cl[2] * W[k];
+          sum = sum + cl[3] * Ek;
+          v[k] = vo + a.dt * sum;
+        }
+        frc.apply(l - 1, v, rv);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) v[k] = doV ? v[k] : T(0);
+        if (rv >= j0 && rv < j1) {
+          if (((recMask >> (l - 1)) & 1u) && (recvRow < 0 || (int64_t)rv == recvRow) && owner)
+            record(l - 1, v, rv * nth + i);
+          if (l == NS && owner) store(vOut + (rv * vStride + oV), v);
+        }
+      }
+      // stress of level l at row rsr = r - l (srp needs v_l rows rsr, rsr+1;
+      // stp needs v_l(i+1, rsr) of the previous input row, from LDS)
+      const uint32_t rsr = r - l;
+      const bool rowS = r >= (uint32_t)l && rsr >= lo[l] && rsr < nr;
+      const bool doStp = active && pl + 1 >= (uint32_t)l && pl + l + 1 <= P && i + 1 < nth;
+      if (l < NS) {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) { srpN[l][k] = S[l][k]; stpN[l][k] = T(0); }
+        if (rowS) {
+          if (rsr + 1 < nr) {
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) {
+              const T x = S[l - 1][k] + a.dt * (cl[4] * v[k] + cl[5] * vj[l - 1][k]);
+              srpN[l][k] = doV ? x : S[l][k];
+            }
+          }
+          T vE[CPT];  // lane P-1 reads the next level's first entry: unused, in bounds (l < NS)
+          load(vE, vRow + ((l - 1) * P + pl + 1) * MC + m0);
+#pragma unroll
+          for (int k = 0; k < CPT; ++k) {
+            const T y = E[l - 1][k] + a.dt * (cl[6] * vj[l - 1][k] + cl[7] * vE[k]);
+            stpN[l][k] = doStp ? y : T(0);
+          }
+        }
+        store(stpOut + (pl + 1) * MC + m0, stpN[l]);
+        __syncthreads();  // stp of level l visible to level l + 1
+      } else if (owner && rowS && rsr >= j0 && rsr < j1) {
+        if (rsr + 1 < nr) {
+          T so[CPT];
+#pragma unroll
+          for (int k = 0; k < CPT; ++k) so[k] = S[l - 1][k] + a.dt * (cl[4] * v[k] + cl[5] * vj[l - 1][k]);
+          store(sOut + (rsr * sStride + oSrp), so);
+        }
+        if (i + 1 < nth) {  // owner and i <= nth-2: iE == i
+          T vE[CPT], so[CPT];
+          load(vE, vRow + ((l - 1) * P + pl + 1) * MC + m0);
+#pragma unroll
+          for (int k = 0; k < CPT; ++k) so[k] = E[l - 1][k] + a.dt * (cl[6] * vj[l - 1][k] + cl[7] * vE[k]);
+          store(sOut + (rsr + 1 < nr ? rsr * sStride + oStpE : topS + oStpETop), so);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) vNew[l - 1][k] = v[k];
+    }
+    __syncthreads();  // readers of vRow and of the ring slot of row r+1-NS done
+#pragma unroll
+    for (int l = 0; l < NS; ++l) store(vRow + (l * P + pl) * MC + m0, vNew[l]);
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      S[0][k] = cur.N[k];
+      E[0][k] = cur.E[k];
+    }
+#pragma unroll
+    for (int l = 1; l < NS; ++l)
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) { S[l][k] = srpN[l][k]; E[l][k] = stpN[l][k]; }
+#pragma unroll
+    for (int l = 0; l < NS; ++l)
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) vj[l][k] = vNew[l][k];
+  };
+
+  // input rows rFirst .. j1 + NS - 1 (level NS's stress of row j1 - 1 happens at r = j1 - 1 + NS)
+  const uint32_t rLast = j1 + NS - 1;
+  RowState<T, CPT> A, B;
+  fetch(A, rFirst);
+  for (uint32_t r = rFirst; r <= rLast; r += 2) {
+    row(A, B, r);
+    if (r + 1 <= rLast) row(B, A, r + 1);
+  }
+}
+
 // ------------------------------------------------------------ gathers
 template <class T>
 __global__ void k_snapshot(const T* __restrict__ x, uint64_t n, uint32_t MC, const uint32_t* __restrict__ cols,
@@ -631,7 +933,7 @@ struct shaw_fom {
   virtual void setState(const double* vp, const double* sp) = 0;
   virtual void time(uint64_t nsteps, double* total, double* perStep) = 0;
   virtual void bytes(double* kernel, double* model) const = 0;
-  virtual void config(int* kernel, int* cpt, int* strip, int* seg, int* pf, int* spl) const = 0;
+  virtual void config(int* kernel, int* cpt, int* strip, int* seg, int* pf, int* spl, int* colBlock) const = 0;
   virtual void pod(int dof, uint64_t k, const double* w, double* phi, double* sigma) = 0;
   virtual void autotune(uint64_t trialSteps, double* bestMs) = 0;
 };
@@ -654,15 +956,22 @@ class FomEngine final : public shaw_fom {
     SHAW_REQUIRE(M >= 1 && M <= (1u << 20), SHAW_ERR_ARG, "forcing column count out of range");
     SHAW_REQUIRE(nr_ >= 2 && nth_ >= 3 && nVp_ < (1ull << 40), SHAW_ERR_ARG, "bad grid");
     // column block: MC*sizeof(T) <= 256 B, power of two >= M when M is small
+    // (or the requested power of two)
     const uint32_t cap = 256 / sizeof(T);
+    SHAW_REQUIRE(o.col_block >= 0 && (uint32_t)o.col_block <= cap && (o.col_block & (o.col_block - 1)) == 0,
+                 SHAW_ERR_ARG, "col_block must be a power of two of at most 256 bytes of state");
+    // default: 64 bytes of state per grid point (8 FP64 columns): 4 threads
+    // per point at 2 columns per thread, so 64-point strips make 256-thread
+    // workgroups, two per CU, whose barriers interleave
+    const uint32_t want = o.col_block > 0 ? (uint32_t)o.col_block : cap / 4;
     MC_ = 1;
-    while (MC_ < M && MC_ < cap) MC_ *= 2;
+    while (MC_ < M && MC_ < want) MC_ *= 2;
     nb_ = (uint32_t)((M + MC_ - 1) / MC_);
     kernel_ = o.kernel == SHAW_KERNEL_SPLIT ? SHAW_KERNEL_SPLIT : SHAW_KERNEL_FUSED;
     // defaults from sweeps of the fused kernels on MI355X: at 1024 x 4096,
-    // M = 32, 2 columns per thread, 32-point strips, 64-row segments and two
-    // steps per launch (one step per launch: 32-row segments); at 200 x 1000,
-    // M = 1, 4 and 16, the fastest segments give about 1700 wavefronts a launch
+    // M = 32, 8-column blocks, 2 columns per thread, 64-point strips, 64-row
+    // segments and four steps per launch; at 200 x 1000, M = 1, 4 and 16, the
+    // fastest segments give about 1700 wavefronts a launch
     cpt_ = o.cols_per_thread > 0 ? o.cols_per_thread : (int)std::min<uint32_t>(MC_, sizeof(T) == 8 ? 2 : 4);
     SHAW_REQUIRE(cpt_ == 1 || cpt_ == 2 || cpt_ == 4 || cpt_ == 8, SHAW_ERR_ARG, "cols_per_thread must be 1, 2, 4 or 8");
     SHAW_REQUIRE(MC_ % cpt_ == 0, SHAW_ERR_ARG, "cols_per_thread must divide the column block width");
@@ -672,24 +981,31 @@ class FomEngine final : public shaw_fom {
       SHAW_REQUIRE(P_ >= 2 && P_ * tpp <= 512, SHAW_ERR_ARG, "strip too wide for a 512-thread block");
       pf_ = o.prefetch_rows > 0 ? o.prefetch_rows : 1;
       SHAW_REQUIRE(pf_ == 1 || pf_ == 2, SHAW_ERR_ARG, "prefetch_rows must be 1 or 2");
-      spl_ = o.steps_per_launch > 0 ? o.steps_per_launch : (pf_ == 1 && P_ >= 4 ? 2 : 1);
-      graphSteps_ = o.graph_steps < 0 ? 0 : (o.graph_steps > 0 ? o.graph_steps : 64);
+      // four steps per launch where the strip and the 32-bit addressing allow
+      spl_ = o.steps_per_launch > 0 ? o.steps_per_launch
+                                    : (pf_ != 1 ? 1 : (P_ >= 8 && fits32() ? 4 : (P_ >= 4 ? 2 : 1)));
+      SHAW_REQUIRE(spl_ >= 1 && spl_ <= kMaxSpl, SHAW_ERR_ARG, "steps_per_launch must be 1, 2, 3 or 4");
+      SHAW_REQUIRE(spl_ == 1 || P_ >= 2u * spl_, SHAW_ERR_ARG,
+                   "k steps per launch need a strip of at least 2k points");
+      graphStepsOpt_ = o.graph_steps;
+      graphSteps_ = graphStepsFor(spl_);
       SHAW_REQUIRE(graphSteps_ % (2 * spl_) == 0, SHAW_ERR_ARG,
                    "graph_steps must be a multiple of 2 x steps per launch (an even number of launches)");
-      SHAW_REQUIRE(spl_ == 1 || spl_ == 2, SHAW_ERR_ARG, "steps_per_launch must be 1 or 2");
-      SHAW_REQUIRE(spl_ == 1 || P_ >= 4, SHAW_ERR_ARG, "two steps per launch need a strip of at least 4 points");
-      // segment: 64 rows (two steps per launch) or 32, halved while a launch
-      // has fewer than kMinWaves wavefronts (small grids and few columns:
-      // 200 x 1000 with M = 1 is fastest with 2-row segments)
+      // segment: 64 rows (several steps per launch) or 32, halved while a
+      // launch has fewer than kMinWaves wavefronts (small grids and few
+      // columns: 200 x 1000 with M = 1 is fastest with 2-row segments)
+      // (a segment of a k-step launch recomputes 2k-1 halo rows, so it is not
+      // halved below k rows)
       if (o.segment > 0) {
         SEG_ = (uint32_t)o.segment;
       } else {
-        SEG_ = spl_ == 2 ? 64 : 32;
-        while (SEG_ > 2 && launchWaves(cpt_, P_, SEG_, spl_) < kMinWaves) SEG_ /= 2;
+        SEG_ = spl_ >= 2 ? 64 : 32;
+        while (SEG_ > std::max(2, spl_) && launchWaves(cpt_, P_, SEG_, spl_) < kMinWaves) SEG_ /= 2;
       }
-      nStrips_ = (uint32_t)((nth_ + (P_ - 1) - 1) / (P_ - 1));
-      nStrips2_ = P_ >= 4 ? (uint32_t)((nth_ + (P_ - 3) - 1) / (P_ - 3)) : 0;
-      nSegs_ = (uint32_t)((nr_ + SEG_ - 1) / SEG_);
+      SHAW_REQUIRE(ldsBytes(cpt_, P_, spl_) <= kMaxLds, SHAW_ERR_ARG, "tile needs more LDS than a workgroup has");
+      SHAW_REQUIRE(spl_ <= 2 || fits32(), SHAW_ERR_ARG,
+                   "3 or 4 steps per launch need a column block of fewer than 2^32 stress values");
+      setStrips();
     }
     SHAW_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     SHAW_HIP(hipEventCreate(&ev0_));
@@ -813,16 +1129,26 @@ class FomEngine final : public shaw_fom {
         step_ += graphSteps_;
       }
     }
+    // eager launches: the widest pack (up to spl_ steps) that fits the
+    // remaining steps and has no snapshot due at an intermediate step
     for (; k < nsteps;) {
-      if (spl_ == 2 && k + 2 <= nsteps && !snapDue(step_ + 1)) {
-        launchPair();
-        k += 2;
-      } else {
-        launchStep();
-        k += 1;
+      int ns = spl_;
+      while (ns > 1 && (k + ns > nsteps || packBlocked(ns))) --ns;
+      switch (ns) {
+        case 1: launchStep(); break;
+        case 2: launchPair(); break;
+        case 3: launchPackN<3>(); break;
+        default: launchPackN<4>(); break;
       }
+      k += ns;
     }
     SHAW_HIP(hipGetLastError());
+  }
+  // a snapshot due at one of the intermediate steps of a pack of ns steps
+  bool packBlocked(int ns) const {
+    for (int j = 1; j < ns; ++j)
+      if (snapDue(step_ + j)) return true;
+    return false;
   }
   // A graph of graphSteps_ steps of the current shape, captured from the
   // current buffer parity; every launch reads its step from stepDev_ (set
@@ -842,6 +1168,14 @@ class FomEngine final : public shaw_fom {
         aa.s1 = deviceStepArgs(k + 1);
         aa.s2 = deviceStepArgs(k + 2);
         dispatchPair(aa, false);
+      } else if (spl_ == 3) {
+        StepArgsN<T, 3> aa;
+        for (int l = 0; l < 3; ++l) aa.s[l] = deviceStepArgs(k + 1 + l);
+        dispatchN<3>(aa, false);
+      } else if (spl_ == 4) {
+        StepArgsN<T, 4> aa;
+        for (int l = 0; l < 4; ++l) aa.s[l] = deviceStepArgs(k + 1 + l);
+        dispatchN<4>(aa, false);
       } else {
         StepArgs<T> a = deviceStepArgs(k + 1);
         dispatchStep(a, false);
@@ -876,6 +1210,15 @@ class FomEngine final : public shaw_fom {
       case 2: launchFused<2>(a, observe); break;
       case 4: launchFused<4>(a, observe); break;
       default: launchFused<8>(a, observe); break;
+    }
+  }
+  template <int NS>
+  void dispatchN(StepArgsN<T, NS>& aa, bool observe) {
+    switch (cpt_) {
+      case 1: launchFusedN<1, NS>(aa, observe); break;
+      case 2: launchFusedN<2, NS>(aa, observe); break;
+      case 4: launchFusedN<4, NS>(aa, observe); break;
+      default: launchFusedN<8, NS>(aa, observe); break;
     }
   }
   void dispatchPair(StepArgs2<T>& aa, bool observe) {
@@ -936,11 +1279,14 @@ class FomEngine final : public shaw_fom {
     *perStep = nsteps ? ms / (double)nsteps : 0.0;
   }
   void bytes(double* kernel, double* model) const override {
-    const double s = sizeof(T), cols = (double)nb_ * MC_;
-    const double coef = (4.0 * nVp_ + 2.0 * nSp_) * s * nb_;
+    // algorithmic minimum: the M state columns and the coefficients once
+    // (a launch over several column blocks re-reads the coefficients per
+    // block, and a strip re-reads its halo points: that is traffic above it)
+    const double s = sizeof(T), cols = (double)M_;
+    const double coef = (4.0 * nVp_ + 2.0 * nSp_) * s;
     if (kernel) {
       // fused: read + write each state once; split: v (r+w) + s (r) / v (r) + s (r+w)
-      // (per step: a launch of the two-step kernel moves these bytes for two steps)
+      // (per step: a launch of the k-step kernel moves these bytes for k steps)
       const double passes = kernel_ == SHAW_KERNEL_FUSED ? 2.0 : 3.0;
       *kernel = (s * cols * passes * (double)(nVp_ + nSp_) + coef) / (kernel_ == SHAW_KERNEL_FUSED ? spl_ : 1);
     }
@@ -975,7 +1321,8 @@ class FomEngine final : public shaw_fom {
         for (uint64_t i = 0; i < n; ++i) phi[i + n * c] /= sw[i];
     s.ncols = 0;  // the buffer was consumed
   }
-  void config(int* kernel, int* cpt, int* strip, int* seg, int* pf, int* spl) const override {
+  void config(int* kernel, int* cpt, int* strip, int* seg, int* pf, int* spl, int* colBlock) const override {
+    if (colBlock) *colBlock = (int)MC_;
     if (kernel) *kernel = kernel_;
     if (cpt) *cpt = cpt_;
     if (strip) *strip = (int)P_;
@@ -1052,9 +1399,8 @@ class FomEngine final : public shaw_fom {
     double bestT = 1e300;
     auto apply = [&](const Cfg& c) {
       cpt_ = c.cpt; P_ = c.P; SEG_ = c.SEG; pf_ = c.pf; spl_ = c.spl;
-      nStrips_ = (uint32_t)((nth_ + (P_ - 1) - 1) / (P_ - 1));
-      nStrips2_ = P_ >= 4 ? (uint32_t)((nth_ + (P_ - 3) - 1) / (P_ - 3)) : 0;
-      nSegs_ = (uint32_t)((nr_ + SEG_ - 1) / SEG_);
+      graphSteps_ = graphStepsFor(spl_);
+      setStrips();
     };
     for (int cpt : {1, 2, 4, 8}) {
       if (MC_ % cpt) continue;
@@ -1063,8 +1409,13 @@ class FomEngine final : public shaw_fom {
         if (P * tpp > 512 || P * tpp < 64) continue;
         for (uint32_t SEG : {2u, 4u, 8u, 16u, 32u, 64u})
           for (int pf : {1, 2})
-            for (int spl : {1, 2}) {
-              if (spl == 2 && pf == 2) continue;  // the two-step kernel loads one row ahead
+            for (int spl = 1; spl <= kMaxSpl; ++spl) {
+              if (spl >= 2 && pf == 2) continue;  // the multi-step kernels load one row ahead
+              if (spl >= 2 && P < 2u * spl) continue;
+              if (step_ + spl > ftabSteps_) continue;  // a trial reads the forcing of its steps
+              if (graphStepsFor(spl) % (2 * spl)) continue;  // a requested graph length must fit
+              if (ldsBytes(cpt, P, spl) > kMaxLds) continue;
+              if (spl >= 3 && !fits32()) continue;
               // short segments only where the grid is too small to fill the chip
               if (SEG < 16 && launchWaves(cpt, P, SEG, spl) > 16 * kMinWaves) continue;
               const Cfg c{cpt, P, SEG, pf, spl};
@@ -1087,9 +1438,41 @@ class FomEngine final : public shaw_fom {
     cur_ = cur0;
     if (bestMs) *bestMs = bestT;
   }
+  // graph chunk for `spl` steps per launch: the requested length, or 64
+  // rounded up to an even number of launches; 0 = no graphs
+  int graphStepsFor(int spl) const {
+    if (graphStepsOpt_ < 0) return 0;
+    if (graphStepsOpt_ > 0) return graphStepsOpt_;
+    return (64 + 2 * spl - 1) / (2 * spl) * (2 * spl);
+  }
+  // strips of every kernel for the current strip width and segment
+  void setStrips() {
+    nStrips_ = (uint32_t)((nth_ + (P_ - 1) - 1) / (P_ - 1));
+    for (int ns = 2; ns <= kMaxSpl; ++ns) {
+      const uint32_t q = P_ >= 2u * ns ? P_ - (2 * ns - 1) : 0;
+      nStripsK_[ns] = q ? (uint32_t)((nth_ + q - 1) / q) : 0;
+    }
+    nStrips2_ = nStripsK_[2];
+    nSegs_ = (uint32_t)((nr_ + SEG_ - 1) / SEG_);
+  }
+  // dynamic LDS of one fused launch
+  size_t ldsBytes(int cpt, uint32_t P, int spl) const {
+    (void)cpt;
+    if (spl == 1) return (size_t)(2 * P + 1) * MC_ * sizeof(T);
+    if (spl == 2) return (size_t)(4 * P + 2) * MC_ * sizeof(T);
+    return ldsBytesN(P, spl);
+  }
+  size_t ldsBytesN(uint32_t P, int ns) const {
+    return ((size_t)ns * P * 8 + 2 * (size_t)(P + 1) * MC_ + (size_t)ns * P * MC_) * sizeof(T);
+  }
+  // k_fusedN addresses a column block with 32-bit offsets
+  bool fits32() const {
+    const uint64_t lim = 1ull << 32;
+    return (uint64_t)nSp_ * MC_ + 2ull * nth_ * MC_ < lim && 4ull * nVp_ < lim && 2ull * nSp_ + 4ull * nth_ < lim;
+  }
   // wavefronts of one fused launch of the given shape
   uint64_t launchWaves(int cpt, uint32_t P, uint32_t SEG, int spl) const {
-    const uint32_t halo = spl == 2 ? 3 : 1;
+    const uint32_t halo = spl == 1 ? 1 : 2 * spl - 1;
     const uint64_t strips = (nth_ + (P - halo) - 1) / (P - halo), segs = (nr_ + SEG - 1) / SEG;
     const uint64_t wavesPerBlock = ((uint64_t)P * (MC_ / cpt) + 63) / 64;
     return strips * segs * nb_ * wavesPerBlock;
@@ -1097,7 +1480,10 @@ class FomEngine final : public shaw_fom {
   // one launch of the current shape (one step or a pair) that records
   // nothing and does not advance the run
   void trialLaunch() {
-    if (spl_ == 2) {
+    if (spl_ >= 3) {
+      if (spl_ == 3) trialPackN<3>();
+      else trialPackN<4>();
+    } else if (spl_ == 2) {
       StepArgs2<T> aa;
       pairArgs(aa);
       aa.s1.recv = aa.s2.recv = nullptr;
@@ -1111,6 +1497,14 @@ class FomEngine final : public shaw_fom {
     } else {
       trialStep();
     }
+  }
+  template <int NS>
+  void trialPackN() {
+    StepArgsN<T, NS> aa;
+    packArgs(aa);
+    for (int l = 0; l < NS; ++l) aa.s[l].recv = nullptr;
+    dispatchN<NS>(aa, false);
+    SHAW_HIP(hipGetLastError());
   }
   void trialStep() {
     StepArgs<T> a = args(step_ + 1);
@@ -1158,6 +1552,37 @@ class FomEngine final : public shaw_fom {
       snapshot(0, v_[cur_].p);
       snapshot(1, s_[cur_].p);
     }
+  }
+  template <int CPT, int NS>
+  void launchFusedN(StepArgsN<T, NS>& aa, bool observe = true) {
+    const uint32_t tpp = MC_ / CPT;
+    for (int l = 0; l < NS; ++l) {
+      StepArgs<T>& a = aa.s[l];
+      a.vIn = v_[cur_].p;
+      a.sIn = s_[cur_].p;
+      a.vOut = v_[cur_ ^ 1].p;
+      a.sOut = s_[cur_ ^ 1].p;
+      a.nStrips = nStripsK_[NS];
+    }
+    const unsigned blocks = nStripsK_[NS] * nSegs_ * nb_;
+    hipLaunchKernelGGL((k_fusedN<T, CPT, NS>), dim3(blocks), dim3(P_ * tpp), ldsBytesN(P_, NS), stream_, aa);
+    cur_ ^= 1;
+    if (observe) {
+      snapshot(0, v_[cur_].p);
+      snapshot(1, s_[cur_].p);
+    }
+  }
+  template <int NS>
+  void packArgs(StepArgsN<T, NS>& aa) {
+    for (int l = 0; l < NS; ++l) aa.s[l] = args(step_ + 1 + l);
+  }
+  template <int NS>
+  void launchPackN() {
+    StepArgsN<T, NS> aa;
+    packArgs(aa);
+    stepForSnap_ = step_ + NS;
+    dispatchN<NS>(aa, true);
+    step_ += NS;
   }
   void pairArgs(StepArgs2<T>& aa) {
     aa.s1 = args(step_ + 1);
@@ -1225,7 +1650,10 @@ class FomEngine final : public shaw_fom {
   DevBuf<uint64_t> stepDev_;
   hipGraphExec_t gexec_ = nullptr;
   int graphKey_[7] = {0, 0, 0, 0, 0, -1, 0};
-  int graphSteps_ = 64;
+  int graphSteps_ = 64, graphStepsOpt_ = 0;
+  uint32_t nStripsK_[5] = {0, 0, 0, 0, 0};  // strips of the k-step kernels (k = 2..4)
+  static constexpr int kMaxSpl = 4;
+  static constexpr size_t kMaxLds = 160 * 1024;
   static constexpr uint64_t kMinWaves = 1536;  // 6 per CU of the 256
   std::vector<double> rhoInv_;
   std::vector<std::pair<uint32_t, uint32_t>> dupRecv_;
@@ -1317,8 +1745,11 @@ int shaw_fom_time(shaw_fom_t h, uint64_t nsteps, double* total, double* perStep)
 int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel, double* model) {
   return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->bytes(kernel, model); });
 }
-int shaw_fom_config(shaw_fom_t h, int* kernel, int* cpt, int* strip, int* seg, int* pf, int* spl) {
-  return guarded([&] { SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle"); h->config(kernel, cpt, strip, seg, pf, spl); });
+int shaw_fom_config(shaw_fom_t h, int* kernel, int* cpt, int* strip, int* seg, int* pf, int* spl, int* col_block) {
+  return guarded([&] {
+    SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle");
+    h->config(kernel, cpt, strip, seg, pf, spl, col_block);
+  });
 }
 int shaw_fom_autotune(shaw_fom_t h, uint64_t trial_steps, double* best_ms_per_step) {
   return guarded([&] {

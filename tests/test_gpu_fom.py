@@ -29,7 +29,11 @@ KERNELS = [dict(kernel=shaw_amd.KERNEL_SPLIT), dict(kernel=shaw_amd.KERNEL_FUSED
            dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=1, strip=5, segment=1, steps_per_launch=1),
            dict(kernel=shaw_amd.KERNEL_FUSED, prefetch_rows=2, strip=6, segment=4),
            dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2),
-           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2, strip=5, segment=3)]
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2, strip=5, segment=3),
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=3),
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=3, strip=7, segment=2),
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=4),
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=4, cols_per_thread=1, strip=8, segment=5)]
 
 
 def say(msg):
@@ -58,7 +62,7 @@ def run_both(test, **kw):
 
 
 @pytest.mark.parametrize("kw", KERNELS, ids=["split", "fused", "fused-narrow", "fused-cpt1", "fused-pf2", "fused2",
-                                             "fused2-narrow"])
+                                             "fused2-narrow", "fused3", "fused3-narrow", "fused4", "fused4-min"])
 @pytest.mark.parametrize("test", FOM_TESTS)
 def test_regression_inputs_bit_exact(test, kw):
     say(f"[gpu] {test} {kw}")
@@ -101,6 +105,17 @@ def mesh64x256(tmp_path_factory):
                                             strip=16, segment=64)),
                                   (7, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2, strip=4, segment=2)),
                                   (7, dict(kernel=shaw_amd.KERNEL_FUSED)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=3)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=4)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=4, cols_per_thread=4,
+                                            strip=16, segment=7)),
+                                  (40, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=3, cols_per_thread=1,
+                                            strip=16, segment=64)),
+                                  (7, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=4, strip=9, segment=1)),
+                                  (3, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=3, strip=64, segment=2)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, col_block=32, steps_per_launch=4)),
+                                  (40, dict(kernel=shaw_amd.KERNEL_FUSED, col_block=8, steps_per_launch=4, strip=128)),
+                                  (40, dict(kernel=shaw_amd.KERNEL_FUSED, col_block=4, steps_per_launch=2)),
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED)),
                                   (40, dict(kernel=shaw_amd.KERNEL_SPLIT, cols_per_thread=2))])
 def test_random_rank2_bit_exact(mesh64x256, M, kw):
@@ -138,7 +153,7 @@ def test_autotune_keeps_state_and_parity(mesh64x256):
         np.testing.assert_array_equal(x, y)
     cfg = fom.config()
     assert ms > 0 and cfg["cols_per_thread"] in (1, 2, 4, 8) and cfg["prefetch_rows"] in (1, 2)
-    assert cfg["steps_per_launch"] in (1, 2)
+    assert cfg["steps_per_launch"] in (1, 2, 3, 4)
     assert cfg["segment"] in (2, 4, 8, 16, 32, 64) and cfg["strip"] in (8, 16, 32, 64, 128, 256)
     fom.close()
     rec = list(np.arange(0, 180, 7.5))
@@ -148,8 +163,8 @@ def test_autotune_keeps_state_and_parity(mesh64x256):
         np.testing.assert_array_equal(a[key], b[key], err_msg=key)
 
 
-@pytest.mark.parametrize("spl", [1, 2])
-def test_graph_replays_equal_eager(mesh64x256, spl):
+@pytest.mark.parametrize("spl,gsteps", [(1, 8), (2, 8), (3, 12), (4, 8)])
+def test_graph_replays_equal_eager(mesh64x256, spl, gsteps):
     """Chunks of 8 steps replay one captured HIP graph whose kernels read their
     step from a device counter. Checked against eager launches after every
     call: several replays, eager remainders, host state reads and a reset in
@@ -169,7 +184,7 @@ def test_graph_replays_equal_eager(mesh64x256, spl):
         f.set_receivers(rec, 3, n // 3)
         return f
 
-    g, e = make(8), make(-1)
+    g, e = make(gsteps), make(-1)
     for chunk in (30, 17, 8, 41):
         g.advance(chunk)
         e.advance(chunk)
@@ -209,17 +224,18 @@ def test_config1_shape_rank1_bit_exact(tmp_path):
 
 
 def test_default_segment_fills_the_chip():
-    """Default tile shapes: segments are halved until a launch has at least
-    1536 wavefronts (6 per CU), so 200 x 1000 with one column runs 2-row
-    segments while 1024 x 4096 with 32 columns keeps 64-row segments."""
+    """Default tile shapes: four steps per launch, and segments halved until a
+    launch has at least 1536 wavefronts (6 per CU) but not below four rows,
+    so 200 x 1000 with one column runs 4-row segments while 1024 x 4096 with
+    32 columns (two 16-column blocks) keeps 64-row segments."""
     say("[gpu] default segments")
     small = shaw_amd.Fom(shaw_amd.App.structured(200, 1000, LAYERED), 1, 0.25)
     cfg = small.config()
-    assert (cfg["strip"], cfg["segment"], cfg["steps_per_launch"]) == (64, 2, 2)
+    assert (cfg["strip"], cfg["segment"], cfg["steps_per_launch"]) == (64, 4, 4)
     small.close()
     big = shaw_amd.Fom(shaw_amd.App.structured(1024, 4096, LAYERED), 32, 0.05)
     cfg = big.config()
-    assert (cfg["cols_per_thread"], cfg["strip"], cfg["segment"], cfg["steps_per_launch"]) == (2, 32, 64, 2)
+    assert (cfg["cols_per_thread"], cfg["strip"], cfg["segment"], cfg["steps_per_launch"]) == (2, 64, 64, 4)
     big.close()
 
 
@@ -242,7 +258,7 @@ def test_fp32_state_error_bound(mesh64x256):
 
 def test_full_config2_shape_properties():
     """1024 x 4096, M = 32 (config 2's full single-GPU shape), 10 steps:
-    fused == split == two steps per launch bit for bit, and doubling the forcing doubles the state
+    fused == split == 2, 3 and 4 steps per launch bit for bit, and doubling the forcing doubles the state
     exactly (linearity; scaling by 2 is exact in binary floating point)."""
     say("[gpu] config-2 shape properties")
     app = shaw_amd.App.structured(1024, 4096, LAYERED)
@@ -254,7 +270,8 @@ def test_full_config2_shape_properties():
     tab[:] = 1.0 + np.arange(M)  # large constant forcing so the wave reaches many points in 10 steps
     out = {}
     for name, kern, scale, spl in [("fused", shaw_amd.KERNEL_FUSED, 1.0, 1), ("split", shaw_amd.KERNEL_SPLIT, 1.0, 1),
-                                   ("fused2", shaw_amd.KERNEL_FUSED, 2.0, 1), ("pairs", shaw_amd.KERNEL_FUSED, 1.0, 2)]:
+                                   ("fused2", shaw_amd.KERNEL_FUSED, 2.0, 1), ("pairs", shaw_amd.KERNEL_FUSED, 1.0, 2),
+                                   ("threes", shaw_amd.KERNEL_FUSED, 1.0, 3), ("fours", shaw_amd.KERNEL_FUSED, 1.0, 4)]:
         f = shaw_amd.Fom(app, M, dt, kernel=kern, rank1_forcing=False, steps_per_launch=spl)
         f.set_sources(gids)
         f.set_forcing(tab * scale)
@@ -264,8 +281,9 @@ def test_full_config2_shape_properties():
         f.close()
     for a, b in zip(out["fused"], out["split"]):
         np.testing.assert_array_equal(a, b)
-    for a, b in zip(out["fused"], out["pairs"]):  # two steps per launch at the full shape
-        np.testing.assert_array_equal(a, b)
+    for key in ("pairs", "threes", "fours"):  # several steps per launch at the full shape
+        for a, b in zip(out["fused"], out[key]):
+            np.testing.assert_array_equal(a, b)
     for a, b in zip(out["fused"], out["fused2"]):
         np.testing.assert_array_equal(2.0 * a, b)
     vp = out["fused"][0]
