@@ -239,6 +239,23 @@ def test_default_segment_fills_the_chip():
     big.close()
 
 
+@pytest.mark.parametrize("kw", [dict(steps_per_launch=2), dict(steps_per_launch=3, col_block=32),
+                                dict(steps_per_launch=4), dict(steps_per_launch=4, cols_per_thread=2, strip=32)])
+def test_fp32_packs_equal_single_steps(mesh64x256, kw):
+    """The FP32 state variant: k steps per launch == k single steps bit for bit
+    (the same single-precision expressions in the same order)."""
+    say(f"[gpu] fp32 packs {kw}")
+    rng = np.random.default_rng(9)
+    app = shaw_amd.App.from_mesh(mesh64x256, LAYERED)
+    src = random_sources(rng, 20)
+    rec = list(np.arange(0, 180, 5.0))
+    a = shaw_amd.fom_run(app, 0.5, 150, src, True, rec, 3, precision=shaw_amd.FP32, **kw)
+    b = shaw_amd.fom_run(app, 0.5, 150, src, True, rec, 3, precision=shaw_amd.FP32, steps_per_launch=1)
+    for key in b:
+        np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+    assert np.abs(a["seismogram"]).max() > 0
+
+
 def test_fp32_state_error_bound(mesh64x256):
     """FP32 state/arithmetic variant: its seismogram error against FP64 is
     reported; the stencil differences neighbouring values, so single precision
