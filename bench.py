@@ -417,6 +417,7 @@ def main() -> int:
     log(f"{total_ms:.1f} ms for {args.steps} steps -> {ms_per_step:.4f} ms/step, {value:.0f} forcing-steps/s, "
         f"{achieved:.0f} GB/s algorithmic ({kernel_bytes / 1e9:.3f} GB/step)")
 
+    traffic = measured_traffic(fom.config())
     out = {
         "metric": METRIC,
         "value": value,
@@ -433,11 +434,14 @@ def main() -> int:
         "config": {"workload": w.describe() + ", 180 receivers every 10 steps",
                    "kernel": cfg, "fom_variant": "plain FP64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(fom.config()),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": (fused_kernel_name(cfg) if args.kernel == "fused" else "k_velocity+k_stress"),
                      "steps_per_launch": spl,
                      "bytes_per_launch": kernel_bytes * spl,
                      "launch_ms": ms_per_step * spl,
+                     # the counter-measured HBM bytes per launch over this run's launch time
+                     "traffic_GB_s": traffic / (ms_per_step * spl / 1e3) / 1e9 if traffic else None,
+                     "traffic_frac": traffic / (ms_per_step * spl / 1e3) / 1e9 / HBM_PEAK_GBS if traffic else None,
                      "reference_model_bytes_per_step": model_bytes},
     }
     fom.close()
