@@ -753,13 +753,15 @@ __global__ void __launch_bounds__(512) k_fusedN(StepArgsN<T, NS> aa) {
       const bool past = r + 1 >= (uint32_t)l;
       const uint32_t rv = r + 1 - l;
       const bool rowV = past && rv >= lo[l] && rv < nr;
-      const bool doV = l == 1 || (active && pl + 1 >= (uint32_t)l && pl + l <= P);
       const uint32_t slotL = (slot + NS - (l - 1)) % NS;
       T cl[8];
       load(cl, ring + (slotL * P + pl) * 8);
-      // Row conditions are block-uniform branches; lane conditions select
-      // (every lane of a wave executes either way, and an LDS read of an
-      // unused lane stays inside its array).
+      // Row conditions are block-uniform branches. Lanes outside a level's
+      // valid range (velocity and srp: lanes l-1 .. P-l; stp: l-1 .. P-l-1
+      // and east of the last grid point) compute too: their values are finite (every
+      // load is clamped into the grid), only lanes outside the next level's
+      // narrower range read them, and nothing of theirs is stored. An LDS
+      // read of such a lane stays inside its array.
       T v[CPT];
 #pragma unroll
       for (int k = 0; k < CPT; ++k) v[k] = T(0);
@@ -780,8 +782,6 @@ __global__ void __launch_bounds__(512) k_fusedN(StepArgsN<T, NS> aa) {
           v[k] = vo + a.dt * sum;
         }
         frc.apply(l - 1, v, rv);
-#pragma unroll
-        for (int k = 0; k < CPT; ++k) v[k] = doV ? v[k] : T(0);
         if (rv >= j0 && rv < j1) {
           if (((recMask >> (l - 1)) & 1u) && (recvRow < 0 || (int64_t)rv == recvRow) && owner)
             record(l - 1, v, rv * nth + i);
@@ -792,25 +792,18 @@ __global__ void __launch_bounds__(512) k_fusedN(StepArgsN<T, NS> aa) {
       // stp needs v_l(i+1, rsr) of the previous input row, from LDS)
       const uint32_t rsr = r - l;
       const bool rowS = r >= (uint32_t)l && rsr >= lo[l] && rsr < nr;
-      const bool doStp = active && pl + 1 >= (uint32_t)l && pl + l + 1 <= P && i + 1 < nth;
       if (l < NS) {
 #pragma unroll
         for (int k = 0; k < CPT; ++k) { srpN[l][k] = S[l][k]; stpN[l][k] = T(0); }
         if (rowS) {
           if (rsr + 1 < nr) {
 #pragma unroll
-            for (int k = 0; k < CPT; ++k) {
-              const T x = S[l - 1][k] + a.dt * (cl[4] * v[k] + cl[5] * vj[l - 1][k]);
-              srpN[l][k] = doV ? x : S[l][k];
-            }
+            for (int k = 0; k < CPT; ++k) srpN[l][k] = S[l - 1][k] + a.dt * (cl[4] * v[k] + cl[5] * vj[l - 1][k]);
           }
           T vE[CPT];  // lane P-1 reads the next level's first entry: unused, in bounds (l < NS)
           load(vE, vRow + ((l - 1) * P + pl + 1) * MC + m0);
 #pragma unroll
-          for (int k = 0; k < CPT; ++k) {
-            const T y = E[l - 1][k] + a.dt * (cl[6] * vj[l - 1][k] + cl[7] * vE[k]);
-            stpN[l][k] = doStp ? y : T(0);
-          }
+          for (int k = 0; k < CPT; ++k) stpN[l][k] = E[l - 1][k] + a.dt * (cl[6] * vj[l - 1][k] + cl[7] * vE[k]);
         }
         store(stpOut + (pl + 1) * MC + m0, stpN[l]);
         __syncthreads();  // stp of level l visible to level l + 1
