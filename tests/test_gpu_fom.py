@@ -133,6 +133,68 @@ def test_random_rank2_bit_exact(mesh64x256, M, kw):
     assert np.abs(a["seismogram"]).max() > 0
 
 
+def random_case(seed):
+    """A valid random FOM configuration (mesh, columns, forcing form, kernel
+    shape, observers), drawn from a seed."""
+    rng = np.random.default_rng(seed)
+    nr, nth = int(rng.integers(5, 41)), int(rng.integers(7, 97))
+    M = 1 if rng.random() < 0.25 else int(rng.integers(2, 71))
+    spl = int(rng.integers(1, 5))
+    col_block = int(rng.choice([0, 4, 8, 16, 32]))
+    mc = 1
+    while mc < M and mc < (col_block or 8):
+        mc *= 2
+    cpt = int(rng.choice([c for c in (1, 2, 4, 8) if mc % c == 0]))
+    tpp = mc // cpt
+    pmax = min(512 // tpp, 128)
+    pmin = max(2 * spl, 2)
+    strip = int(rng.integers(pmin, pmax + 1)) if pmax >= pmin else 0
+
+    def lds(P):  # dynamic LDS of a fused launch (FP64), at most 160 KiB
+        if spl == 1:
+            return (2 * P + 1) * mc * 8
+        if spl == 2:
+            return (4 * P + 2) * mc * 8
+        return (spl * P * 8 + 2 * (P + 1) * mc + spl * P * mc) * 8
+
+    while strip > pmin and lds(strip) > 160 * 1024:
+        strip -= 1
+    kw = dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=spl, col_block=col_block, cols_per_thread=cpt,
+              segment=int(rng.integers(1, 20)), graph_steps=int(rng.choice([-1, 4 * spl])))
+    if strip:
+        kw["strip"] = strip
+    if spl == 1 and rng.random() < 0.5:
+        kw["prefetch_rows"] = 2
+    if rng.random() < 0.15:  # the split kernels take only the column shape
+        kw = dict(kernel=shaw_amd.KERNEL_SPLIT, col_block=col_block, cols_per_thread=cpt)
+    src = [(float(rng.uniform(10, 2800)), float(rng.uniform(0, 180)), float(rng.uniform(20, 70)),
+            float(rng.uniform(0, 30))) for _ in range(M)]
+    rec = [float(x) for x in rng.uniform(0, 180, int(rng.integers(1, 25)))]
+    rank2 = M > 1 or bool(rng.integers(0, 2))
+    if not rank2:
+        kw["rank1_forcing"] = True
+    every = [int(rng.integers(1, 6)) for _ in range(3)]
+    if rng.random() < 0.4:  # no snapshots: whole graph chunks replay
+        every[1] = every[2] = 0
+    return (nr, nth), src, rank2, rec, every, kw
+
+
+@pytest.mark.parametrize("seed", range(48))
+def test_random_configurations_bit_exact(seed):
+    """Seeded random meshes, column counts, forcing forms, steps per launch,
+    column blocks, tile shapes, graph chunks and observer periods: the GPU run
+    equals the oracle bit for bit."""
+    (nr, nth), src, rank2, rec, every, kw = random_case(seed)
+    say(f"[gpu] random case {seed}: {nr}x{nth} M={len(src)} rank2={rank2} every={every} {kw}")
+    app = shaw_amd.App.structured(nr, nth, LAYERED)
+    ora = oracle.App("", LAYERED, structured=(nr, nth, 0.0, 180.0, 3480.0, 6371.0))
+    n, dt = 60, 0.5
+    a = shaw_amd.fom_run(app, dt, n, src, rank2, rec, every[0], every[1], every[2], **kw)
+    b = ora.fom_run(dt, n, src, rank2, rec, every[0], every[1], every[2])
+    for key in b:
+        np.testing.assert_array_equal(a[key], b[key], err_msg=f"seed {seed} {key}")
+
+
 def test_autotune_keeps_state_and_parity(mesh64x256):
     """The tile autotuner times candidate shapes on the live state and must
     leave it unchanged; the run it tunes stays bit-identical to the oracle."""
