@@ -59,6 +59,8 @@ def fused_kernel_name(cfg: dict) -> str:
         return f"k_fused<double,{cpt},{cfg['prefetch_rows']}>"
     if spl == 2:
         return f"k_fused2<double,{cpt}>"
+    if cpt == cfg["col_block"] and cfg["strip"] == 64 and cpt <= 2:
+        return f"k_fusedW<double,{cpt},{spl}>"
     return f"k_fusedN<double,{cpt},{spl}>"
 
 

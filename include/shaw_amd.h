@@ -126,19 +126,24 @@ typedef struct {
   int cols_per_thread;  /* 0 = default; else 1, 2, 4 or 8 (must divide the column block) */
   int strip;            /* 0 = default; fused kernel: vp points per block row (incl. halo) */
   int segment;          /* 0 = default; fused kernel: radial rows per block */
-  int prefetch_rows;    /* 0 = default; fused kernel: rows loaded ahead, 1 or 2 */
-  int steps_per_launch; /* 0 = default (2 unless prefetch_rows = 2); fused kernel: 1..4
+  int prefetch_rows;    /* 0 = default; fused kernel: rows loaded ahead, 1 or 2 (the
+                           multi-step kernels: 1; the one-wavefront-strip kernel at 3
+                           or 4 steps: 1 or 2)                                        */
+  int steps_per_launch; /* 0 = default: 6 on one-wavefront strips (cols_per_thread =
+                           col_block <= 2, strip 64) when a launch fills the GPU, else 4
+                           (loaded two rows ahead on those strips); fused kernel: 1..6
                            leapfrog steps per launch, the intermediate levels kept on
-                           chip (strip >= 2 x steps); a pack with a snapshot due at an
-                           intermediate step falls back to fewer steps per launch     */
+                           chip (strip >= 2 x steps; 5 and 6 only on one-wavefront
+                           strips); a pack with a snapshot due at an intermediate step
+                           falls back to fewer steps per launch                       */
   int graph_steps;      /* 0 = default (64, rounded up to an even number of launches);
                            < 0: no graphs. Fused kernel without snapshots: whole chunks
                            of this many steps replay one captured HIP graph (kernels
                            read their step from a device counter); a multiple of
                            2 x steps_per_launch                                       */
-  int col_block;        /* 0 = default; columns stored together per grid point (power of
-                           two, at most 256 bytes of state): smaller blocks give more,
-                           smaller workgroups                                          */
+  int col_block;        /* 0 = default (2); columns stored together per grid point
+                           (power of two, at most 256 bytes of state): smaller blocks
+                           give more, smaller workgroups                               */
 } shaw_fom_options;
 
 /* app: operators; m: forcing columns; dt: time step. opts may be NULL. */

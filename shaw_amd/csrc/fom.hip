@@ -34,6 +34,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "rom_internal.hpp"
@@ -89,6 +90,32 @@ struct StepArgs {
   int rank1;
   uint32_t P, SEG, nStrips, nSegs, nb;
 };
+
+// Input rows first..last through a row function whose row sets rotate (the
+// row loads the next set while it computes on the current one). The loop
+// runs whole rotations unconditionally, so every set stays in its registers
+// across the back edge (a conditional row there makes the compiler copy the
+// sets and wait for every load in flight); the rows left over follow it.
+template <class I, class RS, class Row>
+__device__ __forceinline__ void rotateRows(RS& A, RS& B, Row&& row, I first, I last) {
+  I r = first;
+  for (; r + 1 <= last; r += 2) {
+    row(A, B, r);
+    row(B, A, r + 1);
+  }
+  if (r <= last) row(A, B, r);
+}
+template <class I, class RS, class Row>
+__device__ __forceinline__ void rotateRows(RS& A, RS& B, RS& C, Row&& row, I first, I last) {
+  I r = first;
+  for (; r + 2 <= last; r += 3) {
+    row(A, C, r);
+    row(B, A, r + 1);
+    row(C, B, r + 2);
+  }
+  if (r <= last) row(A, C, r);
+  if (r + 1 <= last) row(B, A, r + 1);
+}
 
 template <class T>
 __device__ __forceinline__ void resolveStep(StepArgs<T>& a) {
@@ -338,23 +365,16 @@ __global__ void __launch_bounds__(512) k_fused(StepArgs<T> a) {
     }
   };
 
-  // PF + 1 row sets rotate through an unrolled loop (no register copies)
+  // PF + 1 row sets rotate (no register copies)
   if constexpr (PF == 1) {
     RowIn<T, CPT> A, B;
     fetch(A, j0);
-    for (int64_t jj = (int64_t)j0 - 1; jj < (int64_t)j1; jj += 2) {
-      row(A, B, jj);
-      if (jj + 1 < (int64_t)j1) row(B, A, jj + 1);
-    }
+    rotateRows(A, B, row, (int64_t)j0 - 1, (int64_t)j1 - 1);
   } else {
     RowIn<T, CPT> A, B, C;
     fetch(A, j0);
     fetch(B, j0 + 1);
-    for (int64_t jj = (int64_t)j0 - 1; jj < (int64_t)j1; jj += 3) {
-      row(A, C, jj);
-      if (jj + 1 < (int64_t)j1) row(B, A, jj + 1);
-      if (jj + 2 < (int64_t)j1) row(C, B, jj + 2);
-    }
+    rotateRows(A, B, C, row, (int64_t)j0 - 1, (int64_t)j1 - 1);
   }
 }
 
@@ -552,10 +572,7 @@ __global__ void __launch_bounds__(512) k_fused2(StepArgs2<T> aa) {
   const uint64_t rLast = j1 + 1;
   RowIn<T, CPT> A, B;
   fetch(A, rFirst);
-  for (uint64_t r = rFirst; r <= rLast; r += 2) {
-    row(A, B, r);
-    if (r + 1 <= rLast) row(B, A, r + 1);
-  }
+  rotateRows(A, B, row, rFirst, rLast);
 }
 
 // ------------------------------------------------------------ NS fused steps
@@ -586,6 +603,7 @@ template <class T, int CPT, int NS>
 struct ColumnForcingN {
   uint32_t row[CPT];  // the source's grid row if it lies in this lane's grid column, else ~0u
   T inc[NS][CPT];
+  bool any;           // a lane of the wave holds a source (wave-uniform)
   __device__ __forceinline__ void load(const StepArgsN<T, NS>& aa, uint32_t colBase, uint32_t i, bool active) {
     const StepArgs<T>& a = aa.s[0];
 #pragma unroll
@@ -604,8 +622,13 @@ struct ColumnForcingN {
         }
       }
     }
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) hit = hit || row[k] != ~0u;
+    any = __ballot(hit) != 0;
   }
   __device__ __forceinline__ void apply(int l, T (&vn)[CPT], uint32_t r) const {
+    if (!any) return;
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const T added = vn[k] + inc[l][k];
@@ -847,9 +870,284 @@ __global__ void __launch_bounds__(512) k_fusedN(StepArgsN<T, NS> aa) {
   const uint32_t rLast = j1 + NS - 1;
   RowState<T, CPT> A, B;
   fetch(A, rFirst);
-  for (uint32_t r = rFirst; r <= rLast; r += 2) {
-    row(A, B, r);
-    if (r + 1 <= rLast) row(B, A, r + 1);
+  rotateRows(A, B, row, rFirst, rLast);
+}
+
+// ------------------------------------------------------------ wave strips
+// k_fusedW: the NS-step kernel when a strip is one wavefront (one thread per
+// grid point carrying all CPT = MC columns of its column block, 64-point
+// strips). Lane p of a wave is point p of the strip (NS-1 west halo points,
+// Q = 65 - 2 NS owned, NS east halo points); the G waves of a workgroup
+// (G = 1, 2 or 4) are consecutive column blocks of the same strip and
+// segment. Theta neighbours pass between lanes with ds_bpermute, so the stp
+// rows that consecutive levels exchange and the east v of the previous row
+// never touch an LDS array or a barrier. The waves share only the coefficient
+// ring, each loading 4/G of the four 16-byte coefficient pairs of every input
+// row; with at least NS+1 ring slots one barrier per input row orders both
+// the ring writes after the last reads of the slot and the reads after the
+// writes.
+// Every value is the one-step kernel's expression in its order, as in
+// k_fusedN.
+
+__device__ __forceinline__ double laneRead(double x, int addr) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)u);
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+__device__ __forceinline__ float laneRead(float x, int addr) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
+}
+
+// old state of one input row for one wave-strip thread, with the NP
+// coefficient pairs of that row its wave loads
+template <class T, int CPT, int NP>
+struct RowStateW {
+  T N[CPT], E[CPT], vo[CPT], W[CPT], c[NP][2];
+};
+
+// PF: input rows loaded ahead (1 or 2); PF+1 row states rotate. G: waves
+// (column blocks) per workgroup.
+template <class T, int CPT, int NS, int PF, int G>
+__global__ void __launch_bounds__(256) k_fusedW(StepArgsN<T, NS> aa) {
+#pragma unroll
+  for (int l = 0; l < NS; ++l) resolveStep(aa.s[l]);
+  const StepArgs<T>& a = aa.s[0];
+  // RS >= NS+1 coefficient ring slots, a power of two (slot = row & (RS-1))
+  constexpr uint32_t P = 64, RS = NS + 1 <= 4 ? 4 : 8, MC = CPT;
+  constexpr int NP = 4 / G;  // coefficient pairs a wave loads per row
+  extern __shared__ unsigned char smem_raw[];
+  // [RS][4][64] pairs: pair q of point p of input row rho (slot rho mod RS)
+  // is (J_vp row rho)[2q, 2q+1] for q < 2, the srp J_sp pair of stress row
+  // rho-1 for q = 2, its stp pair for q = 3
+  T* ring = reinterpret_cast<T*>(smem_raw);
+
+  const unsigned lb = xcdLogicalBlock(blockIdx.x, gridDim.x);
+  const uint32_t strip = lb % a.nStrips;
+  const uint32_t seg = (lb / a.nStrips) % a.nSegs;
+  const uint32_t grp = lb / (a.nStrips * a.nSegs);
+  // the wave's index, a wave-uniform (scalar) value
+  const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / P)), pl = threadIdx.x % P;
+  const uint32_t cbRaw = grp * G + w;
+  const bool live = cbRaw < a.nb;  // waves past the last column block load and compute, store nothing
+  const uint32_t cb = live ? cbRaw : a.nb - 1;
+  const int addrW = (int)(((pl + P - 1) % P) * 4), addrE = (int)(((pl + 1) % P) * 4);
+
+  const uint32_t nr = a.nr, nth = a.nth, rs = 2 * nth - 1;
+  const uint32_t Q = P - (2 * NS - 1);
+  const int32_t ii = (int32_t)(strip * Q) - (NS - 1) + (int32_t)pl;
+  const bool active = ii >= 0 && ii < (int32_t)nth;
+  const uint32_t i = active ? (uint32_t)ii : 0;
+  const bool owner = live && active && pl >= NS - 1 && pl < NS - 1 + Q;
+  const uint32_t j0 = seg * a.SEG;
+  const uint32_t j1 = j0 + a.SEG < nr ? j0 + a.SEG : nr;
+
+  const uint64_t nVp = (uint64_t)nr * nth, nSp = (uint64_t)(nr - 1) * nth + (uint64_t)nr * (nth - 1);
+  const T* vIn = a.vIn + (uint64_t)cb * nVp * MC;
+  T* vOut = a.vOut + (uint64_t)cb * nVp * MC;
+  const T* sIn = a.sIn + (uint64_t)cb * nSp * MC;
+  T* sOut = a.sOut + (uint64_t)cb * nSp * MC;
+
+  // per-lane offsets, clamped into the grid (as in k_fusedN, m0 = 0)
+  const uint32_t ic = active ? i : (ii < 0 ? 0 : nth - 1);
+  const uint32_t iE = ic < nth - 2 ? ic : nth - 2;
+  const uint32_t iW = pl == 0 ? (ic > 0 ? ic - 1 : 0) : iE;
+  const uint32_t vStride = nth * MC, sStride = rs * MC, topS = (nr - 1) * sStride;
+  const uint32_t oV = ic * MC, oSrp = 2 * ic * MC;
+  const uint32_t oStpE = (2 * iE + 1) * MC, oStpETop = iE * MC;
+  const uint32_t oStpW = (2 * iW + 1) * MC, oStpWTop = iW * MC;
+  // the coefficient pairs q = w + jG this wave loads (J_vp halves q = 0, 1,
+  // the srp J_sp pair q = 2, the stp J_sp pair q = 3): base, row stride and
+  // lane offset chosen once
+  uint32_t cq[NP], cStr[NP], cOff[NP];
+  const T* cBase[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const uint32_t q = w + j * G;
+    cq[j] = q;
+    cBase[j] = q < 2 ? a.cv : a.cs;
+    cStr[j] = q < 2 ? 4 * nth : 2 * rs;
+    cOff[j] = q < 2 ? 4 * ic + 2 * q : (q == 2 ? 4 * ic : 2 * (2 * iE + 1));
+  }
+  const uint32_t cTop = (nr - 1) * 2 * rs + 2 * iE;  // the surface stp row's pair (q = 3)
+
+  auto fetch = [&](RowStateW<T, CPT, NP>& d, uint32_t r) {
+    const uint32_t rc = r < nr ? r : nr - 1;
+    const uint32_t rn = rc < nr - 1 ? rc : nr - 2;
+    const bool top = rc + 1 >= nr;
+    load(d.N, sIn + (rn * sStride + oSrp));
+    load(d.E, sIn + (top ? topS + oStpETop : rc * sStride + oStpE));
+    load(d.vo, vIn + (rc * vStride + oV));
+    load(d.W, sIn + (top ? topS + oStpWTop : rc * sStride + oStpW));
+    const uint32_t rb = r == 0 ? 0 : (r - 1 < nr ? r - 1 : nr - 1);
+    const uint32_t rbn = rb < nr - 1 ? rb : nr - 2;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const uint32_t rowC = cq[j] < 2 ? rc : (cq[j] == 2 ? rbn : rb);
+      const bool topC = cq[j] == 3 && rb + 1 >= nr;
+      load(d.c[j], cBase[j] + (topC ? cTop : rowC * cStr[j] + cOff[j]));
+    }
+  };
+
+  ColumnForcingN<T, CPT, NS> frc;
+  frc.load(aa, cb * MC, i, active);
+  T S[NS][CPT], E[NS][CPT], vj[NS][CPT];
+#pragma unroll
+  for (int l = 0; l < NS; ++l)
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) S[l][k] = E[l][k] = vj[l][k] = T(0);
+  uint32_t lo[NS + 1];
+#pragma unroll
+  for (int l = 0; l <= NS; ++l) lo[l] = j0 >= (uint32_t)(NS - l) ? j0 - (NS - l) : 0;
+  const uint32_t rFirst = lo[1];
+  {
+    const uint32_t rb = rFirst > 0 ? (rFirst - 1 < nr - 1 ? rFirst - 1 : nr - 2) : 0;
+    load(S[0], sIn + (rb * sStride + oSrp));
+  }
+  const int64_t recvRow = a.recvRow;
+  const uint32_t colBase = cb * MC;
+  const int32_t* recvTab = nullptr;
+  uint32_t recMask = 0, sample[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    if (aa.s[l].recv) { recvTab = aa.s[l].recv; recMask |= 1u << l; }
+    sample[l] = (uint32_t)aa.s[l].sample;
+  }
+  double* const seis = a.seis;
+  const uint32_t R = (uint32_t)a.R, Tcap = (uint32_t)a.Tcap, M = a.M;
+  auto record = [&](int l, const T (&vn)[CPT], uint32_t p) {
+    const int32_t q = recvTab[p];
+    if (q < 0) return;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const uint32_t col = colBase + k;
+      if (col < M) seis[(uint64_t)q + (uint64_t)R * (sample[l] + (uint64_t)Tcap * col)] = (double)vn[k];
+    }
+  };
+
+  // ST: a steady row (j0 + NS <= r < j1): every level's velocity and stress
+  // rows are valid, inside the segment and below the surface, so none of the
+  // row conditions is evaluated
+  auto row = [&](auto steady, const RowStateW<T, CPT, NP>& cur, RowStateW<T, CPT, NP>& nxt, uint32_t r) {
+    constexpr bool ST = decltype(steady)::value;
+    const uint32_t slot = r & (RS - 1);
+#pragma unroll
+    for (int j = 0; j < NP; ++j) store(ring + ((slot * 4 + cq[j]) * P + pl) * 2, cur.c[j]);
+    fetch(nxt, r + PF);
+    __syncthreads();  // the coefficient row of r visible; every read of the slot it replaced done
+    T srpN[NS][CPT], stpN[NS][CPT], vNew[NS][CPT];
+#pragma unroll
+    for (int l = 1; l <= NS; ++l) {
+      const bool past = r + 1 >= (uint32_t)l;
+      const uint32_t rv = r + 1 - l;
+      const bool rowV = ST || (past && rv >= lo[l] && rv < nr);
+      const uint32_t slotL = rv & (RS - 1);  // (r + 1 - l) mod RS, also when it wraps
+      T cl[8];
+      {
+        const T* e = ring + (slotL * 4 * P + pl) * 2;
+        T c0[2], c1[2], c2[2], c3[2];
+        load(c0, e);
+        load(c1, e + 2 * P);
+        load(c2, e + 4 * P);
+        load(c3, e + 6 * P);
+        cl[0] = c0[0]; cl[1] = c0[1]; cl[2] = c1[0]; cl[3] = c1[1];
+        cl[4] = c2[0]; cl[5] = c2[1]; cl[6] = c3[0]; cl[7] = c3[1];
+      }
+      // Row conditions are block-uniform, so every lane takes part in the
+      // lane reads. Lanes outside a level's valid range compute finite values
+      // that only lanes outside the next level's range read.
+      T v[CPT];
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) v[k] = T(0);
+      if (rowV) {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+          // west stp of level l-1: the west lane's (level 1: lane 0 loaded its own)
+          const T Ek = l == 1 ? cur.E[k] : stpN[l - 1][k];
+          const T fromW = laneRead(Ek, addrW);
+          const T Wk = (l == 1 && pl == 0) ? cur.W[k] : fromW;
+          const T Nk = l == 1 ? cur.N[k] : srpN[l - 1][k];
+          const T Sk = S[l - 1][k];
+          const T vo = l == 1 ? cur.vo[k] : vj[l >= 2 ? l - 2 : 0][k];
+          T sum = cl[0] * Nk + cl[1] * Sk;
+          sum = sum + cl[2] * Wk;
+          sum = sum + cl[3] * Ek;
+          v[k] = vo + a.dt * sum;
+        }
+        frc.apply(l - 1, v, rv);
+        if (ST || (rv >= j0 && rv < j1)) {
+          if (((recMask >> (l - 1)) & 1u) && (recvRow < 0 || (int64_t)rv == recvRow) && owner)
+            record(l - 1, v, rv * nth + i);
+          if (l == NS && owner) store(vOut + (rv * vStride + oV), v);
+        }
+      }
+      const uint32_t rsr = r - l;
+      const bool rowS = ST || (r >= (uint32_t)l && rsr >= lo[l] && rsr < nr);
+      if (l < NS) {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) { srpN[l][k] = S[l][k]; stpN[l][k] = T(0); }
+        if (rowS) {
+          if (ST || rsr + 1 < nr) {
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) srpN[l][k] = S[l - 1][k] + a.dt * (cl[4] * v[k] + cl[5] * vj[l - 1][k]);
+          }
+#pragma unroll
+          for (int k = 0; k < CPT; ++k) {
+            const T vE = laneRead(vj[l - 1][k], addrE);  // v_l of the east point, previous input row
+            stpN[l][k] = E[l - 1][k] + a.dt * (cl[6] * vj[l - 1][k] + cl[7] * vE);
+          }
+        }
+      } else if (ST || (rowS && rsr >= j0 && rsr < j1)) {
+        T vE[CPT];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) vE[k] = laneRead(vj[l - 1][k], addrE);
+        if (owner) {
+          if (ST || rsr + 1 < nr) {
+            T so[CPT];
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) so[k] = S[l - 1][k] + a.dt * (cl[4] * v[k] + cl[5] * vj[l - 1][k]);
+            store(sOut + (rsr * sStride + oSrp), so);
+          }
+          if (i + 1 < nth) {
+            T so[CPT];
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) so[k] = E[l - 1][k] + a.dt * (cl[6] * vj[l - 1][k] + cl[7] * vE[k]);
+            store(sOut + ((ST || rsr + 1 < nr) ? rsr * sStride + oStpE : topS + oStpETop), so);
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) vNew[l - 1][k] = v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      S[0][k] = cur.N[k];
+      E[0][k] = cur.E[k];
+    }
+#pragma unroll
+    for (int l = 1; l < NS; ++l)
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) { S[l][k] = srpN[l][k]; E[l][k] = stpN[l][k]; }
+#pragma unroll
+    for (int l = 0; l < NS; ++l)
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) vj[l][k] = vNew[l][k];
+  };
+
+  auto anyRow = [&](const RowStateW<T, CPT, NP>& cur, RowStateW<T, CPT, NP>& nxt, uint32_t r) {
+    if (r >= j0 + NS && r < j1) row(std::true_type{}, cur, nxt, r);
+    else row(std::false_type{}, cur, nxt, r);
+  };
+  const uint32_t rLast = j1 + NS - 1;
+  if constexpr (PF == 1) {
+    RowStateW<T, CPT, NP> A, B;
+    fetch(A, rFirst);
+    rotateRows(A, B, anyRow, rFirst, rLast);
+  } else {
+    RowStateW<T, CPT, NP> A, B, C;
+    fetch(A, rFirst);
+    fetch(B, rFirst + 1);
+    rotateRows(A, B, C, anyRow, rFirst, rLast);
   }
 }
 
@@ -953,18 +1251,17 @@ class FomEngine final : public shaw_fom {
     const uint32_t cap = 256 / sizeof(T);
     SHAW_REQUIRE(o.col_block >= 0 && (uint32_t)o.col_block <= cap && (o.col_block & (o.col_block - 1)) == 0,
                  SHAW_ERR_ARG, "col_block must be a power of two of at most 256 bytes of state");
-    // default: 64 bytes of state per grid point (8 FP64 columns): 4 threads
-    // per point at 2 columns per thread, so 64-point strips make 256-thread
-    // workgroups, two per CU, whose barriers interleave
-    const uint32_t want = o.col_block > 0 ? (uint32_t)o.col_block : cap / 4;
+    // default: 2-column blocks, one thread per grid point at 2 columns per
+    // thread, so a 64-point strip is one wavefront (k_fusedW)
+    const uint32_t want = o.col_block > 0 ? (uint32_t)o.col_block : 2;
     MC_ = 1;
     while (MC_ < M && MC_ < want) MC_ *= 2;
     nb_ = (uint32_t)((M + MC_ - 1) / MC_);
     kernel_ = o.kernel == SHAW_KERNEL_SPLIT ? SHAW_KERNEL_SPLIT : SHAW_KERNEL_FUSED;
     // defaults from sweeps of the fused kernels on MI355X: at 1024 x 4096,
-    // M = 32, 8-column blocks, 2 columns per thread, 64-point strips, 64-row
-    // segments and four steps per launch; at 200 x 1000, M = 1, 4 and 16, the
-    // fastest segments give about 1700 wavefronts a launch
+    // M = 32, 2-column blocks, 2 columns per thread, 64-point strips, 128-row
+    // segments and six steps per launch; at 200 x 1000, M = 1, four steps per
+    // launch two rows ahead, with segments of about 1700 wavefronts a launch
     cpt_ = o.cols_per_thread > 0 ? o.cols_per_thread : (int)std::min<uint32_t>(MC_, sizeof(T) == 8 ? 2 : 4);
     SHAW_REQUIRE(cpt_ == 1 || cpt_ == 2 || cpt_ == 4 || cpt_ == 8, SHAW_ERR_ARG, "cols_per_thread must be 1, 2, 4 or 8");
     SHAW_REQUIRE(MC_ % cpt_ == 0, SHAW_ERR_ARG, "cols_per_thread must divide the column block width");
@@ -974,12 +1271,27 @@ class FomEngine final : public shaw_fom {
       SHAW_REQUIRE(P_ >= 2 && P_ * tpp <= 512, SHAW_ERR_ARG, "strip too wide for a 512-thread block");
       pf_ = o.prefetch_rows > 0 ? o.prefetch_rows : 1;
       SHAW_REQUIRE(pf_ == 1 || pf_ == 2, SHAW_ERR_ARG, "prefetch_rows must be 1 or 2");
+      // one-wavefront strips: six steps per launch where a launch of 64-row
+      // segments already fills the chip, else four steps loaded two rows
+      // ahead (small grids, where a launch is latency-bound); other strips:
       // four steps per launch where the strip and the 32-bit addressing allow
-      spl_ = o.steps_per_launch > 0 ? o.steps_per_launch
-                                    : (pf_ != 1 ? 1 : (P_ >= 8 && fits32() ? 4 : (P_ >= 4 ? 2 : 1)));
-      SHAW_REQUIRE(spl_ >= 1 && spl_ <= kMaxSpl, SHAW_ERR_ARG, "steps_per_launch must be 1, 2, 3 or 4");
+      const bool wave = waveStrip(cpt_, P_) && fits32();
+      if (o.steps_per_launch > 0) {
+        spl_ = o.steps_per_launch;
+      } else if (wave && o.prefetch_rows == 0 && launchWaves(cpt_, P_, 64, 6) >= kMinWaves) {
+        spl_ = 6;
+      } else if (wave) {
+        spl_ = 4;
+        if (o.prefetch_rows == 0) pf_ = 2;
+      } else {
+        spl_ = pf_ != 1 ? 1 : (P_ >= 8 && fits32() ? 4 : (P_ >= 4 ? 2 : 1));
+      }
+      SHAW_REQUIRE(spl_ >= 1 && spl_ <= kMaxSpl, SHAW_ERR_ARG, "steps_per_launch must be 1 to 6");
       SHAW_REQUIRE(spl_ == 1 || P_ >= 2u * spl_, SHAW_ERR_ARG,
                    "k steps per launch need a strip of at least 2k points");
+      SHAW_REQUIRE(spl_ <= 4 || (waveStrip(cpt_, P_) && pf_ == 1), SHAW_ERR_ARG,
+                   "5 or 6 steps per launch need one thread per point (cols_per_thread = col_block <= 2), "
+                   "64-point strips and prefetch_rows 1");
       graphStepsOpt_ = o.graph_steps;
       graphSteps_ = graphStepsFor(spl_);
       SHAW_REQUIRE(graphSteps_ % (2 * spl_) == 0, SHAW_ERR_ARG,
@@ -992,7 +1304,7 @@ class FomEngine final : public shaw_fom {
       if (o.segment > 0) {
         SEG_ = (uint32_t)o.segment;
       } else {
-        SEG_ = spl_ >= 2 ? 64 : 32;
+        SEG_ = spl_ >= 5 ? 128 : (spl_ >= 2 ? 64 : 32);
         while (SEG_ > std::max(2, spl_) && launchWaves(cpt_, P_, SEG_, spl_) < kMinWaves) SEG_ /= 2;
       }
       SHAW_REQUIRE(ldsBytes(cpt_, P_, spl_) <= kMaxLds, SHAW_ERR_ARG, "tile needs more LDS than a workgroup has");
@@ -1131,7 +1443,9 @@ class FomEngine final : public shaw_fom {
         case 1: launchStep(); break;
         case 2: launchPair(); break;
         case 3: launchPackN<3>(); break;
-        default: launchPackN<4>(); break;
+        case 4: launchPackN<4>(); break;
+        case 5: launchPackN<5>(); break;
+        default: launchPackN<6>(); break;
       }
       k += ns;
     }
@@ -1161,14 +1475,13 @@ class FomEngine final : public shaw_fom {
         aa.s1 = deviceStepArgs(k + 1);
         aa.s2 = deviceStepArgs(k + 2);
         dispatchPair(aa, false);
-      } else if (spl_ == 3) {
-        StepArgsN<T, 3> aa;
-        for (int l = 0; l < 3; ++l) aa.s[l] = deviceStepArgs(k + 1 + l);
-        dispatchN<3>(aa, false);
-      } else if (spl_ == 4) {
-        StepArgsN<T, 4> aa;
-        for (int l = 0; l < 4; ++l) aa.s[l] = deviceStepArgs(k + 1 + l);
-        dispatchN<4>(aa, false);
+      } else if (spl_ >= 3) {
+        switch (spl_) {
+          case 3: captureN<3>(k); break;
+          case 4: captureN<4>(k); break;
+          case 5: captureN<5>(k); break;
+          default: captureN<6>(k); break;
+        }
       } else {
         StepArgs<T> a = deviceStepArgs(k + 1);
         dispatchStep(a, false);
@@ -1400,10 +1713,12 @@ class FomEngine final : public shaw_fom {
       const uint32_t tpp = MC_ / cpt;
       for (uint32_t P : {8u, 16u, 32u, 64u, 128u, 256u}) {
         if (P * tpp > 512 || P * tpp < 64) continue;
-        for (uint32_t SEG : {2u, 4u, 8u, 16u, 32u, 64u})
+        for (uint32_t SEG : {2u, 4u, 8u, 16u, 32u, 64u, 128u})
           for (int pf : {1, 2})
             for (int spl = 1; spl <= kMaxSpl; ++spl) {
-              if (spl >= 2 && pf == 2) continue;  // the multi-step kernels load one row ahead
+              // the multi-step kernels load one row ahead, k_fusedW one or two
+              if (spl >= 2 && pf == 2 && !(spl >= 3 && spl <= 4 && waveStrip(cpt, P))) continue;
+              if (spl >= 5 && !waveStrip(cpt, P)) continue;  // 5 and 6 steps: one-wavefront strips only
               if (spl >= 2 && P < 2u * spl) continue;
               if (step_ + spl > ftabSteps_) continue;  // a trial reads the forcing of its steps
               if (graphStepsFor(spl) % (2 * spl)) continue;  // a requested graph length must fit
@@ -1450,14 +1765,20 @@ class FomEngine final : public shaw_fom {
   }
   // dynamic LDS of one fused launch
   size_t ldsBytes(int cpt, uint32_t P, int spl) const {
-    (void)cpt;
     if (spl == 1) return (size_t)(2 * P + 1) * MC_ * sizeof(T);
     if (spl == 2) return (size_t)(4 * P + 2) * MC_ * sizeof(T);
-    return ldsBytesN(P, spl);
+    return waveStrip(cpt, P) ? ldsBytesW(spl) : ldsBytesN(P, spl);
   }
   size_t ldsBytesN(uint32_t P, int ns) const {
     return ((size_t)ns * P * 8 + 2 * (size_t)(P + 1) * MC_ + (size_t)ns * P * MC_) * sizeof(T);
   }
+  // k_fusedW: a ring of 4 (3 steps) or 8 (4 to 6 steps) coefficient rows of a 64-point strip
+  size_t ldsBytesW(int ns) const { return (size_t)(ns + 1 <= 4 ? 4 : 8) * 64 * 8 * sizeof(T); }
+  // 3 or 4 steps per launch run k_fusedW when a strip is one wavefront: one
+  // thread per point (all columns of the block) and 64-point strips
+  bool waveStrip(int cpt, uint32_t P) const { return (uint32_t)cpt == MC_ && P == 64 && cpt <= 2; }
+  // column blocks (waves) per k_fusedW workgroup
+  uint32_t waveGroup() const { return nb_ >= 4 ? 4 : (nb_ >= 2 ? 2 : 1); }
   // k_fusedN addresses a column block with 32-bit offsets
   bool fits32() const {
     const uint64_t lim = 1ull << 32;
@@ -1467,6 +1788,8 @@ class FomEngine final : public shaw_fom {
   uint64_t launchWaves(int cpt, uint32_t P, uint32_t SEG, int spl) const {
     const uint32_t halo = spl == 1 ? 1 : 2 * spl - 1;
     const uint64_t strips = (nth_ + (P - halo) - 1) / (P - halo), segs = (nr_ + SEG - 1) / SEG;
+    if (spl >= 3 && waveStrip(cpt, P))  // whole groups of column blocks, one wave each
+      return strips * segs * ((nb_ + waveGroup() - 1) / waveGroup()) * waveGroup();
     const uint64_t wavesPerBlock = ((uint64_t)P * (MC_ / cpt) + 63) / 64;
     return strips * segs * nb_ * wavesPerBlock;
   }
@@ -1474,8 +1797,12 @@ class FomEngine final : public shaw_fom {
   // nothing and does not advance the run
   void trialLaunch() {
     if (spl_ >= 3) {
-      if (spl_ == 3) trialPackN<3>();
-      else trialPackN<4>();
+      switch (spl_) {
+        case 3: trialPackN<3>(); break;
+        case 4: trialPackN<4>(); break;
+        case 5: trialPackN<5>(); break;
+        default: trialPackN<6>(); break;
+      }
     } else if (spl_ == 2) {
       StepArgs2<T> aa;
       pairArgs(aa);
@@ -1557,13 +1884,50 @@ class FomEngine final : public shaw_fom {
       a.sOut = s_[cur_ ^ 1].p;
       a.nStrips = nStripsK_[NS];
     }
-    const unsigned blocks = nStripsK_[NS] * nSegs_ * nb_;
-    hipLaunchKernelGGL((k_fusedN<T, CPT, NS>), dim3(blocks), dim3(P_ * tpp), ldsBytesN(P_, NS), stream_, aa);
+    if constexpr (CPT <= 2) {
+      if (waveStrip(CPT, P_)) {
+        switch (waveGroup()) {
+          case 1: launchWave<CPT, NS, 1>(aa); break;
+          case 2: launchWave<CPT, NS, 2>(aa); break;
+          default: launchWave<CPT, NS, 4>(aa); break;
+        }
+        cur_ ^= 1;
+        if (observe) {
+          snapshot(0, v_[cur_].p);
+          snapshot(1, s_[cur_].p);
+        }
+        return;
+      }
+    }
+    if constexpr (NS <= 4) {
+      const unsigned blocks = nStripsK_[NS] * nSegs_ * nb_;
+      hipLaunchKernelGGL((k_fusedN<T, CPT, NS>), dim3(blocks), dim3(P_ * tpp), ldsBytesN(P_, NS), stream_, aa);
+    } else {
+      (void)tpp;
+      throw Error(SHAW_ERR_STATE, "5 or 6 steps per launch run only on one-wavefront strips");
+    }
     cur_ ^= 1;
     if (observe) {
       snapshot(0, v_[cur_].p);
       snapshot(1, s_[cur_].p);
     }
+  }
+  template <int CPT, int NS, int G>
+  void launchWave(StepArgsN<T, NS>& aa) {
+    const unsigned blocks = nStripsK_[NS] * nSegs_ * ((nb_ + G - 1) / G);
+    if constexpr (NS <= 4) {
+      if (pf_ == 2) {
+        hipLaunchKernelGGL((k_fusedW<T, CPT, NS, 2, G>), dim3(blocks), dim3(G * 64), ldsBytesW(NS), stream_, aa);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_fusedW<T, CPT, NS, 1, G>), dim3(blocks), dim3(G * 64), ldsBytesW(NS), stream_, aa);
+  }
+  template <int NS>
+  void captureN(int k) {
+    StepArgsN<T, NS> aa;
+    for (int l = 0; l < NS; ++l) aa.s[l] = deviceStepArgs(k + 1 + l);
+    dispatchN<NS>(aa, false);
   }
   template <int NS>
   void packArgs(StepArgsN<T, NS>& aa) {
@@ -1644,8 +2008,8 @@ class FomEngine final : public shaw_fom {
   hipGraphExec_t gexec_ = nullptr;
   int graphKey_[7] = {0, 0, 0, 0, 0, -1, 0};
   int graphSteps_ = 64, graphStepsOpt_ = 0;
-  uint32_t nStripsK_[5] = {0, 0, 0, 0, 0};  // strips of the k-step kernels (k = 2..4)
-  static constexpr int kMaxSpl = 4;
+  uint32_t nStripsK_[7] = {0, 0, 0, 0, 0, 0, 0};  // strips of the k-step kernels (k = 2..6)
+  static constexpr int kMaxSpl = 6;
   static constexpr size_t kMaxLds = 160 * 1024;
   static constexpr uint64_t kMinWaves = 1536;  // 6 per CU of the 256
   std::vector<double> rhoInv_;

@@ -33,7 +33,13 @@ KERNELS = [dict(kernel=shaw_amd.KERNEL_SPLIT), dict(kernel=shaw_amd.KERNEL_FUSED
            dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=3),
            dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=3, strip=7, segment=2),
            dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=4),
-           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=4, cols_per_thread=1, strip=8, segment=5)]
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=4, cols_per_thread=1, strip=8, segment=5),
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=4, col_block=2, strip=64),
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=3, col_block=1, strip=64, segment=3),
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=4, col_block=2, strip=64, prefetch_rows=2,
+                segment=7),
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=5, col_block=2, strip=64, segment=6),
+           dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=6, col_block=1, strip=64)]
 
 
 def say(msg):
@@ -62,7 +68,8 @@ def run_both(test, **kw):
 
 
 @pytest.mark.parametrize("kw", KERNELS, ids=["split", "fused", "fused-narrow", "fused-cpt1", "fused-pf2", "fused2",
-                                             "fused2-narrow", "fused3", "fused3-narrow", "fused4", "fused4-min"])
+                                             "fused2-narrow", "fused3", "fused3-narrow", "fused4", "fused4-min",
+                                             "wave4", "wave3-1col", "wave4-pf2", "wave5", "wave6-1col"])
 @pytest.mark.parametrize("test", FOM_TESTS)
 def test_regression_inputs_bit_exact(test, kw):
     say(f"[gpu] {test} {kw}")
@@ -93,13 +100,13 @@ def mesh64x256(tmp_path_factory):
 @pytest.mark.parametrize("M,kw", [(32, dict(kernel=shaw_amd.KERNEL_SPLIT)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=1)),
-                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=8, strip=32, segment=7)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, col_block=8, cols_per_thread=8, strip=32, segment=7)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, cols_per_thread=2, strip=16, segment=64)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, prefetch_rows=2, strip=32, segment=32)),
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED, prefetch_rows=2, cols_per_thread=1,
                                             strip=16, segment=10)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2)),
-                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2, cols_per_thread=4,
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, col_block=8, steps_per_launch=2, cols_per_thread=4,
                                             strip=12, segment=7)),
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=2, cols_per_thread=1,
                                             strip=16, segment=64)),
@@ -107,7 +114,7 @@ def mesh64x256(tmp_path_factory):
                                   (7, dict(kernel=shaw_amd.KERNEL_FUSED)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=3)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=4)),
-                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=4, cols_per_thread=4,
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, col_block=8, steps_per_launch=4, cols_per_thread=4,
                                             strip=16, segment=7)),
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=3, cols_per_thread=1,
                                             strip=16, segment=64)),
@@ -117,6 +124,9 @@ def mesh64x256(tmp_path_factory):
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED, col_block=8, steps_per_launch=4, strip=128)),
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED, col_block=4, steps_per_launch=2)),
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED)),
+                                  (40, dict(kernel=shaw_amd.KERNEL_FUSED, col_block=8)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=5)),
+                                  (40, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=6, col_block=1, segment=9)),
                                   (40, dict(kernel=shaw_amd.KERNEL_SPLIT, cols_per_thread=2))])
 def test_random_rank2_bit_exact(mesh64x256, M, kw):
     say(f"[gpu] random rank-2 M={M} {kw}")
@@ -142,7 +152,7 @@ def random_case(seed):
     spl = int(rng.integers(1, 5))
     col_block = int(rng.choice([0, 4, 8, 16, 32]))
     mc = 1
-    while mc < M and mc < (col_block or 8):
+    while mc < M and mc < (col_block or 2):
         mc *= 2
     cpt = int(rng.choice([c for c in (1, 2, 4, 8) if mc % c == 0]))
     tpp = mc // cpt
@@ -173,6 +183,18 @@ def random_case(seed):
     rank2 = M > 1 or bool(rng.integers(0, 2))
     if not rank2:
         kw["rank1_forcing"] = True
+    if kw["kernel"] == shaw_amd.KERNEL_FUSED and rng.random() < 0.3:  # one wavefront per strip
+        cbw = int(rng.choice([1, 2]))
+        mcw = 1
+        while mcw < M and mcw < cbw:
+            mcw *= 2
+        splw = int(rng.integers(3, 7))
+        kw.update(col_block=cbw, cols_per_thread=mcw, strip=64, steps_per_launch=splw)
+        kw.pop("prefetch_rows", None)
+        if kw["graph_steps"] > 0:
+            kw["graph_steps"] = 4 * splw
+        pfw = int(rng.integers(1, 3))
+        kw["prefetch_rows"] = pfw if splw <= 4 else 1  # 5 and 6 steps load one row ahead
     every = [int(rng.integers(1, 6)) for _ in range(3)]
     if rng.random() < 0.4:  # no snapshots: whole graph chunks replay
         every[1] = every[2] = 0
@@ -215,8 +237,8 @@ def test_autotune_keeps_state_and_parity(mesh64x256):
         np.testing.assert_array_equal(x, y)
     cfg = fom.config()
     assert ms > 0 and cfg["cols_per_thread"] in (1, 2, 4, 8) and cfg["prefetch_rows"] in (1, 2)
-    assert cfg["steps_per_launch"] in (1, 2, 3, 4)
-    assert cfg["segment"] in (2, 4, 8, 16, 32, 64) and cfg["strip"] in (8, 16, 32, 64, 128, 256)
+    assert cfg["steps_per_launch"] in (1, 2, 3, 4, 5, 6)
+    assert cfg["segment"] in (2, 4, 8, 16, 32, 64, 128) and cfg["strip"] in (8, 16, 32, 64, 128, 256)
     fom.close()
     rec = list(np.arange(0, 180, 7.5))
     a = shaw_amd.fom_run(app, dt, n, src, True, rec, 3, 40, 30, autotune=2)
@@ -286,18 +308,21 @@ def test_config1_shape_rank1_bit_exact(tmp_path):
 
 
 def test_default_segment_fills_the_chip():
-    """Default tile shapes: four steps per launch, and segments halved until a
-    launch has at least 1536 wavefronts (6 per CU) but not below four rows,
-    so 200 x 1000 with one column runs 4-row segments while 1024 x 4096 with
-    32 columns (two 16-column blocks) keeps 64-row segments."""
+    """Default tile shapes: one-wavefront strips (2-column blocks, one thread
+    per point); six steps per launch in 128-row segments where a launch fills
+    the chip (1024 x 4096 with 32 columns), else four steps loaded two rows
+    ahead with segments halved until a launch has at least 1536 wavefronts
+    (6 per CU) but not below four rows (200 x 1000 with one column)."""
     say("[gpu] default segments")
     small = shaw_amd.Fom(shaw_amd.App.structured(200, 1000, LAYERED), 1, 0.25)
     cfg = small.config()
-    assert (cfg["strip"], cfg["segment"], cfg["steps_per_launch"]) == (64, 4, 4)
+    assert (cfg["strip"], cfg["segment"], cfg["steps_per_launch"], cfg["prefetch_rows"], cfg["col_block"]) == \
+        (64, 4, 4, 2, 1)
     small.close()
     big = shaw_amd.Fom(shaw_amd.App.structured(1024, 4096, LAYERED), 32, 0.05)
     cfg = big.config()
-    assert (cfg["cols_per_thread"], cfg["strip"], cfg["segment"], cfg["steps_per_launch"]) == (2, 64, 64, 4)
+    assert (cfg["cols_per_thread"], cfg["strip"], cfg["segment"], cfg["steps_per_launch"], cfg["col_block"]) == \
+        (2, 64, 128, 6, 2)
     big.close()
 
 
