@@ -899,6 +899,55 @@ __device__ __forceinline__ float laneRead(float x, int addr) {
   return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
 }
 
+// ColumnForcing for k_fusedW: the increments of the NS steps sit in a small
+// LDS table of the wave ([NS][CPT], written once by the wave's first CPT
+// lanes and read only by the same wave, whose LDS operations are ordered),
+// so they hold no registers; a wave reads them only when one of its lanes
+// holds a source (a wave-uniform test). Each increment is the one-step
+// kernel's expression.
+template <class T, int CPT, int NS>
+struct ColumnForcingW {
+  uint32_t row[CPT];  // the source's grid row if it lies in this lane's grid column, else ~0u
+  bool any;           // a lane of the wave holds a source (wave-uniform)
+  const T* inc;       // the wave's table
+  __device__ __forceinline__ void load(const StepArgsN<T, NS>& aa, uint32_t colBase, uint32_t i, bool active,
+                                       uint32_t lane, T* table) {
+    const StepArgs<T>& a = aa.s[0];
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const uint32_t col = colBase + k;
+      const int64_t g = col < a.M ? a.src[col] : -1;
+      row[k] = ~0u;
+      if (g >= 0 && active && (uint32_t)(g % a.nth) == i) row[k] = (uint32_t)(g / a.nth);
+      hit = hit || row[k] != ~0u;
+    }
+    any = __ballot(hit) != 0;
+    if (lane < CPT) {
+      const uint32_t col = colBase + lane;
+      const int64_t g = col < a.M ? a.src[col] : -1;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        T v = T(0);
+        if (g >= 0) {
+          const T f = (T)aa.s[l].frow[col];
+          v = a.rank1 ? a.dt * a.srcRhoInv[col] * f : a.srcRhoInv[col] * f * a.dt;
+        }
+        table[l * CPT + lane] = v;
+      }
+    }
+    inc = table;
+  }
+  __device__ __forceinline__ void apply(int l, T (&vn)[CPT], uint32_t r) const {
+    if (!any) return;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const T added = vn[k] + inc[l * CPT + k];
+      vn[k] = row[k] == r ? added : vn[k];
+    }
+  }
+};
+
 // old state of one input row for one wave-strip thread, with the NP
 // coefficient pairs of that row its wave loads
 template <class T, int CPT, int NP>
@@ -989,8 +1038,8 @@ __global__ void __launch_bounds__(256) k_fusedW(StepArgsN<T, NS> aa) {
     }
   };
 
-  ColumnForcingN<T, CPT, NS> frc;
-  frc.load(aa, cb * MC, i, active);
+  ColumnForcingW<T, CPT, NS> frc;  // increment table after the ring
+  frc.load(aa, cb * MC, i, active, pl, ring + RS * 4 * P * 2 + w * NS * CPT);
   T S[NS][CPT], E[NS][CPT], vj[NS][CPT];
 #pragma unroll
   for (int l = 0; l < NS; ++l)
@@ -1773,8 +1822,9 @@ class FomEngine final : public shaw_fom {
     return ((size_t)ns * P * 8 + 2 * (size_t)(P + 1) * MC_ + (size_t)ns * P * MC_) * sizeof(T);
   }
   // k_fusedW: a ring of 4 (3 steps) or 8 (4 to 6 steps) coefficient rows of a 64-point strip
-  size_t ldsBytesW(int ns) const { return (size_t)(ns + 1 <= 4 ? 4 : 8) * 64 * 8 * sizeof(T); }
-  // 3 or 4 steps per launch run k_fusedW when a strip is one wavefront: one
+  // and the forcing increments ([4 waves][steps][2 columns] at most)
+  size_t ldsBytesW(int ns) const { return ((size_t)(ns + 1 <= 4 ? 4 : 8) * 64 * 8 + 4 * ns * 2) * sizeof(T); }
+  // 3 to 6 steps per launch run k_fusedW when a strip is one wavefront: one
   // thread per point (all columns of the block) and 64-point strips
   bool waveStrip(int cpt, uint32_t P) const { return (uint32_t)cpt == MC_ && P == 64 && cpt <= 2; }
   // column blocks (waves) per k_fusedW workgroup

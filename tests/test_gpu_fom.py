@@ -127,6 +127,7 @@ def mesh64x256(tmp_path_factory):
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED, col_block=8)),
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=5)),
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=6, col_block=1, segment=9)),
+                                  (40, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=6, segment=30)),
                                   (40, dict(kernel=shaw_amd.KERNEL_SPLIT, cols_per_thread=2))])
 def test_random_rank2_bit_exact(mesh64x256, M, kw):
     say(f"[gpu] random rank-2 M={M} {kw}")
