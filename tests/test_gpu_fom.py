@@ -363,7 +363,8 @@ def test_fp32_state_error_bound(mesh64x256):
 
 def test_full_config2_shape_properties():
     """1024 x 4096, M = 32 (config 2's full single-GPU shape), 10 steps:
-    fused == split == 2, 3 and 4 steps per launch bit for bit, and doubling the forcing doubles the state
+    fused == split == 2, 3, 4 and 6 steps per launch (and 4 with 8-column blocks) bit for bit, and doubling the
+    forcing doubles the state
     exactly (linearity; scaling by 2 is exact in binary floating point)."""
     say("[gpu] config-2 shape properties")
     app = shaw_amd.App.structured(1024, 4096, LAYERED)
@@ -374,10 +375,15 @@ def test_full_config2_shape_properties():
     tab = shaw_amd.forcing_table(["ricker"] * M, [s[3] for s in src], [s[2] for s in src], dt, n)
     tab[:] = 1.0 + np.arange(M)  # large constant forcing so the wave reaches many points in 10 steps
     out = {}
-    for name, kern, scale, spl in [("fused", shaw_amd.KERNEL_FUSED, 1.0, 1), ("split", shaw_amd.KERNEL_SPLIT, 1.0, 1),
-                                   ("fused2", shaw_amd.KERNEL_FUSED, 2.0, 1), ("pairs", shaw_amd.KERNEL_FUSED, 1.0, 2),
-                                   ("threes", shaw_amd.KERNEL_FUSED, 1.0, 3), ("fours", shaw_amd.KERNEL_FUSED, 1.0, 4)]:
-        f = shaw_amd.Fom(app, M, dt, kernel=kern, rank1_forcing=False, steps_per_launch=spl)
+    for name, kern, scale, spl, cb in [("fused", shaw_amd.KERNEL_FUSED, 1.0, 1, 0),
+                                       ("split", shaw_amd.KERNEL_SPLIT, 1.0, 1, 0),
+                                       ("fused2", shaw_amd.KERNEL_FUSED, 2.0, 1, 0),
+                                       ("pairs", shaw_amd.KERNEL_FUSED, 1.0, 2, 0),
+                                       ("threes", shaw_amd.KERNEL_FUSED, 1.0, 3, 0),
+                                       ("fours", shaw_amd.KERNEL_FUSED, 1.0, 4, 0),
+                                       ("fours8", shaw_amd.KERNEL_FUSED, 1.0, 4, 8),
+                                       ("sixes", shaw_amd.KERNEL_FUSED, 1.0, 6, 0)]:
+        f = shaw_amd.Fom(app, M, dt, kernel=kern, rank1_forcing=False, steps_per_launch=spl, col_block=cb)
         f.set_sources(gids)
         f.set_forcing(tab * scale)
         f.set_receivers([app.map_receiver(a) for a in range(180)], 1, n)
@@ -386,7 +392,7 @@ def test_full_config2_shape_properties():
         f.close()
     for a, b in zip(out["fused"], out["split"]):
         np.testing.assert_array_equal(a, b)
-    for key in ("pairs", "threes", "fours"):  # several steps per launch at the full shape
+    for key in ("pairs", "threes", "fours", "fours8", "sixes"):  # several steps per launch at the full shape
         for a, b in zip(out["fused"], out[key]):
             np.testing.assert_array_equal(a, b)
     for a, b in zip(out["fused"], out["fused2"]):
