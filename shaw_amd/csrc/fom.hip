@@ -1353,7 +1353,8 @@ class FomEngine final : public shaw_fom {
       if (o.segment > 0) {
         SEG_ = (uint32_t)o.segment;
       } else {
-        SEG_ = spl_ >= 5 ? 128 : (spl_ >= 2 ? 64 : 32);
+        // (six steps: 128 rows for FP64, 64 for FP32, the faster in sweeps at 1024 x 4096)
+        SEG_ = spl_ >= 5 ? (sizeof(T) == 8 ? 128 : 64) : (spl_ >= 2 ? 64 : 32);
         while (SEG_ > std::max(2, spl_) && launchWaves(cpt_, P_, SEG_, spl_) < kMinWaves) SEG_ /= 2;
       }
       SHAW_REQUIRE(ldsBytes(cpt_, P_, spl_) <= kMaxLds, SHAW_ERR_ARG, "tile needs more LDS than a workgroup has");
