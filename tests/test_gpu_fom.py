@@ -328,7 +328,8 @@ def test_default_segment_fills_the_chip():
 
 
 @pytest.mark.parametrize("kw", [dict(steps_per_launch=2), dict(steps_per_launch=3, col_block=32),
-                                dict(steps_per_launch=4), dict(steps_per_launch=4, cols_per_thread=2, strip=32)])
+                                dict(steps_per_launch=4), dict(steps_per_launch=4, cols_per_thread=2, strip=32),
+                                dict(steps_per_launch=6), dict(steps_per_launch=5, col_block=1, segment=9)])
 def test_fp32_packs_equal_single_steps(mesh64x256, kw):
     """The FP32 state variant: k steps per launch == k single steps bit for bit
     (the same single-precision expressions in the same order)."""
