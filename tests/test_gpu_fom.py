@@ -248,12 +248,13 @@ def test_autotune_keeps_state_and_parity(mesh64x256):
         np.testing.assert_array_equal(a[key], b[key], err_msg=key)
 
 
-@pytest.mark.parametrize("spl,gsteps", [(1, 8), (2, 8), (3, 12), (4, 8)])
+@pytest.mark.parametrize("spl,gsteps", [(1, 8), (2, 8), (3, 12), (4, 8), (5, 10), (6, 12), (6, 0)])
 def test_graph_replays_equal_eager(mesh64x256, spl, gsteps):
-    """Chunks of 8 steps replay one captured HIP graph whose kernels read their
-    step from a device counter. Checked against eager launches after every
-    call: several replays, eager remainders, host state reads and a reset in
-    between, and a forcing table replaced (which drops the graph)."""
+    """Chunks of gsteps steps (0: the default length) replay one captured HIP
+    graph whose kernels read their step from a device counter. Checked against
+    eager launches after every call: several replays, eager remainders, host
+    state reads and a reset in between, and a forcing table replaced (which
+    drops the graph)."""
     say(f"[gpu] graph replays spl={spl}")
     rng = np.random.default_rng(5)
     app = shaw_amd.App.from_mesh(mesh64x256, LAYERED)
@@ -270,7 +271,7 @@ def test_graph_replays_equal_eager(mesh64x256, spl, gsteps):
         return f
 
     g, e = make(gsteps), make(-1)
-    for chunk in (30, 17, 8, 41):
+    for chunk in (80 if gsteps == 0 else 30, 17, 8, 41):  # the default chunk is 64 steps or more
         g.advance(chunk)
         e.advance(chunk)
         for x, y in zip(g.state(), e.state()):
