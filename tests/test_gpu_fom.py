@@ -218,6 +218,24 @@ def test_random_configurations_bit_exact(seed):
         np.testing.assert_array_equal(a[key], b[key], err_msg=f"seed {seed} {key}")
 
 
+@pytest.mark.parametrize("seed", range(100, 124))
+def test_random_configurations_fp32_equal_single_steps(seed):
+    """The FP32 state variant on seeded random configurations: every kernel
+    and shape equals FP32 single steps of the fused kernel bit for bit (the
+    same single-precision expressions in the same order)."""
+    (nr, nth), src, rank2, rec, every, kw = random_case(seed)
+    say(f"[gpu] fp32 random case {seed}: {nr}x{nth} M={len(src)} rank2={rank2} every={every} {kw}")
+    app = shaw_amd.App.structured(nr, nth, LAYERED)
+    n, dt = 60, 0.5
+    one = dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=1, col_block=kw.get("col_block", 0))
+    if "rank1_forcing" in kw:
+        one["rank1_forcing"] = kw["rank1_forcing"]
+    a = shaw_amd.fom_run(app, dt, n, src, rank2, rec, every[0], every[1], every[2], precision=shaw_amd.FP32, **kw)
+    b = shaw_amd.fom_run(app, dt, n, src, rank2, rec, every[0], every[1], every[2], precision=shaw_amd.FP32, **one)
+    for key in b:
+        np.testing.assert_array_equal(a[key], b[key], err_msg=f"seed {seed} {key}")
+
+
 def test_autotune_keeps_state_and_parity(mesh64x256):
     """The tile autotuner times candidate shapes on the live state and must
     leave it unchanged; the run it tunes stays bit-identical to the oracle."""
