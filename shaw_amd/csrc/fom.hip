@@ -2083,9 +2083,8 @@ int shaw_fom_create(shaw_app_t app, uint64_t m, double dt, const shaw_fom_option
   return guarded([&] {
     SHAW_REQUIRE(app && out, SHAW_ERR_ARG, "null argument");
     SHAW_REQUIRE(dt > 0, SHAW_ERR_ARG, "dt must be > 0");
-    int ndev = 0;
-    SHAW_HIP(hipGetDeviceCount(&ndev));
-    SHAW_REQUIRE(ndev > 0, SHAW_ERR_HIP, "no HIP device available");
+    // the options are validated before the engine's first HIP call, so an
+    // invalid shape is refused with its reason also where no device is present
     shaw_fom_options o{};
     if (opts) o = *opts;
     SHAW_REQUIRE(o.precision == SHAW_FP64 || o.precision == SHAW_FP32, SHAW_ERR_ARG, "unknown precision");
