@@ -1763,7 +1763,7 @@ class FomEngine final : public shaw_fom {
       const uint32_t tpp = MC_ / cpt;
       for (uint32_t P : {8u, 16u, 32u, 64u, 128u, 256u}) {
         if (P * tpp > 512 || P * tpp < 64) continue;
-        for (uint32_t SEG : {2u, 4u, 8u, 16u, 32u, 64u, 128u})
+        for (uint32_t SEG : {2u, 4u, 8u, 16u, 32u, 64u, 128u, 256u})
           for (int pf : {1, 2})
             for (int spl = 1; spl <= kMaxSpl; ++spl) {
               // the multi-step kernels load one row ahead, k_fusedW one or two

@@ -257,7 +257,7 @@ def test_autotune_keeps_state_and_parity(mesh64x256):
     cfg = fom.config()
     assert ms > 0 and cfg["cols_per_thread"] in (1, 2, 4, 8) and cfg["prefetch_rows"] in (1, 2)
     assert cfg["steps_per_launch"] in (1, 2, 3, 4, 5, 6)
-    assert cfg["segment"] in (2, 4, 8, 16, 32, 64, 128) and cfg["strip"] in (8, 16, 32, 64, 128, 256)
+    assert cfg["segment"] in (2, 4, 8, 16, 32, 64, 128, 256) and cfg["strip"] in (8, 16, 32, 64, 128, 256)
     fom.close()
     rec = list(np.arange(0, 180, 7.5))
     a = shaw_amd.fom_run(app, dt, n, src, True, rec, 3, 40, 30, autotune=2)
