@@ -424,7 +424,7 @@ def test_full_config2_shape_properties():
 def test_full_config2_shape_vs_oracle():
     """1024 x 4096 with 32 seeded sources (config 2's mesh, material, depths
     and periods; delay 0 so the Ricker wavelets are large from the start), 24
-    steps = six launches of the default four-step kernel, 180 receivers every
+    steps = four launches of the default six-step kernel, 180 receivers every
     step: seismograms and final states bit-identical to the oracle's CSR run."""
     say("[gpu] config-2 shape vs oracle")
     from shaw_amd import workloads
@@ -440,6 +440,27 @@ def test_full_config2_shape_vs_oracle():
     for key in ref:
         np.testing.assert_array_equal(got[key], ref[key], err_msg=key)
     assert np.count_nonzero(got["final_vp"]) > 32 * 100
+
+
+def test_config5_shape_vs_oracle():
+    """2048 x 8192 (config 5's grid and material) with 4 of its seeded sources
+    (delay 0), 12 steps = two launches of the default six-step kernel, 180
+    receivers every step: seismograms and final states bit-identical to the
+    oracle's CSR run at the largest grid."""
+    say("[gpu] config-5 shape vs oracle (4 columns)")
+    from shaw_amd import workloads
+    w = workloads.config5()
+    mat = workloads.layered_material()
+    app = shaw_amd.App.structured(w.nr, w.nth, mat)
+    ora = oracle.App("", mat, structured=(w.nr, w.nth, 0.0, 180.0, 3480.0, 6371.0))
+    src = [(d, a, p, 0.0) for d, a, p, dl in w.sources[:4]]
+    rec = [float(x) for x in range(180)]
+    n = 12
+    got = shaw_amd.fom_run(app, w.dt, n, src, True, rec, 1)
+    ref = ora.fom_run(w.dt, n, src, True, rec, 1)
+    for key in ref:
+        np.testing.assert_array_equal(got[key], ref[key], err_msg=key)
+    assert np.count_nonzero(got["final_vp"]) > 4 * 20
 
 
 def test_errors_from_gpu_entry_points(mesh64x256):
