@@ -5,11 +5,15 @@ a seismogram sampling period. Each K has one resident engine (reduced
 operators, receiver rows, column slots with their states, forcing series and
 seismogram buffers in HBM: shaw_rom_*). The loop
 
-  1. moves arrived requests to a FIFO queue,
+  1. moves arrived requests to a queue,
   2. admits queued requests while their K's engine has free slots and the
      resident footprint stays within the HBM budget (requests admitted while
-     others are mid-flight join the running batch: continuous batching;
-     a request that does not fit is not skipped over, so order is FIFO),
+     others are mid-flight join the running batch: continuous batching).
+     Policy "fifo" (the default) admits in arrival order and does not skip a
+     request that does not fit; policy "sjf" admits, among the queued
+     requests that fit, the one with the least work (horizon x columns)
+     first, which lowers the median latency under load at the cost of the
+     largest requests,
   3. advances every engine that has active columns by one chunk of steps --
      each engine on its own HIP stream, all launched before any wait, so the
      engines' work overlaps on the device -- and
@@ -61,10 +65,14 @@ def column_bytes(kv: int, ks: int, horizon: int, every: int, r: int) -> int:
 
 
 class Server:
-    def __init__(self, engines: dict, hbm_budget: float, chunk: int = 500, clock=time.perf_counter):
+    def __init__(self, engines: dict, hbm_budget: float, chunk: int = 500, clock=time.perf_counter,
+                 policy: str = "fifo"):
         """engines: K -> engine with .admit(b, ftab, horizons, every) -> slots,
         .advance(n), .sync(), .retire_many(slots, max_samples) -> [seis],
-        .free_slots(), .kv, .ks, .R."""
+        .free_slots(), .kv, .ks, .R. policy: "fifo" or "sjf"."""
+        if policy not in ("fifo", "sjf"):
+            raise ValueError(f"unknown admission policy {policy!r}")
+        self.policy = policy
         self.engines = engines
         self.budget = hbm_budget
         self.chunk = chunk
@@ -94,14 +102,24 @@ class Server:
             raise ValueError(f"request {req.rid}: footprint exceeds the HBM budget")
         self.queue.append(req)
 
+    def _fits(self, req: Request) -> bool:
+        return (self.engines[req.k].free_slots() >= req.m and
+                self.resident_bytes + self.footprint(req) <= self.budget)
+
     def _admit(self) -> None:
         while self.queue:
-            req = self.queue[0]
+            if self.policy == "fifo":
+                req = self.queue[0]
+                if not self._fits(req):
+                    break  # FIFO: wait for slots / memory to free up
+            else:  # least work first among the requests that fit (ties: arrival order)
+                fit = [r for r in self.queue if self._fits(r)]
+                if not fit:
+                    break
+                req = min(fit, key=lambda r: r.horizon * r.m)
+            self.queue.remove(req)
             e = self.engines[req.k]
             fp = self.footprint(req)
-            if e.free_slots() < req.m or self.resident_bytes + fp > self.budget:
-                break  # FIFO: wait for slots / memory to free up
-            self.queue.popleft()
             req.slots = e.admit(req.b, req.ftab, [req.horizon] * req.m, req.every)
             req.admitted = self.now()
             req.steps_left = req.horizon

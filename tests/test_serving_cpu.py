@@ -119,3 +119,38 @@ def test_poisson_request_generator():
     assert all(1 <= r.m <= 256 and 5000 <= r.horizon <= 20000 and r.horizon % 10 == 0 for r in reqs)
     assert {r.k for r in reqs} == {128, 256, 512}
     assert all(np.diff([r.arrival for r in reqs]) > 0)
+
+
+def test_sjf_admits_least_work_first_among_those_that_fit():
+    """Policy "sjf": the least work (horizon x columns) first, not blocked by
+    a queued request that does not fit; ties in arrival order."""
+    engines = {8: make_engine(8, 6)}
+    reqs = [req(0, 8, 4, 30, 0.0, 1), req(1, 8, 4, 30, 0.0, 2), req(2, 8, 1, 10, 0.0, 3)]
+    srv = serving.Server(engines, hbm_budget=1e12, chunk=10, clock=FakeClock(), policy="sjf")
+    for r in reqs:
+        srv.submit(r)
+    srv._admit()
+    assert [r.rid for r in srv.resident] == [2, 0] and [r.rid for r in srv.queue] == [1]
+    rep = srv.run([])
+    assert rep["requests"] == 3 and not srv.queue and not srv.resident
+
+
+def test_sjf_and_fifo_give_the_same_results():
+    def mix():
+        return [req(i, 8 if i % 3 else 12, 1 + i % 4, 15 + 7 * (i % 5), 0.0, 40 + i) for i in range(9)]
+
+    out = {}
+    for policy in ("fifo", "sjf"):
+        engines = {8: make_engine(8, 6), 12: make_engine(12, 5, seed=3)}
+        rs = mix()
+        rep = serving.Server(engines, hbm_budget=1e12, chunk=6, clock=FakeClock(), policy=policy).run(rs)
+        assert rep["requests"] == 9
+        out[policy] = {r.rid: r.seismograms for r in rs}
+    for rid in out["fifo"]:
+        for x, y in zip(out["fifo"][rid], out["sjf"][rid]):
+            np.testing.assert_array_equal(x, y)
+
+
+def test_unknown_policy_is_refused():
+    with pytest.raises(ValueError, match="admission policy"):
+        serving.Server({8: make_engine(8, 6)}, hbm_budget=1e12, policy="lifo")
