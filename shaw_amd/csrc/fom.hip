@@ -5,8 +5,8 @@
 //
 // HBM layout. States are stored in column blocks of MC forcing columns:
 //   x[b][gid][c]  (b = column / MC, c = column % MC), gids as in the mesh
-// files, so MC*sizeof(T) contiguous bytes per grid point (256 B for the FP64
-// MC = 32 case). Coefficients: coefVp[gid][4] (slots N, S, W, E with merged
+// files, so MC*sizeof(T) contiguous bytes per grid point (16 B for the
+// default FP64 MC = 2, 256 B for MC = 32). Coefficients: coefVp[gid][4] (slots N, S, W, E with merged
 // entries in N / W), coefSp[gid][2]. Neighbours are addressed by the
 // structured index formulas of the mesh (shaw_host.hpp).
 //
@@ -26,6 +26,13 @@
 //    ping-pong buffer. Each state value is read once and written once per
 //    step (vs. 3 passes of x_s and x_v traffic for the split pair); the
 //    halos cost one row per segment and one point per strip row.
+//  * k_fused2 / k_fusedN: two, three or four steps per launch, the
+//    intermediate levels kept in registers and LDS, so the states are read
+//    and written once per launch (recomputed halo rows and points).
+//  * k_fusedW: three to six steps per launch when a 64-point strip is one
+//    wavefront (one thread per point, all columns of a 1- or 2-column block);
+//    neighbours move between lanes, and the waves of a workgroup share only
+//    a coefficient ring. The default: six steps per launch.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
