@@ -1089,7 +1089,12 @@ __global__ void __launch_bounds__(256) k_fusedW(StepArgsN<T, NS> aa) {
     const uint32_t slot = r & (RS - 1);
 #pragma unroll
     for (int j = 0; j < NP; ++j) store(ring + ((slot * 4 + cq[j]) * P + pl) * 2, cur.c[j]);
+    // the next row's loads are issued at raised wave priority, so the
+    // arbiter lets them out ahead of the other wave's arithmetic (3-4 %
+    // faster at 1024 x 4096)
+    __builtin_amdgcn_s_setprio(2);
     fetch(nxt, r + PF);
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();  // the coefficient row of r visible; every read of the slot it replaced done
     T srpN[NS][CPT], stpN[NS][CPT], vNew[NS][CPT];
 #pragma unroll
