@@ -309,7 +309,8 @@ int shaw_reconstruct(const double* phi, uint64_t n, uint64_t k, const double* q,
  *         -> d[64][4]. No reference counterpart.                          */
 int shaw_debug_mfma(int kind, const double* a, const double* b, double* d);
 /* Host symmetric eigen-decomposition used by the POD (Householder
- * tridiagonalization + implicit-shift QL): a row-major [n x n] symmetric,
+ * tridiagonalization + implicit symmetric QR with Wilkinson shifts, Golub &
+ * Van Loan, Matrix Computations, 4th ed., Algorithms 8.3.1-8.3.3): a row-major [n x n] symmetric,
  * w[n] eigenvalues (unordered), v row-major [n x n] with eigenvectors in its
  * columns. CPU only. No reference counterpart.                             */
 int shaw_debug_symeig(int n, const double* a, double* w, double* v);

@@ -192,125 +192,6 @@ __global__ void __launch_bounds__(256) k_gemm_nn_rows(const double* __restrict__
       }
 }
 
-// host: symmetric eigen-decomposition (A row-major n x n, destroyed):
-// Householder reduction to tridiagonal form with the transformations
-// accumulated, then the QL algorithm with implicit Wilkinson shifts
-// (Wilkinson & Reinsch, Handbook for Automatic Computation II, tred2/tql2).
-// On return w holds the eigenvalues (unordered) and column k of V the
-// eigenvector of w[k].
-void symmetricEigen(std::vector<double>& A, int n, std::vector<double>& w, std::vector<double>& V) {
-  auto a = [&](int i, int j) -> double& { return A[(size_t)i * n + j]; };
-  std::vector<double> d(n, 0.0), e(n, 0.0);
-  for (int i = n - 1; i > 0; --i) {
-    const int l = i - 1;
-    double h = 0.0;
-    if (l > 0) {
-      double scale = 0.0;
-      for (int k = 0; k <= l; ++k) scale += std::abs(a(i, k));
-      if (scale == 0.0) {
-        e[i] = a(i, l);
-      } else {
-        for (int k = 0; k <= l; ++k) {
-          a(i, k) /= scale;
-          h += a(i, k) * a(i, k);
-        }
-        double f = a(i, l);
-        double g = f >= 0.0 ? -std::sqrt(h) : std::sqrt(h);
-        e[i] = scale * g;
-        h -= f * g;
-        a(i, l) = f - g;
-        f = 0.0;
-        for (int j = 0; j <= l; ++j) {
-          a(j, i) = a(i, j) / h;
-          g = 0.0;
-          for (int k = 0; k <= j; ++k) g += a(j, k) * a(i, k);
-          for (int k = j + 1; k <= l; ++k) g += a(k, j) * a(i, k);
-          e[j] = g / h;
-          f += e[j] * a(i, j);
-        }
-        const double hh = f / (h + h);
-        for (int j = 0; j <= l; ++j) {
-          f = a(i, j);
-          e[j] = g = e[j] - hh * f;
-          for (int k = 0; k <= j; ++k) a(j, k) -= (f * e[k] + g * a(i, k));
-        }
-      }
-    } else {
-      e[i] = a(i, l);
-    }
-    d[i] = h;
-  }
-  d[0] = 0.0;
-  e[0] = 0.0;
-  for (int i = 0; i < n; ++i) {  // accumulate the transformations
-    const int l = i - 1;
-    if (d[i] != 0.0) {
-      for (int j = 0; j <= l; ++j) {
-        double g = 0.0;
-        for (int k = 0; k <= l; ++k) g += a(i, k) * a(k, j);
-        for (int k = 0; k <= l; ++k) a(k, j) -= g * a(k, i);
-      }
-    }
-    d[i] = a(i, i);
-    a(i, i) = 1.0;
-    for (int j = 0; j <= l; ++j) a(j, i) = a(i, j) = 0.0;
-  }
-  // QL with implicit shifts on the tridiagonal (d, e)
-  for (int i = 1; i < n; ++i) e[i - 1] = e[i];
-  if (n > 0) e[n - 1] = 0.0;
-  // An off-diagonal entry is negligible relative to its neighbours, or
-  // below 1e-3 eps ||T|| (an absolute eigenvalue error the POD never sees:
-  // each pass uses only eigenvalues above 1e-6 of the largest). The second
-  // test ends the iteration on clusters of noise-level eigenvalues.
-  double tnorm = 0.0;
-  for (int i = 0; i < n; ++i) tnorm = std::max(tnorm, std::abs(d[i]) + std::abs(e[i]));
-  const double eps = std::numeric_limits<double>::epsilon(), floorAbs = 1e-3 * eps * tnorm;
-  for (int l = 0; l < n; ++l) {
-    int iter = 0, m;
-    do {
-      for (m = l; m < n - 1; ++m) {
-        const double dd = std::abs(d[m]) + std::abs(d[m + 1]);
-        if (std::abs(e[m]) <= eps * dd || std::abs(e[m]) <= floorAbs) break;
-      }
-      if (m != l) {
-        SHAW_REQUIRE(iter++ < 100, SHAW_ERR_ARG, "eigensolver did not converge");
-        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-        double r = std::hypot(g, 1.0);
-        g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
-        double s = 1.0, c = 1.0, p = 0.0;
-        int i;
-        for (i = m - 1; i >= l; --i) {
-          double f = s * e[i];
-          const double b = c * e[i];
-          e[i + 1] = (r = std::hypot(f, g));
-          if (r == 0.0) {
-            d[i + 1] -= p;
-            e[m] = 0.0;
-            break;
-          }
-          s = f / r;
-          c = g / r;
-          g = d[i + 1] - p;
-          r = (d[i] - g) * s + 2.0 * c * b;
-          d[i + 1] = g + (p = s * r);
-          g = c * r - b;
-          for (int k = 0; k < n; ++k) {
-            f = a(k, i + 1);
-            a(k, i + 1) = s * a(k, i) + c * f;
-            a(k, i) = c * a(k, i) - s * f;
-          }
-        }
-        if (r == 0.0 && i >= l) continue;
-        d[l] -= p;
-        e[l] = g;
-        e[m] = 0.0;
-      }
-    } while (m != l);
-  }
-  w = d;
-  V = A;
-}
-
 // ------------------------------------------------------------ MFMA lane layouts
 // v_mfma_f64_16x16x4f64 (one 16x16 output block, K = 4):
 //   A: lane l holds A[l % 16][l / 16];  B: lane l holds B[l / 16][l % 16]

@@ -171,7 +171,7 @@ def test_library_exports_every_declared_symbol():
 
 @pytest.mark.parametrize("n", [1, 2, 7, 64, 300])
 def test_pod_eigensolver_against_numpy(n):
-    """The host eigensolver of the POD (tridiagonal QL): eigenvalues of a
+    """The host eigensolver of the POD (tridiagonal symmetric QR): eigenvalues of a
     graded Gram matrix to ~eps ||G||, orthonormal eigenvectors, small residual."""
     rng = np.random.default_rng(n)
     X = rng.standard_normal((n + 5, n))
@@ -187,7 +187,7 @@ def test_pod_eigensolver_against_numpy(n):
 
 def test_pod_eigensolver_rank_deficient_gram():
     """A Gram matrix of rank 40 (plus rounding-level noise) of size 600: the
-    cluster of noise eigenvalues must not stall the QL iteration, and the
+    cluster of noise eigenvalues must not stall the QR iteration, and the
     resolved eigenvalues stay accurate to ~eps ||G||."""
     rng = np.random.default_rng(5)
     X = rng.standard_normal((700, 40)) @ rng.standard_normal((40, 600))
@@ -199,3 +199,24 @@ def test_pod_eigensolver_rank_deficient_gram():
     assert np.abs(np.sort(w) - ref).max() <= 1e-12 * scale
     assert np.abs(G @ V - V * w).max() <= 1e-12 * scale
 
+
+
+@pytest.mark.parametrize("n", [3, 50, 257])
+def test_pod_eigensolver_closed_form(n):
+    """The second-difference matrix tridiag(-1, 2, -1), rotated by a random
+    orthogonal matrix so the tridiagonalization has work to do: eigenvalues
+    2 - 2 cos(k pi / (n + 1)), k = 1..n (all distinct), and the eigenvectors
+    rotate back to the discrete sines sin(j k pi / (n + 1)) up to sign."""
+    T = 2.0 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    Qr, _ = np.linalg.qr(np.random.default_rng(n).standard_normal((n, n)))
+    A = Qr @ T @ Qr.T
+    A = 0.5 * (A + A.T)
+    w, V = shaw_amd.debug_symeig(A)
+    k = np.arange(1, n + 1)
+    lam = 2.0 - 2.0 * np.cos(k * np.pi / (n + 1))
+    order = np.argsort(w)
+    assert np.abs(w[order] - lam).max() <= 1e-13 * 4.0
+    sines = np.sin(np.outer(k, k) * np.pi / (n + 1))
+    sines /= np.linalg.norm(sines, axis=0)
+    U = Qr.T @ V[:, order]
+    assert np.abs(np.abs(U) - np.abs(sines)).max() <= 1e-10
