@@ -53,6 +53,8 @@ def _load():
     lib.orc_map_source.argtypes = [C.c_void_p, C.c_double, C.c_double]
     lib.orc_map_receiver.restype = C.c_int64
     lib.orc_map_receiver.argtypes = [C.c_void_p, C.c_double]
+    lib.orc_material_at.restype = None
+    lib.orc_material_at.argtypes = [C.c_int, _dp, C.c_int, C.c_double, C.c_uint64, _dp, _dp, _dp]
     lib.orc_signal.restype = C.c_double
     lib.orc_signal.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
     vp = C.c_void_p
@@ -209,6 +211,18 @@ class App:
         ArS = np.zeros((Ks, Kv))
         _lib.orc_rom_project(self._h, pv, Kv, ps, Ks, ArV, ArS)
         return ArV, ArS
+
+
+def material_at(material: dict, radius_m, r_surf_km: float = 6371.0):
+    """(rho [kg/m^3], vs [m/s]) of a parsed material section at the given
+    radii [m] (Material::computeAt), for a domain whose surface is at
+    r_surf_km."""
+    kind, params = material_params(material)
+    r = np.ascontiguousarray(np.atleast_1d(np.asarray(radius_m, dtype=np.float64)))
+    rho, vs = np.empty_like(r), np.empty_like(r)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    _load().orc_material_at(kind, params, len(params), r_surf_km * 1000.0, len(r), r, rho, vs)
+    return rho, vs
 
 
 def signal(kind: str, delay: float, period: float, t: float) -> float:

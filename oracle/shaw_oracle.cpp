@@ -501,6 +501,17 @@ int64_t orc_map_receiver(void* h, double angleDeg) {
   catch (const std::exception& e) { setErr(e); return -1; }
 }
 
+// Material::computeAt at n radii [m] for a domain whose surface lies at
+// r_surf_m (the depth origin of unilayer and bilayer); rho [kg/m^3], vs [m/s].
+void orc_material_at(int kind, const double* params, int nparams, double rSurfM, uint64_t n, const double* r,
+                     double* rho, double* vs) {
+  Material m;
+  m.kind = kind;
+  m.p.assign(params, params + nparams);
+  m.rSurfM = rSurfM;
+  for (uint64_t i = 0; i < n; ++i) m.computeAt(r[i], rho[i], vs[i]);
+}
+
 double orc_signal(int kind, double delay, double period, double t) {
   return signalAt(kind, delay, period, t);
 }

@@ -14,6 +14,8 @@ and stores them as data:
                             to the exact double)
   unit_golds.json           literal arrays of tests/{jacobian_vp,jacobian_sp,graphs,
                             coords,stress_labels,meshInfo,seismogram,forcing_rank1}
+  prem_table.json           the PREM table various/prem.txt (radius [km], density
+                            [g/cm^3], vsv and vsh [km/s]; rows in file order)
 
 Usage: python tests/golden/make_fixtures.py [/root/reference]
 """
@@ -147,6 +149,22 @@ def main(ref: str) -> None:
                              "gid": int(re.search(r"getVpGid\(\) != (\d+)", fr).group(1))}
     with open(os.path.join(HERE, "unit_golds.json"), "w") as f:
         json.dump(unit, f, indent=1)
+    prem_table(ref)
+
+
+def prem_table(ref: str) -> None:
+    """Columns 1, 3, 6 and 7 of various/prem.txt (radius, density, vsv, vsh)."""
+    rows = []
+    with open(os.path.join(ref, "various", "prem.txt")) as f:
+        for ln in f:
+            x = ln.split()
+            if len(x) >= 7:
+                rows.append([float(x[0]), float(x[2]), float(x[5]), float(x[6])])
+    with open(os.path.join(HERE, "prem_table.json"), "w") as f:
+        f.write('{"source": "various/prem.txt (columns radius km, density g/cm^3, vsv km/s, vsh km/s)",\n')
+        f.write(' "rows": [\n')
+        f.write(",\n".join("  " + json.dumps(r) for r in rows))
+        f.write("\n ]}\n")
 
 
 if __name__ == "__main__":

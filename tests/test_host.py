@@ -13,6 +13,7 @@ import pytest
 
 import oracle
 import shaw_amd
+from shaw_amd import workloads
 from tests.helpers import GOLDEN, MESHES, unit_golds
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -62,6 +63,20 @@ def test_operators_bit_exact_vs_oracle_csr(mesh, mat):
     np.testing.assert_array_equal(ri, ori)
     np.testing.assert_array_equal(g, og)
     assert (a.minVs, a.maxVs) == (o.minVs, o.maxVs)
+
+
+@pytest.mark.parametrize("mat", [workloads.layered_material(), {"kind": "prem"}, BILAYER],
+                         ids=["layered", "prem", "bilayer"])
+def test_material_arrays_bit_exact_vs_oracle_config1_mesh(mat):
+    """rho^-1 at every velocity point and the shear modulus at every stress
+    point of the 200 x 1000 mantle mesh: product == oracle bit for bit (the
+    oracle's materials are pinned by tests/test_oracle_pins.py)."""
+    a = shaw_amd.App.structured(200, 1000, mat)
+    o = oracle.App("", mat, structured=(200, 1000, 0.0, 180.0, 3480.0, 6371.0))
+    _, _, ri, g = a.operators()
+    ori, og = o.material()
+    np.testing.assert_array_equal(ri, ori)
+    np.testing.assert_array_equal(g, og)
 
 
 def test_jacobian_gold_via_product():
