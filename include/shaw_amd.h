@@ -179,6 +179,21 @@ int shaw_fom_set_state(shaw_fom_t h, const double* vp, const double* sp);
 /* Time `nsteps` steps with HIP events on the handle's stream: total ms and
  * the average duration of one leapfrog step (all its kernels).            */
 int shaw_fom_time(shaw_fom_t h, uint64_t nsteps, double* total_ms, double* ms_per_step);
+/* Enqueue nsteps steps in chunks of `chunk` steps with a HIP event between
+ * chunks (no host wait between them), wait, and write each chunk's duration
+ * [ms] to chunk_ms (ceil(nsteps / chunk) values). Steps run fused in
+ * multi-step launches, so a chunk's duration over its steps is the per-step
+ * time whose minimum and maximum runFom's perfTimes keep per step
+ * (src/kokkos/fom_run.hpp:144-148).                                        */
+int shaw_fom_time_chunks(shaw_fom_t h, uint64_t nsteps, uint64_t chunk, double* chunk_ms);
+/* The reference's byte/flop model of one FOM step, as runFom prints it
+ * (complexityFom, src/kokkos/fom_complexities.hpp:57-137, over
+ * src/shared/complexities/{spmv,spmm,mult}.hpp): nnz of J_vp and J_sp, FP64
+ * values and 8-byte CSR ordinals; rank2 = 0: spmv + mult_beta_one + spmv
+ * (one forcing), rank2 = 1: spmm + the per-forcing add + spmm over m
+ * columns. Host arithmetic only.                                           */
+int shaw_fom_complexity(uint64_t nvp, uint64_t nsp, uint64_t nnz_vp, uint64_t nnz_sp, uint64_t m, int rank2,
+                        double* mem_mb, double* flops);
 /* Algorithmic HBM bytes of one leapfrog step of the selected kernel (the
  * state read and written once per launch, the coefficients read once per
  * launch, divided by the steps per launch), and the reference model

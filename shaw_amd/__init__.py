@@ -103,6 +103,8 @@ def lib():
         "shaw_fom_get_state": (i32, [vp, vp, vp]),
         "shaw_fom_set_state": (i32, [vp, vp, vp]),
         "shaw_fom_time": (i32, [vp, u64, P(dbl), P(dbl)]),
+        "shaw_fom_time_chunks": (i32, [vp, u64, u64, vp]),
+        "shaw_fom_complexity": (i32, [u64, u64, u64, u64, u64, i32, P(dbl), P(dbl)]),
         "shaw_fom_bytes_per_step": (i32, [vp, P(dbl), P(dbl)]),
         "shaw_fom_config": (i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i32), P(i32), P(i32)]),
         "shaw_fom_autotune": (i32, [vp, u64, P(C.c_double)]),
@@ -361,6 +363,13 @@ class Fom:
         _check(lib().shaw_fom_time(self._h, nsteps, C.byref(tot), C.byref(per)))
         return tot.value, per.value
 
+    def time_chunks(self, nsteps: int, chunk: int) -> np.ndarray:
+        """Durations [ms] of the run's chunks of `chunk` steps (events between
+        chunks, one wait at the end)."""
+        out = np.zeros((nsteps + chunk - 1) // chunk)
+        _check(lib().shaw_fom_time_chunks(self._h, nsteps, chunk, _p(out)))
+        return out
+
     def bytes_per_step(self):
         k, m = C.c_double(), C.c_double()
         _check(lib().shaw_fom_bytes_per_step(self._h, C.byref(k), C.byref(m)))
@@ -604,6 +613,13 @@ def reconstruct(phi: np.ndarray, q: np.ndarray) -> np.ndarray:
     out = np.zeros((q.shape[1], n))  # column-major [n x ncols]
     _check(lib().shaw_reconstruct(_p(phi), n, k, _p(q), q.shape[1], _p(out)))
     return out.T
+
+
+def fom_complexity(nvp: int, nsp: int, nnz_vp: int, nnz_sp: int, m: int, rank2: bool):
+    """(memMB, flops) of one FOM step in the reference's printPerf model."""
+    mb, fl = C.c_double(), C.c_double()
+    _check(lib().shaw_fom_complexity(nvp, nsp, nnz_vp, nnz_sp, m, int(rank2), C.byref(mb), C.byref(fl)))
+    return mb.value, fl.value
 
 
 def debug_copy(nbytes: int, reps: int = 10) -> float:

@@ -140,4 +140,35 @@ int shaw_forcing_table(int m, const int* kinds, const double* delays, const doub
   });
 }
 
+// complexityFom (src/kokkos/fom_complexities.hpp:57-137) with the counts of
+// src/shared/complexities/spmv.hpp, spmm.hpp and mult.hpp (mult_beta_one):
+// FP64 values, 8-byte ordinals, each term summed in the same order.
+int shaw_fom_complexity(uint64_t nvp, uint64_t nsp, uint64_t nnzVp, uint64_t nnzSp, uint64_t m, int rank2,
+                        double* memMB, double* flops) {
+  return guarded([&] {
+    SHAW_REQUIRE(memMB && flops && m > 0, SHAW_ERR_ARG, "bad argument");
+    constexpr uint64_t sc = sizeof(double), ord = sizeof(uint64_t);
+    // spmm with k columns (spmv: k = 1, the same counts)
+    auto spmm = [&](uint64_t nnz, uint64_t rows, uint64_t k, double& mb, double& fl) {
+      const double matsize = 1.0 * (nnz * (sc + ord) + rows * ord);
+      const double xr = 1. * nnz * k * sc, cr = 1. * rows * k * sc, cw = 1. * rows * k * sc;
+      mb = (matsize + xr + cr + cw) / 1024. / 1024.;
+      fl = 2. * nnz * k + rows * k;
+    };
+    double mb[3], fl[3];
+    const uint64_t k = rank2 ? m : 1;
+    spmm(nnzVp, nvp, k, mb[0], fl[0]);
+    if (rank2) {  // the per-forcing add: about 4 words and 3 flops per forcing
+      mb[1] = 4. * m * sc / 1024. / 1024.;
+      fl[1] = 3. * m;
+    } else {  // mult_beta_one over nvp
+      mb[1] = (4.0 * nvp * sc) / 1024. / 1024.;
+      fl[1] = 3.0 * nvp;
+    }
+    spmm(nnzSp, nsp, k, mb[2], fl[2]);
+    *memMB = mb[0] + mb[1] + mb[2];
+    *flops = fl[0] + fl[1] + fl[2];
+  });
+}
+
 }  // extern "C"

@@ -1284,6 +1284,7 @@ struct shaw_fom {
   virtual void getState(double* vp, double* sp) = 0;
   virtual void setState(const double* vp, const double* sp) = 0;
   virtual void time(uint64_t nsteps, double* total, double* perStep) = 0;
+  virtual void timeChunks(uint64_t nsteps, uint64_t chunk, double* ms) = 0;
   virtual void bytes(double* kernel, double* model) const = 0;
   virtual void config(int* kernel, int* cpt, int* strip, int* seg, int* pf, int* spl, int* colBlock) const = 0;
   virtual void pod(int dof, uint64_t k, const double* w, double* phi, double* sigma) = 0;
@@ -1497,9 +1498,16 @@ class FomEngine final : public shaw_fom {
       }
     }
     // eager launches: the widest pack (up to spl_ steps) that fits the
-    // remaining steps and has no snapshot due at an intermediate step
+    // remaining steps and has no snapshot due at an intermediate step. A
+    // launch of the multi-step kernels costs about the same whatever its
+    // steps (its time follows the bytes it moves), so when the tail would
+    // end in a pack of one or two steps (which run the pair or single-step
+    // kernel, shaped for wide column blocks) the last two packs split the
+    // tail evenly instead: 8 steps at six per launch run 4 + 4, not 6 + 2.
     for (; k < nsteps;) {
+      const uint64_t left = nsteps - k;
       int ns = spl_;
+      if (spl_ >= 3 && left > (uint64_t)spl_ && left < (uint64_t)spl_ + 3) ns = (int)((left + 1) / 2);
       while (ns > 1 && (k + ns > nsteps || packBlocked(ns))) --ns;
       switch (ns) {
         case 1: launchStep(); break;
@@ -1645,6 +1653,32 @@ class FomEngine final : public shaw_fom {
     SHAW_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
     *total = ms;
     *perStep = nsteps ? ms / (double)nsteps : 0.0;
+  }
+  // the run split into chunks of `chunk` steps with an event between
+  // chunks, all enqueued before one wait: ms[i] = duration of chunk i
+  void timeChunks(uint64_t nsteps, uint64_t chunk, double* ms) override {
+    const uint64_t n = (nsteps + chunk - 1) / chunk;
+    std::vector<hipEvent_t> ev(n + 1, nullptr);
+    try {
+      for (auto& e : ev) SHAW_HIP(hipEventCreate(&e));
+      SHAW_HIP(hipEventRecord(ev[0], stream_));
+      for (uint64_t i = 0; i < n; ++i) {
+        advance(std::min(chunk, nsteps - i * chunk));
+        SHAW_HIP(hipEventRecord(ev[i + 1], stream_));
+      }
+      SHAW_HIP(hipEventSynchronize(ev[n]));
+      for (uint64_t i = 0; i < n; ++i) {
+        float t = 0;
+        SHAW_HIP(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+        ms[i] = t;
+      }
+    } catch (...) {
+      (void)hipStreamSynchronize(stream_);
+      for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+      throw;
+    }
+    for (auto e : ev) (void)hipEventDestroy(e);
   }
   void bytes(double* kernel, double* model) const override {
     // algorithmic minimum: the M state columns and the coefficients once
@@ -2159,6 +2193,12 @@ int shaw_fom_time(shaw_fom_t h, uint64_t nsteps, double* total, double* perStep)
   return guarded([&] {
     SHAW_REQUIRE(h && total && perStep, SHAW_ERR_ARG, "null argument");
     h->time(nsteps, total, perStep);
+  });
+}
+int shaw_fom_time_chunks(shaw_fom_t h, uint64_t nsteps, uint64_t chunk, double* chunk_ms) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && chunk_ms && chunk > 0, SHAW_ERR_ARG, "bad argument");
+    h->timeChunks(nsteps, chunk, chunk_ms);
   });
 }
 int shaw_fom_bytes_per_step(shaw_fom_t h, double* kernel, double* model) {

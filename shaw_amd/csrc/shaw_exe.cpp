@@ -51,40 +51,42 @@ std::vector<Realization> realizations(const InputParser& p) {
   return r;
 }
 
-// src/shared/various/print_perf.hpp:57-84 with the reference's flop/byte
-// model (fom_complexities.hpp:57-137, complexities/{spmv,spmm,mult}.hpp)
-void printPerf(uint64_t loops, double aveSec, double memMB, double flops) {
+// src/shared/various/print_perf.hpp:57-84; perfTimes = {min, max, total}
+// seconds per step (fom_run.hpp:144-148), the byte/flop model from
+// shaw_fom_complexity
+void printPerf(uint64_t loops, const double (&perf)[3], double memMB, double flops) {
+  const double minSec = perf[0], maxSec = perf[1], aveSec = perf[2] / (double)loops;
   const double memGB = memMB / 1024.;
   std::cout << "flops = " << flops << std::endl;
   std::cout << "memMB = " << memMB << std::endl;
   std::cout << "flops/bytes = " << flops / (memMB * 1024. * 1024.) << std::endl;
   std::printf("aveBandwidth(GB/s) = %8.2lf \n", memGB / aveSec);
-  std::printf("minBandwidth(GB/s) = %8.2lf \n", memGB / aveSec);
-  std::printf("maxBandwidth(GB/s) = %8.2lf \n", memGB / aveSec);
+  std::printf("minBandwidth(GB/s) = %8.2lf \n", memGB / maxSec);
+  std::printf("maxBandwidth(GB/s) = %8.2lf \n", memGB / minSec);
   std::printf("aveGFlop = %7.4lf \n", flops / aveSec / 1e9);
-  std::printf("minGFlop = %7.4lf \n", flops / aveSec / 1e9);
-  std::printf("maxGFlop = %7.4lf \n", flops / aveSec / 1e9);
-  std::printf("totTime(se) = %7.4lf \n", aveSec * (double)loops);
+  std::printf("minGFlop = %7.4lf \n", flops / maxSec / 1e9);
+  std::printf("maxGFlop = %7.4lf \n", flops / minSec / 1e9);
+  std::printf("totTime(se) = %7.4lf \n", perf[2]);
   std::printf("aveTime(ms) = %7.4lf \n", aveSec * 1000);
-  std::printf("maxTime(ms) = %7.4lf \n", aveSec * 1000);
-  std::printf("minTime(ms) = %7.4lf \n", aveSec * 1000);
+  std::printf("maxTime(ms) = %7.4lf \n", maxSec * 1000);
+  std::printf("minTime(ms) = %7.4lf \n", minSec * 1000);
+  std::fflush(stdout);
 }
 
-void fomComplexity(uint64_t nVp, uint64_t nSp, uint64_t m, bool rank2, double& memMB, double& flops) {
-  const double sc = 8, ord = 8;
-  const double nnzVp = 2.0 * nSp, nnzSp = 2.0 * nSp;  // 4 nVp - 2 nr - 2 nth == 2 nSp
-  auto spmm = [&](double nnz, double rows, double k, double& mb, double& fl) {
-    mb = (nnz * (sc + ord) + rows * ord + nnz * k * sc + 2 * rows * k * sc) / 1024. / 1024.;
-    fl = 2. * nnz * k + rows * k;
-  };
-  double m0, f0, m1, f1, m2, f2;
-  spmm(nnzVp, (double)nVp, rank2 ? (double)m : 1.0, m0, f0);
-  if (rank2) { m1 = 4. * m * sc / 1024. / 1024.; f1 = 3. * m; }
-  else { m1 = 4.0 * nVp * sc / 1024. / 1024.; f1 = 3.0 * nVp; }
-  spmm(nnzSp, (double)nSp, rank2 ? (double)m : 1.0, m2, f2);
-  memMB = m0 + m1 + m2;
-  flops = f0 + f1 + f2;
-}
+// steps per timed chunk of the FOM run (two graph replays at six steps per
+// launch)
+constexpr uint64_t kTimingChunk = 144;
+
+// per-step times of a run timed in chunks: chunk i's duration over its steps
+struct PerfTimes {
+  double v[3] = {1e300, 0.0, 0.0};  // min, max, total [s]
+  void add(double chunkMs, uint64_t steps) {
+    const double perStep = chunkMs / 1000.0 / (double)steps;
+    v[0] = std::min(v[0], perStep);
+    v[1] = std::max(v[1], perStep);
+    v[2] += chunkMs / 1000.0;
+  }
+};
 
 class Driver {
  public:
@@ -197,13 +199,15 @@ class Driver {
       check(shaw_fom_set_snapshots(fom, 1, p_.spSnapFreq, M, cols.data(), N / p_.spSnapFreq));
     }
     const auto t0 = std::chrono::high_resolution_clock::now();
-    const uint64_t chunk = 2000;
-    double ms = 0;
-    for (uint64_t s = 0; s < N; s += chunk) {
-      const uint64_t n = std::min(chunk, N - s);
-      double tot, per;
-      check(shaw_fom_time(fom, n, &tot, &per));
-      ms += tot;
+    // blocks of 2000 steps (the reference's progress lines), each timed in
+    // chunks of kTimingChunk steps for the per-step minimum and maximum
+    const uint64_t block = 2000;
+    PerfTimes perf;
+    std::vector<double> chunkMs(block / kTimingChunk + 1);
+    for (uint64_t s = 0; s < N; s += block) {
+      const uint64_t n = std::min(block, N - s);
+      check(shaw_fom_time_chunks(fom, n, kTimingChunk, chunkMs.data()));
+      for (uint64_t c = 0; c * kTimingChunk < n; ++c) perf.add(chunkMs[c], std::min(kTimingChunk, n - c * kTimingChunk));
       if ((s + n) % 2000 == 0) std::cout << "Doing step = " << s + n << std::endl;
     }
     const auto t1 = std::chrono::high_resolution_clock::now();
@@ -226,8 +230,10 @@ class Driver {
     std::cout.unsetf(std::ios::fixed);
     std::cout << std::setprecision(6);
     double memMB, flops;
-    fomComplexity(info_.nvp, info_.nsp, M, !rank1Runs, memMB, flops);
-    printPerf(N, ms / 1000.0 / (double)N, memMB * (rank1Runs ? (double)M : 1.0), flops * (rank1Runs ? (double)M : 1.0));
+    // nnz(J_vp) = 4 nvp - 2 nr - 2 nth = 2 nsp and nnz(J_sp) = 2 nsp on the full mesh
+    check(shaw_fom_complexity(info_.nvp, info_.nsp, 2 * info_.nsp, 2 * info_.nsp, M, rank1Runs ? 0 : 1, &memMB,
+                              &flops));
+    printPerf(N, perf.v, memMB * (rank1Runs ? (double)M : 1.0), flops * (rank1Runs ? (double)M : 1.0));
 
     const auto w0 = std::chrono::high_resolution_clock::now();
     const uint64_t R = recv.size();
@@ -383,7 +389,7 @@ class Driver {
       av.resize(Kv * Tv * fs);
       as.resize(Ks * Tsp * fs);
       const auto l0 = std::chrono::high_resolution_clock::now();
-      double ms = 0;
+      PerfTimes perf;
       uint64_t step = 0;
       while (step < N) {
         uint64_t next = N;
@@ -393,7 +399,7 @@ class Driver {
         }
         double tot;
         check(shaw_rom_time(rom, next - step, &tot));
-        ms += tot;
+        perf.add(tot, next - step);
         step = next;
         if (snaps) {
           std::vector<double> qv(Kv), qs(Ks);
@@ -413,7 +419,7 @@ class Driver {
       std::cout << std::setprecision(6);
       const double memMB = (2.0 * Kv * Ks * 8 * 2 + 3.0 * (Kv + Ks) * 8 * fs) / 1024. / 1024.;
       const double flops = 4.0 * Kv * Ks * fs + 3.0 * Kv * fs;
-      printPerf(N, ms / 1000.0 / (double)N, memMB, flops);
+      printPerf(N, perf.v, memMB, flops);
       std::vector<double> seis(R * (every ? N / every : 0) * fs);
       for (uint64_t j = 0; j < fs; ++j) {
         uint64_t T = 0;

@@ -11,6 +11,7 @@ import subprocess
 import pytest
 import yaml
 
+import oracle
 import shaw_amd
 from tests.helpers import MESHES, load_input
 
@@ -120,3 +121,37 @@ material:
 """
     r = run(text, tmp_path)
     assert "numSteps = 8000" in r.stdout and "mesh_info.dat" in r.stderr
+
+
+def _reference_complexity(nvp, nsp, nnz_vp, nnz_sp, m, rank2):
+    """complexityFom (src/kokkos/fom_complexities.hpp:57-137) over the counts
+    of src/shared/complexities/spmv.hpp:56-71, spmm.hpp:58-75 and mult.hpp
+    (mult_beta_one: 4 n words, 3 n flops), FP64 values, 8-byte ordinals."""
+    def spmm(nnz, rows, k):
+        return ((nnz * 16 + rows * 8) + nnz * k * 8 + rows * k * 8 + rows * k * 8) / 1024 / 1024, 2 * nnz * k + rows * k
+    k = m if rank2 else 1
+    a = spmm(nnz_vp, nvp, k)
+    b = (4 * m * 8 / 1024 / 1024, 3 * m) if rank2 else (4 * nvp * 8 / 1024 / 1024, 3 * nvp)
+    c = spmm(nnz_sp, nsp, k)
+    return a[0] + b[0] + c[0], a[1] + b[1] + c[1]
+
+
+@pytest.mark.parametrize("mesh", ["fullMesh4x5", "fullMesh21x51"])
+@pytest.mark.parametrize("m,rank2", [(1, False), (3, True), (32, True)])
+def test_fom_complexity_model(mesh, m, rank2):
+    """The byte/flop model shawExe prints (memMB, flops) equals the
+    reference's, with the nnz of the oracle's CSR Jacobians; shawExe passes
+    2 nsp for both, which is that nnz on every full mesh."""
+    o = oracle.App(os.path.join(MESHES, mesh), {"kind": "unilayer", "layer": {"density": [2500.0],
+                                                                           "velocity": [5000.0]}})
+    nnz_vp = len(o.csr("vp")[2])
+    nnz_sp = len(o.csr("sp")[2])
+    nvp, nsp = o.nVp, o.nSp
+    assert nnz_vp == nnz_sp == 2 * nsp
+    got = shaw_amd.fom_complexity(nvp, nsp, nnz_vp, nnz_sp, m, rank2)
+    want = _reference_complexity(nvp, nsp, nnz_vp, nnz_sp, m, rank2)
+    assert got[0] == pytest.approx(want[0], rel=1e-15) and got[1] == want[1]
+    if mesh == "fullMesh4x5" and not rank2:
+        # by hand: nnz 62, 20 velocity and 31 stress points
+        assert got[1] == (2 * 62 + 20) + 3 * 20 + (2 * 62 + 31)
+        assert got[0] * 1024 * 1024 == pytest.approx((62 * 16 + 160 + 62 * 8 + 320) + 640 + (62 * 16 + 248 + 62 * 8 + 496))

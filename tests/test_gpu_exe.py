@@ -37,6 +37,35 @@ def read_ascii(path, header):
     return np.array([[float(x) for x in ln.split()] for ln in lines])
 
 
+def parse_perf(stdout: str) -> dict:
+    """The printPerf block (src/shared/various/print_perf.hpp:57-84)."""
+    out = {}
+    for ln in stdout.splitlines():
+        if " = " in ln:
+            k, v = ln.split(" = ", 1)
+            try:
+                out[k.strip()] = float(v.split()[0])
+            except (ValueError, IndexError):
+                pass
+    return out
+
+
+def check_perf(perf: dict, nvp: int, nsp: int, m: int, rank2: bool, nsteps: int):
+    """memMB and flops equal the reference's complexityFom model (nnz = 2 nsp,
+    checked against the oracle CSR in test_exe_cpu.py); the per-step times
+    order as min <= ave <= max, the bandwidths and GFlop/s the other way, and
+    the total is the average times the steps (printed to 4 decimals)."""
+    from tests.test_exe_cpu import _reference_complexity
+    mb, fl = _reference_complexity(nvp, nsp, 2 * nsp, 2 * nsp, m, rank2)
+    scale = 1 if rank2 else m  # rank-1 realizations run together as m columns
+    assert perf["memMB"] == pytest.approx(mb * scale, rel=1e-5)
+    assert perf["flops"] == pytest.approx(fl * scale, rel=1e-5)
+    assert perf["minTime(ms)"] <= perf["aveTime(ms)"] <= perf["maxTime(ms)"]
+    assert perf["minBandwidth(GB/s)"] <= perf["aveBandwidth(GB/s)"] <= perf["maxBandwidth(GB/s)"]
+    assert perf["minGFlop"] <= perf["aveGFlop"] <= perf["maxGFlop"]
+    assert perf["totTime(se)"] == pytest.approx(perf["aveTime(ms)"] * nsteps / 1e3, rel=1e-3, abs=2e-4)
+
+
 @pytest.mark.parametrize("test", FOM_TESTS)
 def test_exe_regression_inputs(test, tmp_path):
     cfg = load_input(test)
@@ -52,6 +81,7 @@ def test_exe_regression_inputs(test, tmp_path):
     sets = [reals[i:i + fs] for i in range(0, len(reals), fs)] if fs else [[r] for r in reals]
     ora = oracle.App(cfg["general"]["meshDir"], cfg["material"])
     nsteps = int(g["finalTime"] / g["dt"])
+    check_perf(parse_perf(res.stdout), ora.nVp, ora.nSp, fs if fs else len(reals), fs is not None, nsteps)
     for rid, s in enumerate(sets):
         ref = ora.fom_run(g["dt"], nsteps, s, fs is not None, io["seismogram"]["receivers"], io["seismogram"]["freq"],
                           io["snapshotMatrix"]["velocity"]["freq"], io["snapshotMatrix"]["stress"]["freq"], sig["kind"])
