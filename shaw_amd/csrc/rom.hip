@@ -1,10 +1,9 @@
 // Galerkin ROM on gfx950.
 //
 // Projection (FP64): Ar_v = Phi_v^T (J_vp Phi_s), Ar_s = Phi_s^T (J_sp Phi_v)
-// (rom_compute_jacobians.hpp:58-109). The sparse products apply the same
-// matrix-free stencil as the FOM kernels to the basis columns; the tall-skinny
-// Phi^T (J Phi) contraction is a split-K FP64 GEMM over row chunks whose
-// partial products are reduced in a fixed order (deterministic).
+// (rom_compute_jacobians.hpp:58-109), on FP64 MFMA with the FOM stencil
+// applied to the basis columns inside the product (proj.hip); split-K over
+// row chunks whose partial tiles are reduced in a fixed order.
 //
 // Reduced leapfrog (rom_run_rank_one.hpp:112-151 per column; batched as in
 // torevise/run_rom_rank_two_forcing.hpp:149-188), for resident column slots:
@@ -69,46 +68,7 @@ __host__ __device__ inline float bf16ToF32(unsigned short h) {
   return x;
 }
 
-// ------------------------------------------------------------ projection
-// tmp[p + n*k] = (J_vp Phi_s)[p][k] in CSR row order; Phi column-major
-__global__ void k_apply_jvp(const double* __restrict__ cv, const double* __restrict__ phi, double* __restrict__ out,
-                            uint32_t nr, uint32_t nth, uint64_t K) {
-  const uint64_t nVp = (uint64_t)nr * nth, nSp = (uint64_t)(nr - 1) * nth + (uint64_t)nr * (nth - 1);
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t k = blockIdx.y;
-  if (p >= nVp || k >= K) return;
-  const uint64_t rs = 2ull * nth - 1, i = p % nth, j = p / nth;
-  auto stp = [&](uint64_t ii, uint64_t jj) { return jj + 1 < nr ? jj * rs + 2 * ii + 1 : (uint64_t)(nr - 1) * rs + ii; };
-  const uint64_t iN = (j < nr - 1 ? j : nr - 2) * rs + 2 * i, iS = (j > 0 ? j - 1 : 0) * rs + 2 * i;
-  const uint64_t iW = stp(i > 0 ? i - 1 : 0, j), iE = stp(i < nth - 2 ? i : nth - 2, j);
-  const double* x = phi + nSp * k;
-  const double* c = cv + 4 * p;
-  double s = c[0] * x[iN] + c[1] * x[iS];
-  s = s + c[2] * x[iW];
-  s = s + c[3] * x[iE];
-  out[p + nVp * k] = s;
-}
-
-__global__ void k_apply_jsp(const double* __restrict__ cs, const double* __restrict__ phi, double* __restrict__ out,
-                            uint32_t nr, uint32_t nth, uint64_t K) {
-  const uint64_t nVp = (uint64_t)nr * nth, nSp = (uint64_t)(nr - 1) * nth + (uint64_t)nr * (nth - 1);
-  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t k = blockIdx.y;
-  if (q >= nSp || k >= K) return;
-  const uint64_t rs = 2ull * nth - 1, last = (uint64_t)(nr - 1) * rs;
-  uint64_t n0, n1;
-  if (q < last) {
-    const uint64_t j = q / rs, kk = q % rs;
-    if ((kk & 1) == 0) { n0 = (j + 1) * nth + kk / 2; n1 = j * nth + kk / 2; }
-    else { n0 = j * nth + (kk - 1) / 2; n1 = n0 + 1; }
-  } else {
-    n0 = (uint64_t)(nr - 1) * nth + (q - last);
-    n1 = n0 + 1;
-  }
-  const double* x = phi + nVp * k;
-  out[q + nSp * k] = cs[2 * q] * x[n0] + cs[2 * q + 1] * x[n1];
-}
-
+// ------------------------------------------------------------ POD products
 // part[chunk][a][b] = sum_{r in chunk} A[r + n*a] * B[r + n*b]   (A: n x Ka, B: n x Kb, col-major)
 constexpr int GT = 32;  // output tile
 __global__ void __launch_bounds__(256) k_gemm_tn_partial(const double* __restrict__ A, const double* __restrict__ B,
@@ -787,33 +747,20 @@ int shaw_rom_project(shaw_app_t app, const double* phiV, uint64_t kv, const doub
     SHAW_REQUIRE(app && phiV && phiS && arV && arS && kv > 0 && ks > 0, SHAW_ERR_ARG, "bad argument");
     const Grid& g = app->ops.grid;
     const uint64_t nVp = g.nVp(), nSp = g.nSp();
-    DevBuf<double> cv, cs, pv, ps, tmp, part, out;
+    DevBuf<double> cv, cs, pv, ps, part, out;
     cv.alloc(app->ops.coefVp.size()); cv.upload(app->ops.coefVp.data(), cv.n);
     cs.alloc(app->ops.coefSp.size()); cs.upload(app->ops.coefSp.data(), cs.n);
     pv.alloc(nVp * kv); pv.upload(phiV, pv.n);
     ps.alloc(nSp * ks); ps.upload(phiS, ps.n);
-    auto gemm = [&](const double* A, uint64_t n, uint32_t Ka, const double* B, uint32_t Kb, double* host) {
-      const uint64_t chunk = std::max<uint64_t>(4096, (n + 255) / 256 / GT * GT + GT);
-      const uint32_t nch = (uint32_t)((n + chunk - 1) / chunk);
-      part.alloc((size_t)nch * Ka * Kb);
-      out.alloc((size_t)Ka * Kb);
-      dim3 grid((Ka + GT - 1) / GT, (Kb + GT - 1) / GT, nch);
-      hipLaunchKernelGGL(k_gemm_tn_partial, grid, dim3(256), 0, 0, A, B, part.p, n, Ka, Kb, chunk);
-      const uint64_t ne = (uint64_t)Ka * Kb;
-      hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, 0, part.p, out.p, ne, nch);
-      SHAW_HIP(hipGetLastError());
-      SHAW_HIP(hipMemcpy(host, out.p, ne * sizeof(double), hipMemcpyDeviceToHost));
-    };
-    // velocity: tmp = J_vp Phi_s (nVp x ks); Ar_v = Phi_v^T tmp
-    tmp.alloc(nVp * ks);
-    hipLaunchKernelGGL(k_apply_jvp, dim3((unsigned)((nVp + 255) / 256), (unsigned)ks), dim3(256), 0, 0, cv.p, ps.p, tmp.p,
-                       (uint32_t)g.nr, (uint32_t)g.nth, ks);
-    gemm(pv.p, nVp, (uint32_t)kv, tmp.p, (uint32_t)ks, arV);
-    // stress: tmp = J_sp Phi_v (nSp x kv); Ar_s = Phi_s^T tmp
-    tmp.alloc(nSp * kv);
-    hipLaunchKernelGGL(k_apply_jsp, dim3((unsigned)((nSp + 255) / 256), (unsigned)kv), dim3(256), 0, 0, cs.p, pv.p, tmp.p,
-                       (uint32_t)g.nr, (uint32_t)g.nth, kv);
-    gemm(ps.p, nSp, (uint32_t)ks, tmp.p, (uint32_t)kv, arS);
+    part.alloc(std::max(projPartDoubles(nVp, kv, ks), projPartDoubles(nSp, ks, kv)));
+    out.alloc(kv * ks);
+    // Ar_v = Phi_v^T (J_vp Phi_s), Ar_s = Phi_s^T (J_sp Phi_v)
+    projectTN(0, (uint32_t)g.nr, (uint32_t)g.nth, nVp, nSp, cv.p, pv.p, (uint32_t)kv, nullptr, ps.p, (uint32_t)ks,
+              part.p, out.p, 0);
+    SHAW_HIP(hipMemcpy(arV, out.p, kv * ks * sizeof(double), hipMemcpyDeviceToHost));
+    projectTN(1, (uint32_t)g.nr, (uint32_t)g.nth, nSp, nVp, cs.p, ps.p, (uint32_t)ks, nullptr, pv.p, (uint32_t)kv,
+              part.p, out.p, 0);
+    SHAW_HIP(hipMemcpy(arS, out.p, kv * ks * sizeof(double), hipMemcpyDeviceToHost));
   });
 }
 
@@ -824,27 +771,17 @@ int shaw_rom_project_energy(shaw_app_t app, const double* phiV, uint64_t kv, con
     SHAW_REQUIRE(app && phiV && phiS && mv && arV && arS && kv > 0 && ks > 0, SHAW_ERR_ARG, "bad argument");
     const Grid& g = app->ops.grid;
     const uint64_t nVp = g.nVp(), nSp = g.nSp();
-    DevBuf<double> cv, pv, ps, tmp, part, out;
+    DevBuf<double> cv, pv, ps, w, part, out;
     cv.alloc(app->ops.coefVp.size()); cv.upload(app->ops.coefVp.data(), cv.n);
-    // left factor diag(Mv) Phi_v, scaled on the host (exact products)
-    std::vector<double> left(nVp * kv);
-    for (uint64_t c = 0; c < kv; ++c)
-      for (uint64_t i = 0; i < nVp; ++i) left[i + nVp * c] = mv[i] * phiV[i + nVp * c];
-    pv.alloc(nVp * kv); pv.upload(left.data(), pv.n);
+    pv.alloc(nVp * kv); pv.upload(phiV, pv.n);
     ps.alloc(nSp * ks); ps.upload(phiS, ps.n);
-    tmp.alloc(nVp * ks);
-    hipLaunchKernelGGL(k_apply_jvp, dim3((unsigned)((nVp + 255) / 256), (unsigned)ks), dim3(256), 0, 0, cv.p, ps.p, tmp.p,
-                       (uint32_t)g.nr, (uint32_t)g.nth, ks);
-    const uint64_t chunk = std::max<uint64_t>(4096, (nVp + 255) / 256 / GT * GT + GT);
-    const uint32_t nch = (uint32_t)((nVp + chunk - 1) / chunk);
-    part.alloc((size_t)nch * kv * ks);
-    out.alloc((size_t)kv * ks);
-    hipLaunchKernelGGL(k_gemm_tn_partial, dim3((unsigned)((kv + GT - 1) / GT), (unsigned)((ks + GT - 1) / GT), nch), dim3(256),
-                       0, 0, pv.p, tmp.p, part.p, nVp, (uint32_t)kv, (uint32_t)ks, chunk);
-    const uint64_t ne = kv * ks;
-    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, 0, part.p, out.p, ne, nch);
-    SHAW_HIP(hipGetLastError());
-    SHAW_HIP(hipMemcpy(arV, out.p, ne * sizeof(double), hipMemcpyDeviceToHost));
+    w.alloc(nVp); w.upload(mv, nVp);
+    part.alloc(projPartDoubles(nVp, kv, ks));
+    out.alloc(kv * ks);
+    // Ar_v = (diag(Mv) Phi_v)^T (J_vp Phi_s); Ar_s = -Ar_v^T
+    projectTN(0, (uint32_t)g.nr, (uint32_t)g.nth, nVp, nSp, cv.p, pv.p, (uint32_t)kv, w.p, ps.p, (uint32_t)ks, part.p,
+              out.p, 0);
+    SHAW_HIP(hipMemcpy(arV, out.p, kv * ks * sizeof(double), hipMemcpyDeviceToHost));
     for (uint64_t r = 0; r < ks; ++r)
       for (uint64_t c = 0; c < kv; ++c) arS[r * kv + c] = -arV[c * ks + r];
   });

@@ -1,5 +1,7 @@
 // Entry points shared between the FOM and ROM translation units.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <vector>
 
@@ -14,4 +16,15 @@ void podDevice(double* dS, uint64_t n, uint64_t ns, uint64_t k, double* phi, dou
 // w[n] eigenvalues (unordered), V row-major [n x n] with the eigenvector of
 // w[c] in column c. symeig.cpp.
 void symmetricEigen(std::vector<double>& A, int n, std::vector<double>& w, std::vector<double>& V);
+// out (device, row-major [Ka][Kb]) = L^T diag(w) (J X) in FP64 on MFMA
+// (proj.hip): op 0 J = J_vp (L: n = nVp rows, X: nX = nSp rows, coef =
+// coefVp [nVp][4]), op 1 J = J_sp (n = nSp, nX = nVp, coef = coefSp
+// [nSp][2]), op 2 J = I (nX = n, coef unused). L, X column-major on the
+// device; w [n] or null. part: device scratch of projPartDoubles(n, Ka, Kb)
+// doubles (split-K partial tiles, every entry written before it is read).
+void projectTN(int op, uint32_t nr, uint32_t nth, uint64_t n, uint64_t nX, const double* coef, const double* L,
+               uint32_t Ka, const double* w, const double* X, uint32_t Kb, double* part, double* out,
+               hipStream_t stream);
+uint32_t projChunks(uint64_t n, uint64_t Ka, uint64_t Kb);
+uint64_t projPartDoubles(uint64_t n, uint64_t Ka, uint64_t Kb);
 }  // namespace shaw
