@@ -251,6 +251,22 @@ int shaw_rom_project(shaw_app_t app, const double* phi_v, uint64_t kv, const dou
 int shaw_rom_project_energy(shaw_app_t app, const double* phi_v, uint64_t kv, const double* phi_s,
                             uint64_t ks, const double* mv, double* ar_v, double* ar_s);
 
+/* shaw_rom_project on device pointers that allocates nothing. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc). coef_vp [nvp][4] and coef_sp [nsp][2] (as shaw_app_operators
+ * returns them), phi_v LayoutLeft [nvp x kv], phi_s LayoutLeft [nsp x ks],
+ * ar_v [kv][ks] and ar_s [ks][kv]: all device memory of the current device;
+ * stream: a hipStream_t (NULL: the null stream). Only enqueues: the
+ * operators are ready when the stream reaches them. The split-K partial
+ * tiles live in a 16 MiB array of the library on each GPU, so kv * ks is at
+ * most 2097152 (SHAW_ERR_ARG otherwise, before any launch), and calls that
+ * overlap on one GPU race. Same kernels and chunking as shaw_rom_project:
+ * equal operators bit for bit.                                             */
+int shaw_rom_project_noalloc(uint64_t nr, uint64_t nth, const double* coef_vp, const double* coef_sp,
+                             const double* phi_v, uint64_t kv, const double* phi_s, uint64_t ks,
+                             double* ar_v, double* ar_s, void* stream);
+
 typedef struct {
   uint64_t kv, ks;
   double dt;
