@@ -442,25 +442,93 @@ def test_full_config2_shape_vs_oracle():
     assert np.count_nonzero(got["final_vp"]) > 32 * 100
 
 
-def test_config5_shape_vs_oracle():
-    """2048 x 8192 (config 5's grid and material) with 4 of its seeded sources
-    (delay 0), 12 steps = two launches of the default six-step kernel, 180
-    receivers every step: seismograms and final states bit-identical to the
-    oracle's CSR run at the largest grid."""
-    say("[gpu] config-5 shape vs oracle (4 columns)")
+def _config5_run(app, w, gids, tab, n, rec, cols, kernel=None, precision=None, every=1, snap=True):
+    """One config-5 FOM run of all 64 columns: seismograms [64][T][180] and
+    the states of the columns `cols` at step n (snapshots, so the 51 GB
+    state is never copied to the host)."""
+    kw = {}
+    if kernel is not None:
+        kw["kernel"] = kernel
+    if precision is not None:
+        kw["precision"] = precision
+    f = shaw_amd.Fom(app, w.m, w.dt, rank1_forcing=False, **kw)
+    f.set_sources(gids)
+    f.set_forcing(tab)
+    f.set_receivers(rec, every, n // every)
+    if snap:
+        f.set_snapshots("vp", n, cols, 1)
+        f.set_snapshots("sp", n, cols, 1)
+    f.advance(n)
+    f.sync()
+    out = [f.seismogram()]
+    if snap:
+        out += [f.snapshots("vp"), f.snapshots("sp")]
+    f.close()
+    return out
+
+
+def test_config5_full_width_vs_oracle_split_and_linearity():
+    """Config 5 at its full width: 2048 x 8192, all M = 64 seeded sources
+    (delay 0, so the first steps carry signal), 12 steps = two launches of the
+    default six-step kernel, 180 receivers every step.
+    * Columns 0, 21, 42, 63: seismograms and the states at step 12 are
+      bit-identical to the oracle's CSR run of those four sources (the columns
+      are independent, so a subset run is the same computation).
+    * All 64 columns: the fused run equals the reference's half-step structure
+      (split kernels) bit for bit, and doubling the forcing doubles every
+      seismogram sample and state value exactly (the step is linear and a
+      factor of two commutes with rounding)."""
+    say("[gpu] config-5 full width (M = 64) vs oracle subset, split, linearity")
     from shaw_amd import workloads
     w = workloads.config5()
     mat = workloads.layered_material()
     app = shaw_amd.App.structured(w.nr, w.nth, mat)
-    ora = oracle.App("", mat, structured=(w.nr, w.nth, 0.0, 180.0, 3480.0, 6371.0))
-    src = [(d, a, p, 0.0) for d, a, p, dl in w.sources[:4]]
-    rec = [float(x) for x in range(180)]
+    src = [(d, a, p, 0.0) for d, a, p, dl in w.sources]
+    assert len(src) == 64
     n = 12
-    got = shaw_amd.fom_run(app, w.dt, n, src, True, rec, 1)
-    ref = ora.fom_run(w.dt, n, src, True, rec, 1)
-    for key in ref:
-        np.testing.assert_array_equal(got[key], ref[key], err_msg=key)
-    assert np.count_nonzero(got["final_vp"]) > 4 * 20
+    rec = [app.map_receiver(float(a)) for a in range(180)]
+    gids = [app.map_source(d, a) for d, a, p, dl in src]
+    tab = shaw_amd.forcing_table(["ricker"] * 64, [0.0] * 64, [s[2] for s in src], w.dt, n)
+    cols = [0, 21, 42, 63]
+    fused = _config5_run(app, w, gids, tab, n, rec, cols)
+    ora = oracle.App("", mat, structured=(w.nr, w.nth, 0.0, 180.0, 3480.0, 6371.0))
+    ref = ora.fom_run(w.dt, n, [src[c] for c in cols], True, [float(a) for a in range(180)], 1)
+    for i, c in enumerate(cols):
+        np.testing.assert_array_equal(fused[0][c], ref["seismogram"][i], err_msg=f"seismogram column {c}")
+        np.testing.assert_array_equal(fused[1][i, 0], ref["final_vp"][:, i], err_msg=f"vp column {c}")
+        np.testing.assert_array_equal(fused[2][i, 0], ref["final_sp"][:, i], err_msg=f"sp column {c}")
+    assert np.count_nonzero(fused[1]) > 4 * 20
+    say("[gpu] config-5 split kernels")
+    split = _config5_run(app, w, gids, tab, n, rec, cols, kernel=shaw_amd.KERNEL_SPLIT)
+    for a, b in zip(fused, split):
+        np.testing.assert_array_equal(a, b)
+    say("[gpu] config-5 doubled forcing")
+    double = _config5_run(app, w, gids, 2.0 * tab, n, rec, cols)
+    for a, b in zip(fused, double):
+        np.testing.assert_array_equal(2.0 * a, b)
+
+
+def test_config5_fp32_state_error():
+    """Config 5's FP32-state variant against the FP64 run over the first 8000
+    steps (200 s: every source has fired, delay 105 s, and its waves reach the
+    surface receivers), all 64 columns, 180 receivers every 10 steps: the
+    relative L2 error of each column's seismogram is at most 5e-4 (max) and
+    5e-5 (median)."""
+    say("[gpu] config-5 FP32 state vs FP64, 8000 steps")
+    from shaw_amd import workloads
+    w = workloads.config5()
+    mat = workloads.layered_material()
+    app = shaw_amd.App.structured(w.nr, w.nth, mat)
+    n = 8000
+    rec = [app.map_receiver(float(a)) for a in range(180)]
+    gids = [app.map_source(d, a) for d, a, p, dl in w.sources]
+    tab = shaw_amd.forcing_table(["ricker"] * 64, [s[3] for s in w.sources], [s[2] for s in w.sources], w.dt, n)
+    s64 = _config5_run(app, w, gids, tab, n, rec, [], every=10, snap=False)[0]
+    s32 = _config5_run(app, w, gids, tab, n, rec, [], precision=shaw_amd.FP32, every=10, snap=False)[0]
+    err = np.array([np.linalg.norm(s32[m] - s64[m]) / np.linalg.norm(s64[m]) for m in range(64)])
+    say(f"[gpu] config-5 FP32 vs FP64: max {err.max():.3e}, median {np.median(err):.3e}")
+    assert np.all(np.isfinite(err)) and np.linalg.norm(s64) > 0
+    assert err.max() <= 5e-4 and np.median(err) <= 5e-5
 
 
 def test_two_handles_interleaved_equal_alone(mesh64x256):
