@@ -100,6 +100,12 @@ int shaw_app_coords(shaw_app_t app, int dof, double* coords);
  * map_nominal_location_to_velocity_grid_point.hpp:57-117).                */
 int shaw_map_source(shaw_app_t app, double depth_km, double angle_deg, uint64_t* vp_gid);
 int shaw_map_receiver(shaw_app_t app, double angle_deg, uint64_t* vp_gid);
+/* FOM energy weights (no counterpart in the reference, whose ROM is
+ * Euclidean): mv [nvp], ms [nsp] > 0 with Mv J_vp = -(Ms J_sp)^T on every
+ * unmerged coupling, normalised to 1 at the mesh centre; the leapfrog
+ * conserves 1/2 v^T Mv v + 1/2 s^T Ms s. For shaw_fom_pod_weighted and
+ * shaw_rom_project_energy. Either pointer may be NULL.                     */
+int shaw_app_energy_weights(shaw_app_t app, double* mv, double* ms);
 void shaw_app_destroy(shaw_app_t app);
 
 /* Signal kinds (signal.hpp:86-128): 0 ricker, 1 sinusoid, 2 gaussDer.
@@ -306,6 +312,75 @@ int shaw_rom_state(shaw_rom_t h, uint64_t slot, double* qv, double* qs, uint64_t
 int shaw_rom_free_slots(shaw_rom_t h, uint64_t* n);
 int shaw_rom_time(shaw_rom_t h, uint64_t nsteps, double* total_ms);
 void shaw_rom_destroy(shaw_rom_t h);
+
+/* Shape of an engine: kv, ks, receivers, column slots, samples per column
+ * (any pointer may be NULL).                                               */
+int shaw_rom_info(shaw_rom_t h, uint64_t* kv, uint64_t* ks, uint64_t* r, uint64_t* max_cols,
+                  uint64_t* max_samples);
+
+/* ------------------------------------------------------------ serving --
+ * Continuous batching of ROM requests on resident engines. No counterpart in
+ * the reference, which runs one ROM problem per process
+ * (src/kokkos/main.cc:66-73, rom_problem_rank_one.hpp:123-159).
+ * An engine is a table of the shaw_rom_* calls the loop makes, so the same
+ * loop drives GPU engines (shaw_engine_from_rom) and any other engine with
+ * the same contract (a CPU one in the tests).                              */
+typedef struct {
+  void* ctx;
+  uint64_t kv, ks, r, capacity;  /* modes, receivers, column slots */
+  int (*admit)(void* ctx, uint64_t n, const double* b, uint64_t nsteps, const double* ftab,
+               const uint64_t* horizons, uint64_t every, uint64_t* slots);
+  int (*advance)(void* ctx, uint64_t nsteps);  /* enqueue only */
+  int (*sync)(void* ctx);
+  int (*retire_batch)(void* ctx, uint64_t n, const uint64_t* slots, uint64_t max_samples, double* seis,
+                      uint64_t* samples);
+  int (*free_slots)(void* ctx, uint64_t* n);
+} shaw_engine;
+/* The table of a GPU engine: shaw_rom_admit/advance/sync/retire_batch/free_slots. */
+int shaw_engine_from_rom(shaw_rom_t rom, shaw_engine* out);
+
+typedef struct {
+  uint64_t id;
+  int engine;            /* index into the engines */
+  uint64_t m;            /* forcing columns */
+  uint64_t horizon;      /* steps */
+  uint64_t every;        /* seismogram sampling period (0: none) */
+  double arrival_s;      /* arrival time on the server clock (0 = start of the run) */
+  const double* b;       /* [kv][m]: reduced forcing vector of each column */
+  const double* ftab;    /* [horizon][m]: forcing values */
+  double* seis;          /* out: [m][r x horizon/every] (NULL: discarded) */
+  double admitted_s;     /* out: admission time */
+  double done_s;         /* out: completion time (latency = done_s - arrival_s) */
+} shaw_request;
+
+#define SHAW_SERVE_FIFO 0        /* arrival order; the head is never passed */
+#define SHAW_SERVE_LEAST_WORK 1  /* least horizon x columns first among the requests that fit */
+
+typedef struct {
+  uint64_t hbm_budget;   /* bytes of resident columns (states, forcing, seismograms) */
+  uint64_t chunk_steps;  /* steps per engine launch between admissions */
+  int policy;            /* SHAW_SERVE_* */
+  double max_wait_s;     /* LEAST_WORK: a request waiting this long is admitted before any
+                            other and is not passed (0: no bound) */
+  double wall_limit_s;   /* fail with SHAW_ERR_STATE past this (0: none) */
+  /* optional clock [s] and idle wait (tests): NULL = steady clock and sleep */
+  double (*clock)(void* clock_ctx);
+  void (*wait_until)(void* clock_ctx, double t);
+  void* clock_ctx;
+} shaw_serve_options;
+
+typedef struct {
+  uint64_t requests, forcing_steps, launches, max_queue;
+  double span_s;         /* first arrival to last completion */
+  double p50_latency_s, p99_latency_s, max_latency_s;
+  double max_wait_s;     /* longest time from arrival to admission */
+  double throughput;     /* forcing-steps / span_s */
+} shaw_serve_stats;
+
+/* Serve n requests arriving at their arrival times until all are done;
+ * writes each request's seismograms, admitted_s and done_s, and the stats. */
+int shaw_serve(const shaw_engine* engines, int n_engines, const shaw_serve_options* opts,
+               shaw_request* reqs, uint64_t n, shaw_serve_stats* stats);
 
 /* POD basis: the k leading left singular vectors of the snapshot matrix
  * S LayoutLeft [n x ns] (the lsv_<dof> file content) and ns singular values,

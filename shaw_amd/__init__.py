@@ -104,6 +104,7 @@ def lib():
         "shaw_fom_set_state": (i32, [vp, vp, vp]),
         "shaw_fom_time": (i32, [vp, u64, P(dbl), P(dbl)]),
         "shaw_fom_time_chunks": (i32, [vp, u64, u64, vp]),
+        "shaw_app_energy_weights": (i32, [vp, vp, vp]),
         "shaw_fom_complexity": (i32, [u64, u64, u64, u64, u64, i32, P(dbl), P(dbl)]),
         "shaw_fom_bytes_per_step": (i32, [vp, P(dbl), P(dbl)]),
         "shaw_fom_config": (i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i32), P(i32), P(i32)]),
@@ -257,6 +258,12 @@ class App:
         c = np.zeros((n, 2))
         _check(lib().shaw_app_coords(self._h, 0 if dof == "vp" else 1, _p(c)))
         return c
+
+    def energy_weights(self):
+        """(Mv [nVp], Ms [nSp]): the FOM energy weights (shaw_app_energy_weights)."""
+        mv, ms = np.zeros(self.nVp), np.zeros(self.nSp)
+        _check(lib().shaw_app_energy_weights(self.handle, _p(mv), _p(ms)))
+        return mv, ms
 
     def map_source(self, depth_km: float, angle_deg: float) -> int:
         g = C.c_uint64()
@@ -485,6 +492,10 @@ class Rom:
         _check(lib().shaw_rom_create(C.byref(d), C.byref(hh)))
         self._h = hh
         self.every = {}
+
+    @property
+    def handle(self):
+        return self._h
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

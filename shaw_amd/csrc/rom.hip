@@ -234,27 +234,27 @@ struct Prec<0> {
   using Acc = dbl4;
 };
 
+// B operand of k-step group g. FP64: X is the state [K][16] in LDS. BF16:
+// X is the state's bf16 split (splitState): hi parts, then lo parts, each
+// [K/4][16] items of 4 bf16 values (k = 4 kq .. 4 kq + 3 of one column), so
+// a lane's operand of k-step s is one 8-byte read.
 template <int PARTS, int TPW>
-__device__ __forceinline__ void groupProducts(const typename Prec<PARTS>::Item (&ag)[TPW], int g, const double* X, int lane,
-                                              typename Prec<PARTS>::Acc (&acc)[TPW]) {
+__device__ __forceinline__ void groupProducts(const typename Prec<PARTS>::Item (&ag)[TPW], int g, const void* X,
+                                              int loOff, int lane, typename Prec<PARTS>::Acc (&acc)[TPW]) {
   if constexpr (PARTS == 0) {
+    const double* Xd = static_cast<const double*>(X);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const double xb = X[((4 * g + s) * 4 + lane / 16) * kTileCols + lane % 16];
+      const double xb = Xd[((4 * g + s) * 4 + lane / 16) * kTileCols + lane % 16];
 #pragma unroll
       for (int t = 0; t < TPW; ++t) acc[t] = mfmaF64(ag[t][s], xb, acc[t]);
     }
   } else {
+    const shrt4* Xh = static_cast<const shrt4*>(X);
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      shrt4 xh, xl;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const float x = (float)X[(16 * (2 * g + s) + 4 * (lane / 16) + e) * kTileCols + lane % 16];
-        const unsigned short h = f32ToBf16(x);
-        xh[e] = (short)h;
-        xl[e] = (short)f32ToBf16(x - bf16ToF32(h));
-      }
+      const int item = (4 * (2 * g + s) + lane / 16) * kTileCols + lane % 16;
+      const shrt4 xh = Xh[item], xl = Xh[loOff + item];
 #pragma unroll
       for (int t = 0; t < TPW; ++t) {
         const shrt8 w0 = __builtin_bit_cast(shrt8, ag[t].p[0]);
@@ -274,9 +274,28 @@ __device__ __forceinline__ void groupProducts(const typename Prec<PARTS>::Item (
   }
 }
 
+// the bf16 split of the state X [Kp][16] (FP64, LDS) into S: x = float(X),
+// hi = bf16(x), lo = bf16(x - hi), both rounded to nearest even; item
+// (k / 4) * 16 + c holds k % 4 = 0..3 of column c (layout of groupProducts)
+__device__ __forceinline__ void splitState(const double* X, int Kp, shrt4* S, int loOff, int tid, int nthreads) {
+  for (int e = tid; e < Kp * kTileCols / 4; e += nthreads) {
+    const int kq = e / kTileCols, c = e % kTileCols;
+    shrt4 h, l;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float x = (float)X[(4 * kq + r) * kTileCols + c];
+      const unsigned short hb = f32ToBf16(x);
+      h[r] = (short)hb;
+      l[r] = (short)f32ToBf16(x - bf16ToF32(hb));
+    }
+    S[e] = h;
+    S[loOff + e] = l;
+  }
+}
+
 template <int PARTS, int TPW>
 __device__ __forceinline__ void tileProducts(const void* __restrict__ Araw, int kGroups, const int (&rt)[TPW],
-                                             const double* X, int lane, typename Prec<PARTS>::Acc (&acc)[TPW]) {
+                                             const void* X, int loOff, int lane, typename Prec<PARTS>::Acc (&acc)[TPW]) {
   using Item = typename Prec<PARTS>::Item;
   // BF16x4 with 4 tiles per wave: a 2-deep ring keeps the registers within
   // the 256 of a 2-waves-per-SIMD workgroup (kGroups stays a multiple of it)
@@ -299,7 +318,7 @@ __device__ __forceinline__ void tileProducts(const void* __restrict__ Araw, int 
   for (; g0 + depth < kGroups; g0 += depth) {
 #pragma unroll
     for (int d = 0; d < depth; ++d) {
-      groupProducts<PARTS, TPW>(ring[d], g0 + d, X, lane, acc);
+      groupProducts<PARTS, TPW>(ring[d], g0 + d, X, loOff, lane, acc);
 #pragma unroll
       for (int t = 0; t < TPW; ++t) ring[d][t] = row[t][(g0 + d + depth) * 64];
       // keep this group's refill here: the scheduler would otherwise sink all
@@ -308,7 +327,7 @@ __device__ __forceinline__ void tileProducts(const void* __restrict__ Araw, int 
     }
   }
 #pragma unroll
-  for (int d = 0; d < depth; ++d) groupProducts<PARTS, TPW>(ring[d], g0 + d, X, lane, acc);
+  for (int d = 0; d < depth; ++d) groupProducts<PARTS, TPW>(ring[d], g0 + d, X, loOff, lane, acc);
 }
 
 __device__ __forceinline__ int accRow(int parts, int lane, int r) { return parts ? rowBf16(lane, r) : rowF64(lane, r); }
@@ -316,9 +335,9 @@ __device__ __forceinline__ int accRow(int parts, int lane, int r) { return parts
 // Y[rows][cols] += dt * (A X) (+ (f dt) b for the velocity half), active
 // columns only. Row tiles rt0 + t * kWaves, t < TPW, belong to this wave.
 template <int PARTS, int TPW>
-__device__ __forceinline__ void halfStep(const RomArgs& a, const void* A, int rowTiles, int kGroups, const double* Xs,
-                                         double* Ys, int wave, int lane, bool active, double fdt, const double* bcol,
-                                         int slot0) {
+__device__ __forceinline__ void halfStep(const RomArgs& a, const void* A, int rowTiles, int kGroups, const void* Xs,
+                                         int loOff, double* Ys, int wave, int lane, bool active, double fdt,
+                                         const double* bcol, int slot0) {
   const int col = lane % 16;
   for (int rt0 = wave; rt0 < rowTiles; rt0 += kWaves * TPW) {
     int rt[TPW];
@@ -331,7 +350,7 @@ __device__ __forceinline__ void halfStep(const RomArgs& a, const void* A, int ro
 #pragma unroll
         for (int r = 0; r < 4; ++r) bv[t][r] = bcol[(uint64_t)(slot0 + col) * a.Kvp + rt[t] * 16 + accRow(PARTS, lane, r)];
     typename Prec<PARTS>::Acc acc[TPW];
-    tileProducts<PARTS, TPW>(A, kGroups, rt, Xs, lane, acc);
+    tileProducts<PARTS, TPW>(A, kGroups, rt, Xs, loOff, lane, acc);
 #pragma unroll
     for (int t = 0; t < TPW; ++t)
       if (rt0 + t * kWaves < rowTiles && active)
@@ -346,7 +365,7 @@ __device__ __forceinline__ void halfStep(const RomArgs& a, const void* A, int ro
 }
 
 template <int PARTS, int TPW>
-__device__ __forceinline__ void receivers(const RomArgs& a, const double* Qv, int wave, int lane, bool sample,
+__device__ __forceinline__ void receivers(const RomArgs& a, const void* Xv, int loOff, int wave, int lane, bool sample,
                                           uint32_t sampleIdx, int slot0) {
   const int rowTiles = a.Rp / 16, kGroups = PARTS ? a.Kvp / 32 : a.Kvp / 16;
   for (int rt0 = wave; rt0 < rowTiles; rt0 += kWaves * TPW) {
@@ -354,7 +373,7 @@ __device__ __forceinline__ void receivers(const RomArgs& a, const double* Qv, in
 #pragma unroll
     for (int t = 0; t < TPW; ++t) rt[t] = rt0 + t * kWaves < rowTiles ? rt0 + t * kWaves : rt0;
     typename Prec<PARTS>::Acc acc[TPW];
-    tileProducts<PARTS, TPW>(a.c, kGroups, rt, Qv, lane, acc);
+    tileProducts<PARTS, TPW>(a.c, kGroups, rt, Xv, loOff, lane, acc);
     const int col = lane % 16;
 #pragma unroll
     for (int t = 0; t < TPW; ++t)
@@ -382,7 +401,9 @@ __global__ void __launch_bounds__(kWaves * 64) k_rom_steps(RomArgs a) {
   extern __shared__ double sm[];
   double* Qv = sm;                                   // [Kvp][16]
   double* Qs = Qv + (size_t)a.Kvp * kTileCols;       // [Ksp][16]
-  __shared__ uint32_t remain[kTileCols];
+  // BF16: the bf16 split of the half-step's input state (hi items, then lo)
+  shrt4* S = reinterpret_cast<shrt4*>(Qs + (size_t)a.Ksp * kTileCols);
+  const int loOff = (int)(a.Kvp > a.Ksp ? a.Kvp : a.Ksp) * kTileCols / 4;
   const int slot0 = blockIdx.x * kTileCols;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64, col = lane % 16;
   for (int e = tid; e < (int)a.Kvp * kTileCols; e += blockDim.x) {
@@ -398,11 +419,20 @@ __global__ void __launch_bounds__(kWaves * 64) k_rom_steps(RomArgs a) {
   const uint32_t hz = (uint32_t)a.horizon[slot0 + col];
   const uint32_t ev = (uint32_t)a.every[slot0 + col];
   const double* fcol = a.ftab + (uint64_t)(slot0 + col) * a.maxSteps;
-  if (tid < kTileCols) remain[tid] = hz > s0 ? hz - s0 : 0;
-  __syncthreads();
-  uint32_t steps = 0;  // block-uniform: steps until every column is done
-  for (int c = 0; c < kTileCols; ++c) steps = remain[c] > steps ? remain[c] : steps;
+  // block-uniform: steps until every column is done (lanes l and l + 16k hold
+  // the same column, so the wave's maximum is the maximum over the columns)
+  uint32_t steps = hz > s0 ? hz - s0 : 0;
+#pragma unroll
+  for (int off = 32; off >= 1; off /= 2) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)steps, off);
+    steps = o > steps ? o : steps;
+  }
   if ((uint64_t)steps > a.nsteps) steps = (uint32_t)a.nsteps;
+  __syncthreads();  // the states are in LDS
+  if constexpr (PARTS > 0) {
+    splitState(Qs, (int)a.Ksp, S, loOff, tid, blockDim.x);
+    __syncthreads();
+  }
   const int kvTiles = a.Kvp / 16, ksTiles = a.Ksp / 16;
   const int kGroupsS = PARTS ? a.Ksp / 32 : a.Ksp / 16, kGroupsV = PARTS ? a.Kvp / 32 : a.Kvp / 16;
   double fcur = s0 < hz ? fcol[s0] : 0.0;
@@ -417,12 +447,24 @@ __global__ void __launch_bounds__(kWaves * 64) k_rom_steps(RomArgs a) {
     // lanes 0..15 of every wave hold columns 0..15: the same answer in all waves
     const bool anySample = (__ballot(smp) & 0xffffull) != 0;
     // velocity: q_v += dt Ar_v q_s ; q_v += (f dt) b
-    halfStep<PARTS, TPW>(a, a.av, kvTiles, kGroupsS, Qs, Qv, wave, lane, act, fdt, a.b, slot0);
+    const void* Xs = PARTS ? static_cast<const void*>(S) : static_cast<const void*>(Qs);
+    halfStep<PARTS, TPW>(a, a.av, kvTiles, kGroupsS, Xs, loOff, Qv, wave, lane, act, fdt, a.b, slot0);
     __syncthreads();
-    if (anySample) receivers<PARTS, TPW>(a, Qv, wave, lane, smp, nextIdx - 1, slot0);
+    if constexpr (PARTS > 0) {  // every read of the split of q_s is done
+      splitState(Qv, (int)a.Kvp, S, loOff, tid, blockDim.x);
+      __syncthreads();
+    }
+    const void* Xv = PARTS ? static_cast<const void*>(S) : static_cast<const void*>(Qv);
+    if (anySample) receivers<PARTS, TPW>(a, Xv, loOff, wave, lane, smp, nextIdx - 1, slot0);
     // stress: q_s += dt Ar_s q_v
-    halfStep<PARTS, TPW>(a, a.as, ksTiles, kGroupsV, Qv, Qs, wave, lane, act, 0.0, nullptr, slot0);
+    halfStep<PARTS, TPW>(a, a.as, ksTiles, kGroupsV, Xv, loOff, Qs, wave, lane, act, 0.0, nullptr, slot0);
     __syncthreads();
+    if constexpr (PARTS > 0) {
+      if (it + 1 < steps) {
+        splitState(Qs, (int)a.Ksp, S, loOff, tid, blockDim.x);
+        __syncthreads();
+      }
+    }
     fcur = fnext;
     if (ev && act) {
       phase = phase + 1 == ev ? 0 : phase + 1;
@@ -713,6 +755,12 @@ struct shaw_rom {
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
   }
+  // LDS of one workgroup: the FP64 states, plus for BF16 the hi/lo split of
+  // the larger state (4 bytes an element)
+  size_t ldsBytes() const {
+    const size_t st = (size_t)(kvp + ksp) * shaw::kTileCols * sizeof(double);
+    return precision == SHAW_ROM_FP64 ? st : st + (size_t)std::max(kvp, ksp) * shaw::kTileCols * 4;
+  }
   void launch(uint64_t nsteps) {
     using namespace shaw;
     RomArgs a{};
@@ -724,7 +772,7 @@ struct shaw_rom {
     a.Kvp = kvp; a.Ksp = ksp; a.Rp = rp; a.R = (uint32_t)R;
     a.maxSteps = maxSteps; a.Tcap = maxSamples; a.nsteps = nsteps; a.dt = dt;
     const dim3 grid((unsigned)(maxCols / kTileCols));
-    const size_t lds = (size_t)(kvp + ksp) * kTileCols * sizeof(double);
+    const size_t lds = ldsBytes();
     if (precision == SHAW_ROM_FP64) launchRomSteps<0>(tpw, grid, lds, stream, a);
     else if (precision == SHAW_ROM_BF16X3) launchRomSteps<2>(tpw, grid, lds, stream, a);
     else launchRomSteps<3>(tpw, grid, lds, stream, a);
@@ -811,8 +859,7 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
     h->maxSamples = d->max_samples;
     SHAW_REQUIRE(h->kvp <= 512 && h->ksp <= 512, SHAW_ERR_ARG, "ROM size must be at most 512 modes per dof");
     h->tpw = (int)((std::max(h->kvp, h->ksp) / 16 + kWaves - 1) / kWaves);
-    const size_t lds = (size_t)(h->kvp + h->ksp) * kTileCols * sizeof(double);
-    SHAW_REQUIRE(lds <= 160 * 1024, SHAW_ERR_ARG, "ROM size too large for the LDS-resident state tile");
+    SHAW_REQUIRE(h->ldsBytes() <= 160 * 1024, SHAW_ERR_ARG, "ROM size too large for the LDS-resident state tile");
     SHAW_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     SHAW_HIP(hipEventCreate(&h->ev0));
     SHAW_HIP(hipEventCreate(&h->ev1));
@@ -977,6 +1024,18 @@ int shaw_rom_retire(shaw_rom_t h, uint64_t slot, double* seis, double* qv, doubl
     h->horizon.upload(&h->hHorizon[slot], 1, slot);
     h->stepCol.upload(&h->hStep[slot], 1, slot);
     h->every.upload(&h->hEvery[slot], 1, slot);
+  });
+}
+
+int shaw_rom_info(shaw_rom_t h, uint64_t* kv, uint64_t* ks, uint64_t* r, uint64_t* maxCols, uint64_t* maxSamples) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle");
+    if (kv) *kv = h->kv;
+    if (ks) *ks = h->ks;
+    if (r) *r = h->R;
+    if (maxCols) *maxCols = h->maxCols;
+    if (maxSamples) *maxSamples = h->maxSamples;
   });
 }
 

@@ -95,6 +95,22 @@ def fom_seismograms(app: shaw_amd.App, sources, dt: float, nsteps: int, receiver
     return out["seismogram"]
 
 
+def amplification_radius(ar_v: np.ndarray, ar_s: np.ndarray, dt: float) -> float:
+    """Spectral radius of the reduced leapfrog's one-step matrix. A step
+    (rom_run_rank_one.hpp:112-151) maps (q_v, q_s) to
+    q_v' = q_v + dt Ar_v q_s and q_s' = q_s + dt Ar_s q_v', i.e. by
+    [[I, dt Ar_v], [dt Ar_s, I + dt^2 Ar_s Ar_v]]; a radius above 1 means
+    homogeneous solutions grow like radius^steps (instability), at most 1
+    means they stay bounded."""
+    kv, ks = ar_v.shape
+    A = np.zeros((kv + ks, kv + ks))
+    A[:kv, :kv] = np.eye(kv)
+    A[:kv, kv:] = dt * ar_v
+    A[kv:, :kv] = dt * ar_s
+    A[kv:, kv:] = np.eye(ks) + dt * dt * (ar_s @ ar_v)
+    return float(np.abs(np.linalg.eigvals(A)).max())
+
+
 def rel_l2(x, ref) -> float:
     n = np.linalg.norm(ref)
     return float(np.linalg.norm(x - ref) / n) if n > 0 else float(np.linalg.norm(x))
@@ -112,56 +128,9 @@ def rel_l2(x, ref) -> float:
 # Galerkin ROM (rom_compute_jacobians.hpp:58-109) is the Euclidean one.
 
 def energy_weights(app: shaw_amd.App):
-    """Diagonal energy weights (Mv [nVp], Ms [nSp]) of the FOM operators, from
-    the ratios Ms[q] / Mv[p] = -J_vp[p, q] / J_sp[q, p] of the unmerged
-    couplings, chained along row 1 and up every column (exact products of the
-    coefficients). Points whose couplings are all merged (rows 0 and nr-1,
-    the two axis columns) take the weight of the adjacent point, the
-    stress points the ratio through an unmerged coupling where one exists."""
-    cv, cs, _, _ = app.operators()
-    nr, nth = app.nr, app.nth
-    rs = 2 * nth - 1
-    cN = cv[:, 0].reshape(nr, nth); cS = cv[:, 1].reshape(nr, nth)
-    cW = cv[:, 2].reshape(nr, nth); cE = cv[:, 3].reshape(nr, nth)
-    # stress coefficient rows: srp(i, j) = j*rs + 2i, stp(i, j) = j*rs + 2i + 1 (j <= nr-2),
-    # stp(i, nr-1) = (nr-1)*rs + i
-    body = cs[:(nr - 1) * rs].reshape(nr - 1, rs, 2)
-    srp0, srp1 = body[:, 0::2, 0], body[:, 0::2, 1]          # [nr-1, nth]: north v, south v
-    stp0 = np.vstack([body[:, 1::2, 0], cs[(nr - 1) * rs:, 0][None, :]])  # [nr, nth-1]: west v
-    stp1 = np.vstack([body[:, 1::2, 1], cs[(nr - 1) * rs:, 1][None, :]])  # east v
-    with np.errstate(divide="ignore", invalid="ignore"):
-        # Mv[j+1]/Mv[j] through srp(i, j): (Ms/Mv[j]) / (Ms/Mv[j+1]) = (-cN[j]/srp1) / (-cS[j+1]/srp0)
-        up = (cN[:-1, :] / srp1) / (cS[1:, :] / srp0)         # valid for 1 <= j <= nr-3
-        # Mv[i+1]/Mv[i] through stp(i, j): (-cE[i]/stp0) / (-cW[i+1]/stp1)
-        east = (cE[:, :-1] / stp0) / (cW[:, 1:] / stp1)       # valid for 1 <= i <= nth-3
-    lv = np.zeros((nr, nth))
-    lv[1, 2:nth - 1] = np.cumsum(np.log(east[1, 1:nth - 2]))
-    lv[2:nr - 1, 1:nth - 1] = lv[1, 1:nth - 1][None, :] + np.cumsum(np.log(up[1:nr - 2, 1:nth - 1]), axis=0)
-    lv -= lv[nr // 2, nth // 2]
-    mv = np.exp(lv)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        # radial stress from the unmerged side: the south point for rows j >= 1, the north one for row 0
-        srp = np.where(np.arange(nr - 1)[:, None] >= 1, -mv[:-1, :] * cN[:-1, :] / srp1, -mv[1:, :] * cS[1:, :] / srp0)
-        # CMB and surface rows: N and S of a point there are one stress point, so its merged
-        # coefficient (in the N slot) and that stress point's coefficient fix the weight
-        mv[0, :] = -srp[0, :] * srp1[0, :] / cN[0, :]
-        mv[nr - 1, :] = -srp[nr - 2, :] * srp0[nr - 2, :] / cN[nr - 1, :]
-        # axis points: all coefficients vanish (the field is zero there); take the neighbour's weight
-        mv[:, 0] = mv[:, 1]; mv[:, nth - 1] = mv[:, nth - 2]
-        srp[:, 0] = srp[:, 1]; srp[:, nth - 1] = srp[:, nth - 2]
-        # angular stress from the point off the axis
-        stp = np.where(np.arange(nth - 1)[None, :] >= 1, -mv[:, :-1] * cE[:, :-1] / stp0, -mv[:, 1:] * cW[:, 1:] / stp1)
-    ms = np.empty(app.nSp)
-    msb = np.empty((nr - 1, rs))
-    msb[:, 0::2] = srp
-    msb[:, 1::2] = stp[:nr - 1, :]
-    ms[:(nr - 1) * rs] = msb.ravel()
-    ms[(nr - 1) * rs:] = stp[nr - 1, :]
-    mvf = mv.ravel()
-    bad = int((~np.isfinite(ms) | (ms <= 0)).sum() + (~np.isfinite(mvf) | (mvf <= 0)).sum())
-    if bad:
-        raise RuntimeError(f"energy weights: {bad} points without a usable coupling")
-    return mvf, ms
+    """Diagonal energy weights (Mv [nVp], Ms [nSp]) of the FOM operators
+    (shaw_app_energy_weights, host C++)."""
+    return app.energy_weights()
 
 
 def energy_forcing_inputs(app: shaw_amd.App, phi_v: np.ndarray, mv: np.ndarray, sources, dt: float, nsteps: int):

@@ -1,37 +1,72 @@
-"""Continuous-batching ROM server on one GPU.
+"""Continuous-batching ROM serving on one GPU: Python glue over shaw_serve.
 
-Requests carry m forcing realizations (columns), a ROM size K, a horizon and
-a seismogram sampling period. Each K has one resident engine (reduced
-operators, receiver rows, column slots with their states, forcing series and
-seismogram buffers in HBM: shaw_rom_*). The loop
-
-  1. moves arrived requests to a queue,
-  2. admits queued requests while their K's engine has free slots and the
-     resident footprint stays within the HBM budget (requests admitted while
-     others are mid-flight join the running batch: continuous batching).
-     Policy "fifo" (the default) admits in arrival order and does not skip a
-     request that does not fit; policy "sjf" admits, among the queued
-     requests that fit, the one with the least work (horizon x columns)
-     first, which lowers the median latency under load at the cost of the
-     largest requests,
-  3. advances every engine that has active columns by one chunk of steps --
-     each engine on its own HIP stream, all launched before any wait, so the
-     engines' work overlaps on the device -- and
-  4. retires requests whose columns all reached their horizon, recording
-     latency = completion time - arrival time.
-
-A column's arithmetic does not depend on the other columns in the batch, so
-a request's result is the same alone and in the mix (tested bit for bit).
-The engine interface is small (admit / advance / sync / retire / free_slots),
-so the scheduler is tested on the CPU with a numpy engine.
+The request loop itself (admission by free slots and HBM footprint in FIFO or
+least-work-first order with a wait bound, chunked advances of every engine
+launched before any wait, batched retirement, latency accounting) is C++
+behind the C ABI (shaw_amd/csrc/serve.cpp, shaw_serve in
+include/shaw_amd.h). This module builds its inputs: request objects, the
+engine tables (a resident shaw_rom engine on the GPU, or the numpy engine
+below through ctypes callbacks, so the loop is tested on the CPU), the
+Poisson arrivals of the benchmark, and the report.
 """
 from __future__ import annotations
 
-import collections
-import time
+import ctypes as C
 from dataclasses import dataclass, field
 
 import numpy as np
+
+import shaw_amd
+
+FIFO, LEAST_WORK = 0, 1
+
+_ADMIT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_double),
+                     C.POINTER(C.c_uint64), C.c_uint64, C.POINTER(C.c_uint64))
+_ADVANCE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64)
+_SYNC = C.CFUNCTYPE(C.c_int, C.c_void_p)
+_RETIRE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64), C.c_uint64, C.POINTER(C.c_double),
+                      C.POINTER(C.c_uint64))
+_FREE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint64))
+_CLOCK = C.CFUNCTYPE(C.c_double, C.c_void_p)
+_WAIT = C.CFUNCTYPE(None, C.c_void_p, C.c_double)
+
+
+class ShawEngine(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("kv", C.c_uint64), ("ks", C.c_uint64), ("r", C.c_uint64),
+                ("capacity", C.c_uint64), ("admit", _ADMIT), ("advance", _ADVANCE), ("sync", _SYNC),
+                ("retire_batch", _RETIRE), ("free_slots", _FREE)]
+
+
+class ShawRequest(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("engine", C.c_int), ("m", C.c_uint64), ("horizon", C.c_uint64),
+                ("every", C.c_uint64), ("arrival_s", C.c_double), ("b", C.POINTER(C.c_double)),
+                ("ftab", C.POINTER(C.c_double)), ("seis", C.POINTER(C.c_double)), ("admitted_s", C.c_double),
+                ("done_s", C.c_double)]
+
+
+class ShawServeOptions(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("chunk_steps", C.c_uint64), ("policy", C.c_int),
+                ("max_wait_s", C.c_double), ("wall_limit_s", C.c_double), ("clock", _CLOCK),
+                ("wait_until", _WAIT), ("clock_ctx", C.c_void_p)]
+
+
+class ShawServeStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("forcing_steps", C.c_uint64), ("launches", C.c_uint64),
+                ("max_queue", C.c_uint64), ("span_s", C.c_double), ("p50_latency_s", C.c_double),
+                ("p99_latency_s", C.c_double), ("max_latency_s", C.c_double), ("max_wait_s", C.c_double),
+                ("throughput", C.c_double)]
+
+
+def _bind():
+    L = shaw_amd.lib()
+    if not getattr(L, "_serve_bound", False):
+        L.shaw_engine_from_rom.restype = C.c_int
+        L.shaw_engine_from_rom.argtypes = [C.c_void_p, C.POINTER(ShawEngine)]
+        L.shaw_serve.restype = C.c_int
+        L.shaw_serve.argtypes = [C.POINTER(ShawEngine), C.c_int, C.POINTER(ShawServeOptions),
+                                 C.POINTER(ShawRequest), C.c_uint64, C.POINTER(ShawServeStats)]
+        L._serve_bound = True
+    return L
 
 
 @dataclass
@@ -42,10 +77,9 @@ class Request:
     ftab: np.ndarray         # [horizon x m]
     horizon: int
     every: int
-    arrival: float           # seconds, on the server clock
+    arrival: float           # seconds after the start of the run
     admitted: float = -1.0
     done: float = -1.0
-    slots: list = field(default_factory=list)
     seismograms: list = field(default_factory=list)  # per column [T x R]
 
     @property
@@ -59,136 +93,72 @@ class Request:
 
 def column_bytes(kv: int, ks: int, horizon: int, every: int, r: int) -> int:
     """HBM bytes one resident column holds: states, forcing projection,
-    forcing series and seismogram buffer (FP64)."""
+    forcing series and seismogram buffer (FP64); the footprint the loop
+    admits by."""
     samples = horizon // every if every else 0
     return 8 * (kv + ks + kv + horizon + r * samples)
 
 
 class Server:
-    def __init__(self, engines: dict, hbm_budget: float, chunk: int = 500, clock=time.perf_counter,
-                 policy: str = "fifo"):
-        """engines: K -> engine with .admit(b, ftab, horizons, every) -> slots,
-        .advance(n), .sync(), .retire_many(slots, max_samples) -> [seis],
-        .free_slots(), .kv, .ks, .R. policy: "fifo" or "sjf"."""
-        if policy not in ("fifo", "sjf"):
+    """One serving run over resident engines (K -> engine)."""
+
+    def __init__(self, engines: dict, hbm_budget: float, chunk: int = 500, policy: str = "fifo",
+                 max_wait: float = 0.0, clock=None):
+        """policy: "fifo" or "least_work" ("sjf"); max_wait [s]: least-work
+        admission serves a request that waited this long before any other
+        (0: no bound). clock: None (steady clock) or an object with
+        now() -> s and wait_until(t) for a simulated clock."""
+        pol = {"fifo": FIFO, "least_work": LEAST_WORK, "sjf": LEAST_WORK}
+        if policy not in pol:
             raise ValueError(f"unknown admission policy {policy!r}")
-        self.policy = policy
-        self.engines = engines
-        self.budget = hbm_budget
-        self.chunk = chunk
-        self.clock = clock
-        self.t0 = clock()
-        self.queue: collections.deque = collections.deque()
-        self.resident: list = []
-        self.finished: list = []
-        self.resident_bytes = 0
-        self.forcing_steps = 0
-        self.launches = 0
+        self.engines, self.budget, self.chunk = engines, hbm_budget, chunk
+        self.policy, self.max_wait, self.clock = pol[policy], max_wait, clock
+        self.stats = None
 
-    def now(self) -> float:
-        return self.clock() - self.t0
-
-    def footprint(self, req: Request) -> int:
-        e = self.engines[req.k]
-        return req.m * column_bytes(e.kv, e.ks, req.horizon, req.every, e.R)
-
-    def submit(self, req: Request) -> None:
-        if req.k not in self.engines:
-            raise ValueError(f"no resident ROM of size {req.k}")
-        e = self.engines[req.k]
-        if req.m > e.capacity:
-            raise ValueError(f"request {req.rid}: {req.m} columns exceed the engine capacity {e.capacity}")
-        if self.footprint(req) > self.budget:
-            raise ValueError(f"request {req.rid}: footprint exceeds the HBM budget")
-        self.queue.append(req)
-
-    def _fits(self, req: Request) -> bool:
-        return (self.engines[req.k].free_slots() >= req.m and
-                self.resident_bytes + self.footprint(req) <= self.budget)
-
-    def _admit(self) -> None:
-        while self.queue:
-            if self.policy == "fifo":
-                req = self.queue[0]
-                if not self._fits(req):
-                    break  # FIFO: wait for slots / memory to free up
-            else:  # least work first among the requests that fit (ties: arrival order)
-                fit = [r for r in self.queue if self._fits(r)]
-                if not fit:
-                    break
-                req = min(fit, key=lambda r: r.horizon * r.m)
-            self.queue.remove(req)
-            e = self.engines[req.k]
-            fp = self.footprint(req)
-            req.slots = e.admit(req.b, req.ftab, [req.horizon] * req.m, req.every)
-            req.admitted = self.now()
-            req.steps_left = req.horizon
-            self.resident_bytes += fp
-            self.resident.append(req)
-
-    def _advance(self) -> None:
-        active = {}
-        for req in self.resident:
-            active[req.k] = max(active.get(req.k, 0), req.steps_left)
-        for k, left in active.items():  # launch all engines before waiting on any
-            self.engines[k].advance(min(self.chunk, left))
-            self.launches += 1
-        for k in active:
-            self.engines[k].sync()
-        for req in self.resident:
-            n = min(self.chunk, active[req.k])
-            taken = min(n, req.steps_left)
-            req.steps_left -= taken
-            self.forcing_steps += taken * req.m
-
-    def _retire(self) -> None:
-        keep = []
-        for req in self.resident:
-            if req.steps_left > 0:
-                keep.append(req)
-                continue
-            e = self.engines[req.k]
-            T = req.horizon // req.every if req.every else 0
-            req.seismograms = e.retire_many(req.slots, T)
-            req.done = self.now()
-            self.resident_bytes -= self.footprint(req)
-            self.finished.append(req)
-        self.resident = keep
-
-    def step(self) -> None:
-        self._admit()
-        if self.resident:
-            self._advance()
-            self._retire()
-
-    def run(self, requests: list, wall_limit: float = 600.0) -> dict:
-        """Serve requests arriving at their `arrival` times (server clock)."""
-        pending = sorted(requests, key=lambda r: r.arrival)
-        i = 0
-        self.t0 = self.clock()
-        while i < len(pending) or self.queue or self.resident:
-            now = self.now()
-            while i < len(pending) and pending[i].arrival <= now:
-                self.submit(pending[i])
-                i += 1
-            if not self.queue and not self.resident:
-                if i < len(pending):
-                    time.sleep(max(0.0, min(0.01, pending[i].arrival - now)))
-                continue
-            self.step()
-            if now > wall_limit:
-                raise RuntimeError("serving run exceeded its wall-clock limit")
-        return self.report()
-
-    def report(self) -> dict:
-        lat = np.array([r.latency for r in self.finished])
-        span = max(r.done for r in self.finished) - min(r.arrival for r in self.finished) if self.finished else 0.0
-        return {"requests": len(self.finished),
-                "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
-                "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
-                "forcing_steps": int(self.forcing_steps),
-                "throughput_forcing_steps_per_s": self.forcing_steps / span if span > 0 else None,
-                "wall_s": span, "launches": self.launches}
+    def run(self, requests: list, wall_limit: float = 600.0, keep=None) -> dict:
+        """Serve the requests; keep(r) -> bool selects the requests whose
+        seismograms are returned (default: all)."""
+        L = _bind()
+        keys = sorted(self.engines)
+        index = {k: i for i, k in enumerate(keys)}
+        for r in requests:
+            if r.k not in index:
+                raise ValueError(f"no resident ROM of size {r.k}")
+        tables = (ShawEngine * len(keys))(*[self.engines[k].table() for k in keys])
+        arr = (ShawRequest * max(1, len(requests)))()
+        alive = []  # arrays and callbacks the C loop points into
+        outs = []
+        for i, r in enumerate(requests):
+            e = self.engines[r.k]
+            b = np.ascontiguousarray(r.b, dtype=np.float64)
+            f = np.ascontiguousarray(np.asarray(r.ftab)[:r.horizon], dtype=np.float64)
+            T = r.horizon // r.every if r.every else 0
+            seis = np.zeros((r.m, T, e.R)) if (T and e.R and (keep is None or keep(r))) else None
+            alive += [b, f]
+            outs.append(seis)
+            q = arr[i]
+            q.id, q.engine, q.m, q.horizon, q.every, q.arrival_s = r.rid, index[r.k], r.m, r.horizon, r.every, r.arrival
+            q.b = b.ctypes.data_as(C.POINTER(C.c_double))
+            q.ftab = f.ctypes.data_as(C.POINTER(C.c_double))
+            q.seis = seis.ctypes.data_as(C.POINTER(C.c_double)) if seis is not None else None
+        opts = ShawServeOptions()
+        opts.hbm_budget, opts.chunk_steps, opts.policy = int(self.budget), self.chunk, self.policy
+        opts.max_wait_s, opts.wall_limit_s = self.max_wait, wall_limit
+        if self.clock is not None:
+            cb_clock = _CLOCK(lambda ctx: self.clock.now())
+            cb_wait = _WAIT(lambda ctx, t: self.clock.wait_until(t))
+            opts.clock, opts.wait_until = cb_clock, cb_wait
+            alive += [cb_clock, cb_wait]
+        st = ShawServeStats()
+        shaw_amd._check(L.shaw_serve(tables, len(keys), C.byref(opts), arr, len(requests), C.byref(st)))
+        for i, r in enumerate(requests):
+            r.admitted, r.done = arr[i].admitted_s, arr[i].done_s
+            r.seismograms = list(outs[i]) if outs[i] is not None else []
+        self.stats = st
+        return {"requests": int(st.requests), "p50_latency_s": st.p50_latency_s, "p99_latency_s": st.p99_latency_s,
+                "max_latency_s": st.max_latency_s, "max_wait_s": st.max_wait_s,
+                "forcing_steps": int(st.forcing_steps), "throughput_forcing_steps_per_s": st.throughput,
+                "wall_s": st.span_s, "launches": int(st.launches), "max_queue": int(st.max_queue)}
 
 
 def poisson_requests(rng, rate: float, duration: float, k_choices, m_range, horizon_range, make_inputs,
@@ -209,43 +179,35 @@ def poisson_requests(rng, rate: float, duration: float, k_choices, m_range, hori
 
 
 class GpuEngine:
-    """shaw_amd.Rom with the attributes the server needs."""
+    """A resident shaw_amd.Rom served through its own entry points."""
 
     def __init__(self, rom, capacity: int):
         self.rom = rom
         self.kv, self.ks, self.R = rom.kv, rom.ks, rom.R
         self.capacity = capacity
 
-    def admit(self, b, ftab, horizons, every):
-        return self.rom.admit(b, ftab, horizons, every)
-
-    def advance(self, n):
-        self.rom.advance(n)
-
-    def sync(self):
-        self.rom.sync()
-
-    def retire(self, slot, max_samples):
-        return self.rom.retire(slot, max_samples)
-
-    def retire_many(self, slots, max_samples):
-        return self.rom.retire_many(slots, max_samples)
-
-    def free_slots(self):
-        return self.rom.free_slots()
+    def table(self) -> ShawEngine:
+        t = ShawEngine()
+        shaw_amd._check(_bind().shaw_engine_from_rom(self.rom.handle, C.byref(t)))
+        return t
 
 
 class NumpyEngine:
-    """CPU engine with the same contract as GpuEngine (FP64 reduced leapfrog
-    in numpy), for testing the scheduler without a GPU."""
+    """CPU engine with the engine contract (FP64 reduced leapfrog in numpy,
+    the step of rom_run_rank_one.hpp:112-151 per column), for testing the
+    serving loop without a GPU. `tick(steps)` is called on every advance so a
+    simulated clock can charge time for the work."""
 
-    def __init__(self, ar_v, ar_s, dt, C, capacity):
-        self.ar_v, self.ar_s, self.dt, self.C = ar_v, ar_s, dt, C
+    def __init__(self, ar_v, ar_s, dt, C_, capacity, tick=None):
+        self.ar_v, self.ar_s, self.dt, self.C = ar_v, ar_s, dt, C_
         self.kv, self.ks = ar_v.shape
-        self.R = 0 if C is None else C.shape[0]
+        self.R = 0 if C_ is None else C_.shape[0]
         self.capacity = capacity
         self.cols = {}
+        self.tick = tick
+        self.calls = []
 
+    # ---- the engine contract, in numpy
     def free_slots(self):
         return self.capacity - len(self.cols)
 
@@ -260,6 +222,8 @@ class NumpyEngine:
         return slots
 
     def advance(self, n):
+        if self.tick:
+            self.tick(n)
         for c in self.cols.values():
             for _ in range(n):
                 if c["step"] >= c["h"]:
@@ -272,13 +236,72 @@ class NumpyEngine:
                     c["seis"].append(self.C @ c["qv"])
                 c["qs"] = c["qs"] + self.dt * (self.ar_s @ c["qv"])
 
-    def sync(self):
-        pass
-
-    def retire(self, slot, max_samples):
+    def retire(self, slot):
         c = self.cols.pop(slot)
-        seis = np.array(c["seis"]).reshape(-1, self.R) if self.R else np.zeros((0, 0))
-        return seis, c["qv"], c["qs"]
+        return np.array(c["seis"]).reshape(-1, self.R) if self.R else np.zeros((0, 0))
 
-    def retire_many(self, slots, max_samples):
-        return [self.retire(s, max_samples)[0] for s in slots]
+    # ---- the same contract as a C table of ctypes callbacks
+    def table(self) -> ShawEngine:
+        def guard(fn):
+            def call(*a):
+                try:
+                    return fn(*a)
+                except Exception as exc:  # reported as an engine failure
+                    self.calls.append(("error", repr(exc)))
+                    return 1
+            return call
+
+        def admit(ctx, n, b, nsteps, ftab, horizons, every, slots):
+            bb = np.ctypeslib.as_array(b, (self.kv, n)).copy()
+            ff = np.ctypeslib.as_array(ftab, (nsteps, n)).copy()
+            hz = [int(horizons[j]) for j in range(n)]
+            for j, s in enumerate(self.admit(bb, ff, hz, int(every))):
+                slots[j] = s
+            self.calls.append(("admit", int(n)))
+            return 0
+
+        def advance(ctx, n):
+            self.advance(int(n))
+            self.calls.append(("advance", int(n)))
+            return 0
+
+        def retire(ctx, n, slots, max_samples, seis, samples):
+            width = self.R * int(max_samples)
+            for j in range(n):
+                y = self.retire(int(slots[j]))
+                if width:
+                    out = np.ctypeslib.as_array(seis, (int(n) * width,))
+                    out[j * width:j * width + y.size] = y.ravel()
+                if samples:
+                    samples[j] = len(y)
+            return 0
+
+        def free(ctx, out):
+            out[0] = self.free_slots()
+            return 0
+
+        t = ShawEngine()
+        t.kv, t.ks, t.r, t.capacity = self.kv, self.ks, self.R, self.capacity
+        t.admit, t.advance, t.sync = _ADMIT(guard(admit)), _ADVANCE(guard(advance)), _SYNC(lambda ctx: 0)
+        t.retire_batch, t.free_slots = _RETIRE(guard(retire)), _FREE(guard(free))
+        self._keep = (t.admit, t.advance, t.sync, t.retire_batch, t.free_slots)
+        return t
+
+
+class SimClock:
+    """A simulated clock for CPU tests: every reading costs `per_call`
+    seconds, engine work costs `per_step` seconds a step, and idle waits jump
+    to the next arrival."""
+
+    def __init__(self, per_call=1e-5, per_step=1e-4):
+        self.t, self.per_call, self.per_step = 0.0, per_call, per_step
+
+    def now(self) -> float:
+        self.t += self.per_call
+        return self.t
+
+    def wait_until(self, t: float) -> None:
+        self.t = max(self.t, t)
+
+    def work(self, steps: int) -> None:
+        self.t += self.per_step * steps

@@ -27,6 +27,69 @@ def csr(ora, dof, shape):
     return sp.csr_matrix((val, ind.astype(np.int64), ptr.astype(np.int64)), shape=shape)
 
 
+def numpy_energy_weights(app: shaw_amd.App):
+    """A numpy restatement of the energy weights (the checker of the C++
+    shaw_app_energy_weights): (Mv [nVp], Ms [nSp]) of the FOM operators, from
+    the ratios Ms[q] / Mv[p] = -J_vp[p, q] / J_sp[q, p] of the unmerged
+    couplings, chained along row 1 and up every column (exact products of the
+    coefficients). Points whose couplings are all merged (rows 0 and nr-1,
+    the two axis columns) take the weight of the adjacent point, the
+    stress points the ratio through an unmerged coupling where one exists."""
+    cv, cs, _, _ = app.operators()
+    nr, nth = app.nr, app.nth
+    rs = 2 * nth - 1
+    cN = cv[:, 0].reshape(nr, nth); cS = cv[:, 1].reshape(nr, nth)
+    cW = cv[:, 2].reshape(nr, nth); cE = cv[:, 3].reshape(nr, nth)
+    # stress coefficient rows: srp(i, j) = j*rs + 2i, stp(i, j) = j*rs + 2i + 1 (j <= nr-2),
+    # stp(i, nr-1) = (nr-1)*rs + i
+    body = cs[:(nr - 1) * rs].reshape(nr - 1, rs, 2)
+    srp0, srp1 = body[:, 0::2, 0], body[:, 0::2, 1]          # [nr-1, nth]: north v, south v
+    stp0 = np.vstack([body[:, 1::2, 0], cs[(nr - 1) * rs:, 0][None, :]])  # [nr, nth-1]: west v
+    stp1 = np.vstack([body[:, 1::2, 1], cs[(nr - 1) * rs:, 1][None, :]])  # east v
+    with np.errstate(divide="ignore", invalid="ignore"):
+        # Mv[j+1]/Mv[j] through srp(i, j): (Ms/Mv[j]) / (Ms/Mv[j+1]) = (-cN[j]/srp1) / (-cS[j+1]/srp0)
+        up = (cN[:-1, :] / srp1) / (cS[1:, :] / srp0)         # valid for 1 <= j <= nr-3
+        # Mv[i+1]/Mv[i] through stp(i, j): (-cE[i]/stp0) / (-cW[i+1]/stp1)
+        east = (cE[:, :-1] / stp0) / (cW[:, 1:] / stp1)       # valid for 1 <= i <= nth-3
+    lv = np.zeros((nr, nth))
+    lv[1, 2:nth - 1] = np.cumsum(np.log(east[1, 1:nth - 2]))
+    lv[2:nr - 1, 1:nth - 1] = lv[1, 1:nth - 1][None, :] + np.cumsum(np.log(up[1:nr - 2, 1:nth - 1]), axis=0)
+    lv -= lv[nr // 2, nth // 2]
+    mv = np.exp(lv)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        # radial stress from the unmerged side: the south point for rows j >= 1, the north one for row 0
+        srp = np.where(np.arange(nr - 1)[:, None] >= 1, -mv[:-1, :] * cN[:-1, :] / srp1, -mv[1:, :] * cS[1:, :] / srp0)
+        # CMB and surface rows: N and S of a point there are one stress point, so its merged
+        # coefficient (in the N slot) and that stress point's coefficient fix the weight
+        mv[0, :] = -srp[0, :] * srp1[0, :] / cN[0, :]
+        mv[nr - 1, :] = -srp[nr - 2, :] * srp0[nr - 2, :] / cN[nr - 1, :]
+        # axis points: all coefficients vanish (the field is zero there); take the neighbour's weight
+        mv[:, 0] = mv[:, 1]; mv[:, nth - 1] = mv[:, nth - 2]
+        srp[:, 0] = srp[:, 1]; srp[:, nth - 1] = srp[:, nth - 2]
+        # angular stress from the point off the axis
+        stp = np.where(np.arange(nth - 1)[None, :] >= 1, -mv[:, :-1] * cE[:, :-1] / stp0, -mv[:, 1:] * cW[:, 1:] / stp1)
+    ms = np.empty(app.nSp)
+    msb = np.empty((nr - 1, rs))
+    msb[:, 0::2] = srp
+    msb[:, 1::2] = stp[:nr - 1, :]
+    ms[:(nr - 1) * rs] = msb.ravel()
+    ms[(nr - 1) * rs:] = stp[nr - 1, :]
+    mvf = mv.ravel()
+    bad = int((~np.isfinite(ms) | (ms <= 0)).sum() + (~np.isfinite(mvf) | (mvf <= 0)).sum())
+    if bad:
+        raise RuntimeError(f"energy weights: {bad} points without a usable coupling")
+    return mvf, ms
+
+
+@pytest.mark.parametrize("nr,nth,mat", [(21, 51, PREM), (40, 96, LAYERED), (200, 1000, LAYERED), (5, 7, BILAYER)])
+def test_energy_weights_cpp_equal_numpy(nr, nth, mat):
+    """shaw_app_energy_weights (C++) against the numpy restatement, within
+    1e-13 relative (the two use different log/exp implementations)."""
+    app = shaw_amd.App.structured(nr, nth, mat)
+    for got, want in zip(app.energy_weights(), numpy_energy_weights(app)):
+        np.testing.assert_allclose(got, want, rtol=1e-13)
+
+
 @pytest.mark.parametrize("nr,nth,mat", [(21, 51, PREM), (40, 96, LAYERED), (33, 70, BILAYER), (5, 7, LAYERED)])
 def test_energy_weights_make_operators_adjoint(nr, nth, mat):
     app = shaw_amd.App.structured(nr, nth, mat)
@@ -56,3 +119,15 @@ def test_energy_weights_from_mesh_files_match_structured():
     b = shaw_amd.App.structured(21, 51, PREM)
     for x, y in zip(rw.energy_weights(a), rw.energy_weights(b)):
         np.testing.assert_allclose(x, y, rtol=1e-13)
+
+
+@pytest.mark.parametrize("a,s,dt", [(1.0, -1.0, 0.1), (3.0, -0.5, 0.2), (1.0, 1.0, 0.1), (2.0, 0.3, 0.05)])
+def test_amplification_radius_closed_form(a, s, dt):
+    """1 x 1 operators: the one-step matrix [[1, dt a], [dt s, 1 + dt^2 s a]]
+    has determinant 1 and trace 2 + dt^2 s a, so its eigenvalues solve
+    l^2 - (2 + dt^2 s a) l + 1 = 0: on the unit circle when -4 < dt^2 s a < 0
+    (a stable leapfrog), real with radius (t + sqrt(t^2 - 4)) / 2 when s a > 0."""
+    t = 2.0 + dt * dt * s * a
+    want = 1.0 if t * t < 4.0 else (abs(t) + np.sqrt(t * t - 4.0)) / 2.0
+    got = rw.amplification_radius(np.array([[a]]), np.array([[s]]), dt)
+    assert got == pytest.approx(want, rel=1e-14)

@@ -1,10 +1,13 @@
-"""CPU tests of the continuous-batching ROM server with the numpy engine:
-FIFO admission gated by free slots and by the HBM footprint budget, requests
-joining a running batch, uneven horizons, latency accounting, rejection of
-requests that can never fit, and alone-vs-mixed equality of results."""
+"""CPU tests of the continuous-batching serving loop (shaw_serve, C++) driving
+numpy engines through the engine table, on a simulated clock: FIFO admission
+gated by free slots and by the HBM footprint budget, requests joining a
+running batch, uneven horizons, latency accounting, refusal of requests that
+can never fit, least-work-first admission, its wait bound under sustained
+overload, and alone-vs-mixed equality of results."""
 import numpy as np
 import pytest
 
+import shaw_amd
 from shaw_amd import serving
 
 
@@ -14,10 +17,10 @@ def stable_ops(k, seed):
     return A, -A.T.copy()  # skew pair: the reduced leapfrog stays bounded
 
 
-def make_engine(k, cap, seed=0, r=3):
+def make_engine(k, cap, seed=0, r=3, clock=None):
     arv, ars = stable_ops(k, seed)
     C = np.random.default_rng(seed + 1).standard_normal((r, k))
-    return serving.NumpyEngine(arv, ars, 0.5, C, cap)
+    return serving.NumpyEngine(arv, ars, 0.5, C, cap, tick=clock.work if clock else None)
 
 
 def req(rid, k, m, h, arrival, seed, every=5):
@@ -25,89 +28,95 @@ def req(rid, k, m, h, arrival, seed, every=5):
     return serving.Request(rid, k, rng.standard_normal((k, m)), rng.standard_normal((h, m)), h, every, arrival)
 
 
-class FakeClock:
-    def __init__(self):
-        self.t = 0.0
-
-    def __call__(self):
-        self.t += 0.001
-        return self.t
+def serve(engines, reqs, clock, **kw):
+    return serving.Server(engines, clock=clock, **kw).run(reqs)
 
 
 def test_results_alone_equal_results_in_mix():
-    engines = {8: make_engine(8, 16), 12: make_engine(12, 16, seed=3)}
+    clk = serving.SimClock()
+    engines = {8: make_engine(8, 16, clock=clk), 12: make_engine(12, 16, seed=3, clock=clk)}
     mix = [req(0, 8, 3, 40, 0.0, 10), req(1, 12, 2, 25, 0.0, 11), req(2, 8, 5, 60, 0.0, 12),
            req(3, 8, 4, 20, 0.0, 13)]
-    srv = serving.Server(engines, hbm_budget=1e12, chunk=7, clock=FakeClock())
-    rep = srv.run(mix)
+    rep = serve(engines, mix, clk, hbm_budget=1e12, chunk=7)
     assert rep["requests"] == 4
     for r in mix:
-        alone_engines = {8: make_engine(8, 16), 12: make_engine(12, 16, seed=3)}
+        c2 = serving.SimClock()
+        alone_engines = {8: make_engine(8, 16, clock=c2), 12: make_engine(12, 16, seed=3, clock=c2)}
         a = req(r.rid, r.k, r.m, r.horizon, 0.0, 10 + r.rid)
-        serving.Server(alone_engines, 1e12, chunk=7, clock=FakeClock()).run([a])
+        serve(alone_engines, [a], c2, hbm_budget=1e12, chunk=7)
+        assert len(a.seismograms) == r.m
         for x, y in zip(r.seismograms, a.seismograms):
             np.testing.assert_array_equal(x, y)
         assert all(s.shape == (r.horizon // r.every, 3) for s in r.seismograms)
 
 
+def test_seismograms_equal_the_engine_run_directly():
+    """The loop's seismograms are the engine's own, column by column."""
+    clk = serving.SimClock()
+    e = make_engine(8, 8, clock=clk)
+    r = req(0, 8, 3, 35, 0.0, 5)
+    serve({8: e}, [r], clk, hbm_budget=1e12, chunk=6)
+    d = make_engine(8, 8)
+    slots = d.admit(r.b, r.ftab, [r.horizon] * r.m, r.every)
+    d.advance(r.horizon)
+    for j, s in enumerate(slots):
+        np.testing.assert_array_equal(r.seismograms[j], d.retire(s))
+
+
 def test_admission_by_slots_is_fifo_and_joins_running_batch():
-    engines = {8: make_engine(8, 6)}
+    clk = serving.SimClock()
+    engines = {8: make_engine(8, 6, clock=clk)}
     reqs = [req(0, 8, 4, 30, 0.0, 1), req(1, 8, 4, 30, 0.0, 2), req(2, 8, 1, 30, 0.0, 3)]
-    srv = serving.Server(engines, hbm_budget=1e12, chunk=10, clock=FakeClock())
-    for r in reqs:
-        srv.submit(r)
-    srv.step()
-    # request 1 does not fit (4 + 4 > 6 slots); request 2 would, but FIFO keeps it queued
-    assert [r.rid for r in srv.resident] == [0]
-    assert [r.rid for r in srv.queue] == [1, 2]
-    while srv.resident or srv.queue:
-        srv.step()
-    assert [r.rid for r in srv.finished] == [0, 1, 2]
-    # 1 and 2 were admitted in the same scheduler step (one clock tick apart), after 0 finished
-    assert abs(reqs[1].admitted - reqs[2].admitted) < 0.0015 and reqs[1].admitted >= reqs[0].done
+    serve(engines, reqs, clk, hbm_budget=1e12, chunk=10)
+    a, b, c = reqs
+    # request 1 does not fit beside 0 (4 + 4 > 6 slots); request 2 would, but FIFO keeps it behind 1
+    assert b.admitted >= a.done and c.admitted >= a.done and c.admitted >= b.admitted
+    # 1 and 2 join in the same admission pass (no engine work between them)
+    assert c.admitted - b.admitted < 1e-4
+
+
+def test_late_arrival_joins_the_running_batch():
+    clk = serving.SimClock(per_step=1e-3)
+    engines = {8: make_engine(8, 16, clock=clk)}
+    reqs = [req(0, 8, 2, 100, 0.0, 1), req(1, 8, 2, 20, 0.015, 2)]
+    serve(engines, reqs, clk, hbm_budget=1e12, chunk=10)
+    a, b = reqs
+    assert a.admitted < b.admitted < a.done  # admitted while request 0 was mid-flight
+    assert b.done < a.done
 
 
 def test_admission_by_hbm_footprint():
-    e = make_engine(8, 64)
-    engines = {8: e}
+    clk = serving.SimClock()
+    e = make_engine(8, 64, clock=clk)
     one = serving.column_bytes(e.kv, e.ks, 30, 5, e.R)
-    srv = serving.Server(engines, hbm_budget=5 * one, chunk=10, clock=FakeClock())
     a, b = req(0, 8, 3, 30, 0.0, 1), req(1, 8, 3, 30, 0.0, 2)
-    srv.submit(a)
-    srv.submit(b)
-    srv.step()
-    assert len(srv.resident) == 1 and srv.resident_bytes == 3 * one
-    with pytest.raises(ValueError, match="footprint"):
-        srv.submit(req(2, 8, 6, 30, 0.0, 3))
-    with pytest.raises(ValueError, match="capacity"):
-        serving.Server({8: make_engine(8, 4)}, 1e12).submit(req(3, 8, 5, 30, 0.0, 4))
+    serve({8: e}, [a, b], clk, hbm_budget=5 * one, chunk=10)
+    assert b.admitted >= a.done  # 3 + 3 columns exceed the 5-column budget
+    with pytest.raises(shaw_amd.ShawError, match="footprint"):
+        serve({8: make_engine(8, 64)}, [req(2, 8, 6, 30, 0.0, 3)], serving.SimClock(), hbm_budget=5 * one)
+    with pytest.raises(shaw_amd.ShawError, match="capacity"):
+        serve({8: make_engine(8, 4)}, [req(3, 8, 5, 30, 0.0, 4)], serving.SimClock(), hbm_budget=1e12)
     with pytest.raises(ValueError, match="no resident ROM"):
-        srv.submit(req(4, 16, 1, 30, 0.0, 5))
-    while srv.resident or srv.queue:
-        srv.step()
-    assert srv.resident_bytes == 0
+        serve({8: make_engine(8, 4)}, [req(4, 16, 1, 30, 0.0, 5)], serving.SimClock(), hbm_budget=1e12)
 
 
 def test_uneven_horizons_and_latency_accounting():
-    engines = {8: make_engine(8, 32)}
+    clk = serving.SimClock()
+    engines = {8: make_engine(8, 32, clock=clk)}
     reqs = [req(i, 8, 2, h, 0.001 * i, 20 + i) for i, h in enumerate([15, 45, 30, 60])]
-    srv = serving.Server(engines, hbm_budget=1e12, chunk=15, clock=FakeClock())
-    rep = srv.run(reqs)
+    rep = serve(engines, reqs, clk, hbm_budget=1e12, chunk=15)
     assert rep["forcing_steps"] == sum(r.horizon * r.m for r in reqs)
-    assert rep["p50_latency_s"] > 0 and rep["p99_latency_s"] >= rep["p50_latency_s"]
-    assert all(r.done >= r.arrival for r in reqs)
-    # submitted together, the shorter horizons finish in earlier scheduler steps
-    engines = {8: make_engine(8, 32)}
+    lat = np.array([r.latency for r in reqs])
+    assert rep["p50_latency_s"] == pytest.approx(np.percentile(lat, 50))
+    assert rep["p99_latency_s"] == pytest.approx(np.percentile(lat, 99))
+    assert rep["max_latency_s"] == pytest.approx(lat.max())
+    assert all(r.done >= r.admitted >= r.arrival for r in reqs)
+    # submitted together, the shorter horizons finish first
+    clk = serving.SimClock()
+    engines = {8: make_engine(8, 32, clock=clk)}
     reqs = [req(i, 8, 2, h, 0.0, 30 + i) for i, h in enumerate([15, 45, 30, 60])]
-    srv = serving.Server(engines, hbm_budget=1e12, chunk=15, clock=FakeClock())
-    for r in reqs:
-        srv.submit(r)
-    order, steps = [], 0
-    while srv.resident or srv.queue:
-        srv.step()
-        steps += 1
-        order += [(r.rid, steps) for r in srv.finished if r.rid not in dict(order)]
-    assert dict(order) == {0: 1, 2: 2, 1: 3, 3: 4}
+    serve(engines, reqs, clk, hbm_budget=1e12, chunk=15)
+    assert [r.rid for r in sorted(reqs, key=lambda r: r.done)] == [0, 2, 1, 3]
     assert all(len(r.seismograms[0]) == r.horizon // r.every for r in reqs)
 
 
@@ -121,34 +130,60 @@ def test_poisson_request_generator():
     assert all(np.diff([r.arrival for r in reqs]) > 0)
 
 
-def test_sjf_admits_least_work_first_among_those_that_fit():
-    """Policy "sjf": the least work (horizon x columns) first, not blocked by
-    a queued request that does not fit; ties in arrival order."""
-    engines = {8: make_engine(8, 6)}
+def test_least_work_first_among_those_that_fit():
+    """Least work (horizon x columns) first, not blocked by a queued request
+    that does not fit."""
+    clk = serving.SimClock()
+    engines = {8: make_engine(8, 6, clock=clk)}
     reqs = [req(0, 8, 4, 30, 0.0, 1), req(1, 8, 4, 30, 0.0, 2), req(2, 8, 1, 10, 0.0, 3)]
-    srv = serving.Server(engines, hbm_budget=1e12, chunk=10, clock=FakeClock(), policy="sjf")
-    for r in reqs:
-        srv.submit(r)
-    srv._admit()
-    assert [r.rid for r in srv.resident] == [2, 0] and [r.rid for r in srv.queue] == [1]
-    rep = srv.run([])
-    assert rep["requests"] == 3 and not srv.queue and not srv.resident
+    rep = serve(engines, reqs, clk, hbm_budget=1e12, chunk=10, policy="least_work")
+    a, b, c = reqs
+    assert rep["requests"] == 3
+    assert c.admitted <= a.admitted < b.admitted and b.admitted >= a.done
 
 
-def test_sjf_and_fifo_give_the_same_results():
+def test_least_work_and_fifo_give_the_same_results():
     def mix():
         return [req(i, 8 if i % 3 else 12, 1 + i % 4, 15 + 7 * (i % 5), 0.0, 40 + i) for i in range(9)]
 
     out = {}
-    for policy in ("fifo", "sjf"):
-        engines = {8: make_engine(8, 6), 12: make_engine(12, 5, seed=3)}
+    for policy in ("fifo", "least_work"):
+        clk = serving.SimClock()
+        engines = {8: make_engine(8, 6, clock=clk), 12: make_engine(12, 5, seed=3, clock=clk)}
         rs = mix()
-        rep = serving.Server(engines, hbm_budget=1e12, chunk=6, clock=FakeClock(), policy=policy).run(rs)
+        rep = serve(engines, rs, clk, hbm_budget=1e12, chunk=6, policy=policy)
         assert rep["requests"] == 9
         out[policy] = {r.rid: r.seismograms for r in rs}
     for rid in out["fifo"]:
-        for x, y in zip(out["fifo"][rid], out["sjf"][rid]):
+        for x, y in zip(out["fifo"][rid], out["least_work"][rid]):
             np.testing.assert_array_equal(x, y)
+
+
+def _overload(max_wait):
+    """A stream of small requests (2 columns, 10 steps) arriving faster than
+    they drain keeps an 8-slot engine full; one large request (all 8 slots)
+    arrives early."""
+    clk = serving.SimClock(per_call=1e-6, per_step=1e-4)
+    e = make_engine(4, 8, r=1, clock=clk)
+    small = [req(i + 1, 4, 2, 10, 2e-4 * i, 100 + i, every=10) for i in range(600)]
+    big = req(0, 4, 8, 10, 1e-3, 7, every=10)
+    rep = serve({4: e}, [big] + small, clk, hbm_budget=1e12, chunk=10, policy="least_work", max_wait=max_wait)
+    return big, small, rep
+
+
+def test_least_work_wait_is_bounded_under_sustained_overload():
+    """Without a bound the large request waits until the stream ends (it
+    never fits while smaller work keeps arriving); with max_wait it is
+    admitted within max_wait plus the time the resident work needs to
+    drain (one chunk here), and every request still completes."""
+    big, small, rep = _overload(0.0)
+    stream_end = max(r.arrival for r in small)
+    assert big.admitted - big.arrival > 0.5 * stream_end
+    W = 0.005
+    big, small, rep = _overload(W)
+    assert big.admitted - big.arrival <= W + 0.003
+    assert rep["requests"] == len(small) + 1
+    assert rep["max_wait_s"] == pytest.approx(max(r.admitted - r.arrival for r in small + [big]))
 
 
 def test_unknown_policy_is_refused():
