@@ -86,44 +86,59 @@ def cpu_baseline(w, material, sample_steps: int) -> dict:
 
 
 def rom_section(app, w, fom_value, args) -> dict:
-    """Config 3 as specified, with the reference's (Euclidean) Galerkin ROM:
-    POD (FP64, K = 256) of 400 + 400 snapshots (every 200 steps) of the first
-    4 config-2 forcings, 1024 seeded forcings (32 depths x 32 periods)
-    advanced for --rom-steps steps in FP64 MFMA and in BF16x3; seismogram
-    errors against the FP64 FOM on 16 spot-checked forcings. The basis spans
-    4 source depths sampled every 10 s, so the error at other depths is
-    large: it is stated, not tuned."""
+    """Config 3 as specified: POD (FP64, K = 256) of 400 + 400 snapshots
+    (every 200 steps) of the first 4 config-2 forcings, 1024 seeded forcings
+    (32 depths x 32 periods) advanced for --rom-steps steps in FP64 MFMA,
+    BF16x3 and BF16x4; seismogram errors against the FP64 FOM on 16
+    spot-checked forcings. Run twice on the same training set: with the
+    reference's Euclidean Galerkin projection, and with the energy-consistent
+    one (POD and projection in the FOM energy weights). The spectral radius
+    of each pair's one-step matrix separates instability (radius > 1: the
+    Euclidean pair's homogeneous solutions grow radius^steps) from basis
+    error (the energy pair's radius is 1; what remains is the basis: 4
+    training depths cannot span 32 test depths at K = 256)."""
     import shaw_amd
     from shaw_amd import rom_workflow as rw, workloads
 
     n = args.rom_steps
     train = [w.sources[8 * i] for i in range(4)]  # one per config-2 depth
-    t0 = time.time()
-    basis = rw.train_basis(app, train, w.dt, n, max(1, n // 100), 256)
-    arv, ars = rw.operators(app, basis, 256)
     d, p = workloads.seeded_depths_periods(32, 32, 3)
     test = [(dd, 0.0, pp, workloads.DELAY_S) for dd in d for pp in p]
-    B, ftab = rw.forcing_inputs(app, basis.phi_v, test, w.dt, n)
-    C = rw.receiver_rows(app, basis.phi_v, workloads.RECEIVERS)
-    setup = time.time() - t0
-    res, rates = {}, {}
-    for prec, name in ((shaw_amd.ROM_FP64, "fp64"), (shaw_amd.ROM_BF16X3, "bf16x3")):
-        res[name], rates[name] = run_rom(arv, ars, B, ftab, C, w.dt, n, prec)
-        log(f"config3 ROM K=256 {name}: {rates[name]:.3e} forcing-steps/s")
     spot = list(range(0, len(test), len(test) // 16))[:16]
     fom = rw.fom_seismograms(app, [test[j] for j in spot], w.dt, n, workloads.RECEIVERS, 10)
-    err_fom = {k: [rw.rel_l2(res[k][j], fom[i]) for i, j in enumerate(spot)] for k in res}
-    err_split = [rw.rel_l2(res["bf16x3"][j], res["fp64"][j]) for j in range(len(test))]
-    return {"workload": f"config3: Euclidean Galerkin ROM (the reference's) K=256, POD FP64 of 400+400 snapshots "
-                        f"of 4 config-2 forcings, M={len(test)} forcings (32 depths x 32 periods), {n} steps, "
-                        "180 receivers every 10 steps",
-            "fp64_forcing_steps_per_s": rates["fp64"], "bf16x3_forcing_steps_per_s": rates["bf16x3"],
-            "speedup_vs_fp64_fom": {k: v / fom_value for k, v in rates.items()},
-            "mfma_tflops": mfma_tflops(256, rates),
-            "err_vs_fp64_fom_16_spot": {k: summary(v) for k, v in err_fom.items()},
-            "err_bf16x3_vs_fp64_rom": summary(err_split),
-            "pod_sigma_ratio_256": float(basis.sigma_v[255] / basis.sigma_v[0]),
-            "setup_s": setup}
+    out = {"workload": f"config3: K=256, POD FP64 of 400+400 snapshots of 4 config-2 forcings, M={len(test)} "
+                       f"forcings (32 depths x 32 periods), {n} steps, 180 receivers every 10 steps"}
+    for energy, tag in ((False, "euclidean"), (True, "energy")):
+        t0 = time.time()
+        basis = rw.train_basis(app, train, w.dt, n, max(1, n // 100), 256, energy=energy)
+        t1 = time.time()
+        arv, ars = rw.operators(app, basis, 256)
+        t2 = time.time()
+        B, ftab = rw.basis_forcing_inputs(app, basis, 256, test, w.dt, n)
+        C = rw.receiver_rows(app, basis.phi_v, workloads.RECEIVERS)
+        t3 = time.time()
+        res, rates = {}, {}
+        for prec, name in ((shaw_amd.ROM_FP64, "fp64"), (shaw_amd.ROM_BF16X3, "bf16x3"),
+                           (shaw_amd.ROM_BF16X4, "bf16x4")):
+            res[name], rates[name] = run_rom(arv, ars, B, ftab, C, w.dt, n, prec)
+            log(f"config3 ROM ({tag}) K=256 {name}: {rates[name]:.3e} forcing-steps/s")
+        err_fom = {k: [rw.rel_l2(res[k][j], fom[i]) for i, j in enumerate(spot)] for k in res}
+        err_split = {k: summary([rw.rel_l2(res[k][j], res["fp64"][j]) for j in range(len(test))])
+                     for k in ("bf16x3", "bf16x4")}
+        rho = rw.amplification_radius(arv, ars, w.dt)
+        out[tag] = {"projection": "reference (Euclidean Galerkin)" if not energy else
+                    "energy-consistent (POD and projection in the FOM energy weights)",
+                    "forcing_steps_per_s": rates,
+                    "speedup_vs_fp64_fom": {k: v / fom_value for k, v in rates.items()},
+                    "mfma_tflops": mfma_tflops(256, rates),
+                    "err_vs_fp64_fom_16_spot": {k: summary(v) for k, v in err_fom.items()},
+                    "err_vs_fp64_rom": err_split,
+                    "amplification_radius": rho, "amplification_over_run": rho ** n,
+                    "pod_sigma_ratio_256": float(basis.sigma_v[255] / basis.sigma_v[0]),
+                    "setup_s": {"fom_training": basis.seconds["fom_training"], "pod": basis.seconds["pod"],
+                                "projection": t2 - t1, "inputs": t3 - t2, "total": t3 - t0}}
+        log(f"config3 ({tag}): radius {rho:.8f}, errors vs FOM { {k: summary(v) for k, v in err_fom.items()} }")
+    return out
 
 
 def run_rom(arv, ars, B, ftab, C, dt, n, prec):
@@ -201,13 +216,17 @@ def rom_err_section(app, w, fom_value, args):
 
 
 def serving_section(app, w, basis, args) -> dict:
-    """Config 4 on the energy-consistent basis of the ROM section: Poisson
-    arrivals at 200 requests/s for --serve-seconds, each with M ~ U{1..256}
-    forcings at the basis's source depth (periods ~ U[20, 70] s), a ROM size
-    K ~ U{128, 256, 512} (nested modes of one POD) and a horizon
-    ~ U[5000, 20000] steps; one resident BF16x4 engine per K; admission by
-    HBM footprint; p50/p99 latency; sampled per-request errors against the
-    FP64 ROM of the same K and the FP64 FOM."""
+    """Config 4 on the energy-consistent basis of the ROM section, served by
+    the C++ loop (shaw_serve): Poisson arrivals at 200 requests/s for
+    --serve-seconds, each with M ~ U{1..256} forcings at the basis's source
+    depth (periods ~ U[20, 70] s), a ROM size K ~ U{128, 256, 512} (nested
+    modes of one POD) and a horizon ~ U[5000, 20000] steps; one resident
+    BF16x4 engine per K; admission by HBM footprint, least work first with a
+    wait bound; p50/p99 latency, the drain rate and the longest queue. A
+    second run at the arrival rate that offers 80 % of the measured drain
+    rate gives the latency of a load the engines sustain. Sampled requests'
+    seismograms are checked against the FP64 ROM of the same K and the FP64
+    FOM; the other requests' seismograms are discarded on retirement."""
     import shaw_amd
     from shaw_amd import rom_workflow as rw, serving, workloads
 
@@ -219,10 +238,13 @@ def serving_section(app, w, basis, args) -> dict:
     ops, engines = {}, {}
     for k in (128, 256, 512):
         ops[k] = rw.operators(app, basis, k)
-        rom = shaw_amd.Rom(ops[k][0], ops[k][1], w.dt, shaw_amd.ROM_BF16X4, C512[:, :k], max_cols=2048,
+        rom = shaw_amd.Rom(ops[k][0], ops[k][1], w.dt, shaw_amd.ROM_BF16X4, C512[:, :k], max_cols=4096,
                            max_samples=2000)
-        engines[k] = serving.GpuEngine(rom, 2048)
+        engines[k] = serving.GpuEngine(rom, 4096)
     bcol = basis.mv[g] * rho_inv[g] * basis.phi_v[g, :]
+    setup = time.time() - t0
+    mean_work = 128.5 * 12500.0  # E[M] E[horizon] forcing-steps per request
+
     drawn = []  # periods per request, in request-id order
 
     def make_inputs(k, m, h, gen):
@@ -232,47 +254,50 @@ def serving_section(app, w, basis, args) -> dict:
         f = shaw_amd.forcing_table(["ricker"] * m, [workloads.DELAY_S] * m, list(ps), w.dt, h)
         return b, f
 
-    reqs = serving.poisson_requests(np.random.default_rng(4), 200.0, args.serve_seconds, [128, 256, 512], (1, 256),
-                                    (5000, 20000), make_inputs)
-    setup = time.time() - t0
-    fresh = [serving.Request(r.rid, r.k, r.b, r.ftab, r.horizon, r.every, r.arrival) for r in reqs]
-    srv = serving.Server(engines, hbm_budget=96e9, chunk=1000)
-    rep = srv.run(reqs, wall_limit=600.0)
-    log(f"serving (fifo): {rep}")
-    # sampled per-request errors (first column of each sampled request)
-    sample = reqs[:: max(1, len(reqs) // 6)][:6]
+    def one_run(rate, seconds, seed, keep_every):
+        drawn.clear()
+        reqs = serving.poisson_requests(np.random.default_rng(seed), rate, seconds, [128, 256, 512], (1, 256),
+                                        (5000, 20000), make_inputs)
+        # keep a few requests' seismograms; the rest are discarded on retirement
+        kept = [r for r in reqs if r.rid % keep_every == 0][:6]
+        for r in reqs:
+            r.keep = r in kept
+        srv = serving.Server(engines, hbm_budget=200e9, chunk=1000, policy="least_work", max_wait=2.0)
+        rep = srv.run(reqs, wall_limit=900.0, keep=lambda r: r.keep)
+        rep["offered_forcing_steps_per_s"] = rate * mean_work
+        rep["arrival_rate_per_s"] = rate
+        rep["arrival_seconds"] = seconds
+        return reqs, kept, rep
+
+    reqs, kept, rep = one_run(200.0, args.serve_seconds, 4, 37)
+    log(f"serving 200 req/s: {rep}")
+    # sampled per-request errors (first column of each kept request)
     e_rom, e_fom = [], []
-    fom_cols = [(depth, 0.0, float(drawn[r.rid][0]), workloads.DELAY_S) for r in sample]
-    hmax = max(r.horizon for r in sample)
-    fom = rw.fom_seismograms(app, fom_cols, w.dt, hmax, workloads.RECEIVERS, 10)
-    for i, r in enumerate(sample):
+    hmax = max(r.horizon for r in kept)
+    periods = [float(drawn[r.rid][0]) for r in kept]
+    fom = rw.fom_seismograms(app, [(depth, 0.0, p_, workloads.DELAY_S) for p_ in periods], w.dt, hmax,
+                             workloads.RECEIVERS, 10)
+    for i, r in enumerate(kept):
         arv, ars = ops[r.k]
         ref = shaw_amd.rom_run(arv, ars, r.b[:, :1], r.ftab[:, :1], w.dt, r.horizon, C512[:, :r.k], r.every)
         e_rom.append(rw.rel_l2(r.seismograms[0], ref["seismogram"][0]))
         e_fom.append(rw.rel_l2(r.seismograms[0], fom[i][:r.horizon // r.every]))
-    # the same arrivals admitted least-work-first: latency under the other
-    # policy, and the sampled results equal to the FIFO run's bit for bit
-    kept = {r.rid: r.seismograms[0] for r in sample}
-    for r in reqs:
-        r.seismograms = []  # the host copies of one run are tens of GB
-    srv2 = serving.Server(engines, hbm_budget=96e9, chunk=1000, policy="sjf")
-    rep2 = srv2.run(fresh, wall_limit=600.0)
-    log(f"serving (sjf): {rep2}")
-    same = all(np.array_equal(fresh[r.rid].seismograms[0], kept[r.rid]) for r in sample)
-    for r in fresh:
-        r.seismograms = []
+    rep["sampled_requests"] = [{"rid": r.rid, "K": r.k, "M": r.m, "horizon": r.horizon,
+                                "err_vs_fp64_rom": a, "err_vs_fp64_fom": b_}
+                               for r, a, b_ in zip(kept, e_rom, e_fom)]
+    del reqs, kept
+    rate2 = 0.8 * rep["throughput_forcing_steps_per_s"] / mean_work
+    _, _, rep2 = one_run(rate2, args.serve_seconds, 5, 10 ** 9)
+    log(f"serving {rate2:.1f} req/s: {rep2}")
+    rep["sustainable"] = {k: rep2[k] for k in ("arrival_rate_per_s", "offered_forcing_steps_per_s", "requests",
+                                                "p50_latency_s", "p99_latency_s", "max_latency_s", "max_wait_s",
+                                                "throughput_forcing_steps_per_s", "wall_s", "launches", "max_queue")}
     for e in engines.values():
         e.rom.close()
-    rep["policy"] = "fifo"
-    rep["sjf"] = {k: rep2[k] for k in ("p50_latency_s", "p99_latency_s", "throughput_forcing_steps_per_s", "wall_s",
-                                       "launches")}
-    rep["sjf"]["sampled_results_equal_fifo"] = bool(same)
-    rep.update({"workload": f"config4 on the energy-consistent basis: Poisson 200 req/s for {args.serve_seconds:g} s "
-                            f"({len(reqs)} requests), M~U{{1..256}}, K~U{{128,256,512}}, horizon~U[5000,20000] steps, "
-                            "source depth 640 km, periods~U[20,70] s, BF16x4 engines, 180 receivers every 10 steps",
-                "sampled_requests": [{"rid": r.rid, "K": r.k, "M": r.m, "horizon": r.horizon,
-                                      "err_vs_fp64_rom": a, "err_vs_fp64_fom": b_}
-                                     for r, a, b_ in zip(sample, e_rom, e_fom)],
+    rep.update({"workload": f"config4 on the energy-consistent basis: Poisson 200 req/s for {args.serve_seconds:g} s, "
+                            "M~U{1..256}, K~U{128,256,512}, horizon~U[5000,20000] steps, source depth 640 km, "
+                            "periods~U[20,70] s, BF16x4 engines, 180 receivers every 10 steps, C++ loop "
+                            "(least work first, 2 s wait bound); then 10 s at 80 % of the measured drain rate",
                 "setup_s": setup})
     return rep
 
@@ -371,6 +396,35 @@ def config5_section(args) -> dict:
     return out
 
 
+def rom_kernel_only(args) -> int:
+    """The batched reduced leapfrog alone (k_rom_steps), for profiling: K x K
+    skew operator pairs (random, seeded), M columns, 180 receivers every 10
+    steps, --rom-steps steps timed per precision with HIP events on the
+    engine's stream. Prints one JSON line of forcing-steps/s and MFMA TFLOP/s."""
+    import shaw_amd
+
+    K, M, n = args.rom_kernel_k, args.rom_kernel_m, args.rom_steps
+    rng = np.random.default_rng(K)
+    A = rng.standard_normal((K, K)) * (0.5 / np.sqrt(K))
+    C = rng.standard_normal((180, K))
+    B = rng.standard_normal((K, M))
+    f = rng.standard_normal((n + 50, M))
+    out = {"workload": f"batched ROM kernel only: K={K}, M={M}, {n} steps, synthetic skew operators"}
+    names = {"fp64": shaw_amd.ROM_FP64, "bf16x3": shaw_amd.ROM_BF16X3, "bf16x4": shaw_amd.ROM_BF16X4}
+    for name in args.rom_kernel_prec.split(","):
+        rom = shaw_amd.Rom(A, -A.T.copy(), 0.05, names[name], C, max_cols=M, max_samples=(n + 50) // 10)
+        rom.admit(B, f, [n + 50] * M, 10)
+        rom.advance(50)
+        rom.sync()
+        ms = rom.time(n)
+        rom.close()
+        rate = M * n / (ms / 1e3)
+        out[name] = {"ms": ms, "forcing_steps_per_s": rate, "mfma_tflops": mfma_tflops(K, {name: rate})[name]}
+        log(f"ROM kernel {name}: {rate:.3e} forcing-steps/s")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -389,10 +443,14 @@ def main() -> int:
     ap.add_argument("--no-rom", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--rom-steps", type=int, default=20000)
-    ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--config5", action="store_true", help="also measure the large FOM (2048 x 8192, M = 64)")
     ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--c5-err-steps", type=int, default=8000)
+    ap.add_argument("--rom-kernel", action="store_true", help="time only the batched ROM kernel (profiling)")
+    ap.add_argument("--rom-kernel-k", type=int, default=256)
+    ap.add_argument("--rom-kernel-m", type=int, default=1024)
+    ap.add_argument("--rom-kernel-prec", default="fp64,bf16x3,bf16x4")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -400,6 +458,9 @@ def main() -> int:
         print(f"bench.py measures one GPU only (this build runs the hot path on a single MI355X; "
               f"--gpus {args.gpus} / WORLD_SIZE {world} is not supported)", file=sys.stderr)
         return 2
+
+    if args.rom_kernel:
+        return rom_kernel_only(args)
 
     import shaw_amd
     from shaw_amd import workloads

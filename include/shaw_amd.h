@@ -273,6 +273,16 @@ int shaw_rom_project_noalloc(uint64_t nr, uint64_t nth, const double* coef_vp, c
                              const double* phi_v, uint64_t kv, const double* phi_s, uint64_t ks,
                              double* ar_v, double* ar_s, void* stream);
 
+/* The basis Phi LayoutLeft [n x k] made orthonormal in the inner product
+ * diag(w) (w [n] > 0) without changing its span: phi_out = Phi R^-1 with
+ * R^T R = Phi^T diag(w) Phi (R upper triangular, Cholesky), so
+ * phi_out^T diag(w) phi_out = I and coordinates in phi_out convert back to
+ * Phi's by q = R^-1 q_out. r_out [k][k] row-major (may be NULL). With the
+ * FOM energy weights this turns the reference's POD bases (lsv files) into
+ * bases for shaw_rom_project_energy. No counterpart in the reference.     */
+int shaw_basis_orthonormalize(const double* phi, uint64_t n, uint64_t k, const double* w, double* phi_out,
+                              double* r_out);
+
 typedef struct {
   uint64_t kv, ks;
   double dt;
@@ -310,6 +320,8 @@ int shaw_rom_retire_batch(shaw_rom_t h, uint64_t n, const uint64_t* slots, uint6
  * step count; any output may be NULL.                                      */
 int shaw_rom_state(shaw_rom_t h, uint64_t slot, double* qv, double* qs, uint64_t* step);
 int shaw_rom_free_slots(shaw_rom_t h, uint64_t* n);
+/* Non-blocking: *idle = 1 when the engine's stream has no work in flight. */
+int shaw_rom_query(shaw_rom_t h, int* idle);
 int shaw_rom_time(shaw_rom_t h, uint64_t nsteps, double* total_ms);
 void shaw_rom_destroy(shaw_rom_t h);
 
@@ -335,6 +347,7 @@ typedef struct {
   int (*retire_batch)(void* ctx, uint64_t n, const uint64_t* slots, uint64_t max_samples, double* seis,
                       uint64_t* samples);
   int (*free_slots)(void* ctx, uint64_t* n);
+  int (*query)(void* ctx, int* idle);  /* non-blocking: *idle = 1 when all enqueued work is done */
 } shaw_engine;
 /* The table of a GPU engine: shaw_rom_admit/advance/sync/retire_batch/free_slots. */
 int shaw_engine_from_rom(shaw_rom_t rom, shaw_engine* out);
@@ -378,7 +391,10 @@ typedef struct {
 } shaw_serve_stats;
 
 /* Serve n requests arriving at their arrival times until all are done;
- * writes each request's seismograms, admitted_s and done_s, and the stats. */
+ * writes each request's seismograms, admitted_s and done_s, and the stats.
+ * Engines run independently: an engine that finished its chunk retires its
+ * finished requests, admits queued ones and advances again while the others
+ * are still running.                                                       */
 int shaw_serve(const shaw_engine* engines, int n_engines, const shaw_serve_options* opts,
                shaw_request* reqs, uint64_t n, shaw_serve_stats* stats);
 

@@ -105,6 +105,7 @@ def lib():
         "shaw_fom_time": (i32, [vp, u64, P(dbl), P(dbl)]),
         "shaw_fom_time_chunks": (i32, [vp, u64, u64, vp]),
         "shaw_app_energy_weights": (i32, [vp, vp, vp]),
+        "shaw_basis_orthonormalize": (i32, [vp, u64, u64, vp, vp, vp]),
         "shaw_fom_complexity": (i32, [u64, u64, u64, u64, u64, i32, P(dbl), P(dbl)]),
         "shaw_fom_bytes_per_step": (i32, [vp, P(dbl), P(dbl)]),
         "shaw_fom_config": (i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i32), P(i32), P(i32)]),
@@ -631,6 +632,17 @@ def fom_complexity(nvp: int, nsp: int, nnz_vp: int, nnz_sp: int, m: int, rank2: 
     mb, fl = C.c_double(), C.c_double()
     _check(lib().shaw_fom_complexity(nvp, nsp, nnz_vp, nnz_sp, m, int(rank2), C.byref(mb), C.byref(fl)))
     return mb.value, fl.value
+
+
+def basis_orthonormalize(phi: np.ndarray, w: np.ndarray):
+    """(phi_out, R): phi [n x k] made orthonormal in diag(w), phi_out = phi R^-1
+    (shaw_basis_orthonormalize)."""
+    n, k = phi.shape
+    p = np.asfortranarray(phi, dtype=np.float64)
+    out = np.zeros((n, k), order="F")
+    R = np.zeros((k, k))
+    _check(lib().shaw_basis_orthonormalize(_p(p), n, k, _p(_f64(w).ravel()), _p(out), _p(R)))
+    return out, R
 
 
 def debug_copy(nbytes: int, reps: int = 10) -> float:

@@ -906,6 +906,32 @@ __device__ __forceinline__ float laneRead(float x, int addr) {
   return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
 }
 
+// Neighbour exchange along a one-wavefront strip without LDS: DPP whole-wave
+// rotates by one lane (gfx9 family), a VALU move instead of a round trip
+// through the LDS crossbar. fromWest: lane p receives lane p - 1 (lane 0
+// receives lane 63, wave_ror:1); fromEast: lane p receives lane p + 1 (lane
+// 63 receives lane 0, wave_rol:1).
+constexpr int kDppWaveRol1 = 0x134, kDppWaveRor1 = 0x13C;
+template <int CTRL>
+__device__ __forceinline__ int dppMove(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dppMove(double x) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const int lo = dppMove<CTRL>((int)(uint32_t)u);
+  const int hi = dppMove<CTRL>((int)(uint32_t)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+template <int CTRL>
+__device__ __forceinline__ float dppMove(float x) {
+  return __int_as_float(dppMove<CTRL>(__float_as_int(x)));
+}
+template <class T>
+__device__ __forceinline__ T fromWest(T x) { return dppMove<kDppWaveRor1>(x); }
+template <class T>
+__device__ __forceinline__ T fromEast(T x) { return dppMove<kDppWaveRol1>(x); }
+
 // ColumnForcing for k_fusedW: the increments of the NS steps sit in a small
 // LDS table of the wave ([NS][CPT], written once by the wave's first CPT
 // lanes and read only by the same wave, whose LDS operations are ordered),
@@ -987,7 +1013,6 @@ __global__ void __launch_bounds__(256) k_fusedW(StepArgsN<T, NS> aa) {
   const uint32_t cbRaw = grp * G + w;
   const bool live = cbRaw < a.nb;  // waves past the last column block load and compute, store nothing
   const uint32_t cb = live ? cbRaw : a.nb - 1;
-  const int addrW = (int)(((pl + P - 1) % P) * 4), addrE = (int)(((pl + 1) % P) * 4);
 
   const uint32_t nr = a.nr, nth = a.nth, rs = 2 * nth - 1;
   const uint32_t Q = P - (2 * NS - 1);
@@ -1125,7 +1150,7 @@ __global__ void __launch_bounds__(256) k_fusedW(StepArgsN<T, NS> aa) {
         for (int k = 0; k < CPT; ++k) {
           // west stp of level l-1: the west lane's (level 1: lane 0 loaded its own)
           const T Ek = l == 1 ? cur.E[k] : stpN[l - 1][k];
-          const T fromW = laneRead(Ek, addrW);
+          const T fromW = fromWest(Ek);
           const T Wk = (l == 1 && pl == 0) ? cur.W[k] : fromW;
           const T Nk = l == 1 ? cur.N[k] : srpN[l - 1][k];
           const T Sk = S[l - 1][k];
@@ -1154,14 +1179,14 @@ __global__ void __launch_bounds__(256) k_fusedW(StepArgsN<T, NS> aa) {
           }
 #pragma unroll
           for (int k = 0; k < CPT; ++k) {
-            const T vE = laneRead(vj[l - 1][k], addrE);  // v_l of the east point, previous input row
+            const T vE = fromEast(vj[l - 1][k]);  // v_l of the east point, previous input row
             stpN[l][k] = E[l - 1][k] + a.dt * (cl[6] * vj[l - 1][k] + cl[7] * vE);
           }
         }
       } else if (ST || (rowS && rsr >= j0 && rsr < j1)) {
         T vE[CPT];
 #pragma unroll
-        for (int k = 0; k < CPT; ++k) vE[k] = laneRead(vj[l - 1][k], addrE);
+        for (int k = 0; k < CPT; ++k) vE[k] = fromEast(vj[l - 1][k]);
         if (owner) {
           if (ST || rsr + 1 < nr) {
             T so[CPT];

@@ -69,53 +69,6 @@ __host__ __device__ inline float bf16ToF32(unsigned short h) {
 }
 
 // ------------------------------------------------------------ POD products
-// part[chunk][a][b] = sum_{r in chunk} A[r + n*a] * B[r + n*b]   (A: n x Ka, B: n x Kb, col-major)
-constexpr int GT = 32;  // output tile
-__global__ void __launch_bounds__(256) k_gemm_tn_partial(const double* __restrict__ A, const double* __restrict__ B,
-                                                         double* __restrict__ part, uint64_t n, uint32_t Ka, uint32_t Kb,
-                                                         uint64_t chunk) {
-  __shared__ double As[GT][GT + 1];
-  __shared__ double Bs[GT][GT + 1];
-  const uint32_t a0 = blockIdx.x * GT, b0 = blockIdx.y * GT;
-  const uint64_t r0 = (uint64_t)blockIdx.z * chunk, r1 = r0 + chunk < n ? r0 + chunk : n;
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;  // 16 x 16 threads, 2 x 2 outputs each
-  double acc[2][2] = {{0, 0}, {0, 0}};
-  for (uint64_t r = r0; r < r1; r += GT) {
-    // load GT rows x GT columns of A and B (coalesced along r)
-    for (int e = threadIdx.x; e < GT * GT; e += 256) {
-      const int rr = e % GT, cc = e / GT;
-      const uint64_t row = r + rr;
-      As[rr][cc] = (row < r1 && a0 + cc < Ka) ? A[row + n * (a0 + cc)] : 0.0;
-      Bs[rr][cc] = (row < r1 && b0 + cc < Kb) ? B[row + n * (b0 + cc)] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int rr = 0; rr < GT; ++rr) {
-      const double x0 = As[rr][tx], x1 = As[rr][tx + 16];
-      const double y0 = Bs[rr][ty], y1 = Bs[rr][ty + 16];
-      acc[0][0] += x0 * y0;
-      acc[0][1] += x0 * y1;
-      acc[1][0] += x1 * y0;
-      acc[1][1] += x1 * y1;
-    }
-    __syncthreads();
-  }
-  double* P = part + (uint64_t)blockIdx.z * Ka * Kb;
-  for (int u = 0; u < 2; ++u)
-    for (int v = 0; v < 2; ++v) {
-      const uint32_t a = a0 + tx + 16 * u, b = b0 + ty + 16 * v;
-      if (a < Ka && b < Kb) P[(uint64_t)a * Kb + b] = acc[u][v];
-    }
-}
-
-__global__ void k_reduce_chunks(const double* __restrict__ part, double* __restrict__ out, uint64_t n, uint32_t nchunks) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  double s = 0.0;
-  for (uint32_t c = 0; c < nchunks; ++c) s += part[(uint64_t)c * n + e];
-  out[e] = s;
-}
-
 // out[r + n*c] = sum_j S[r + n*j] * W[j*k + c]   (S: n x ns col-major, W: ns x k row-major)
 constexpr int NNC = 16;  // output columns per thread
 // out = base + alpha * (S W); base may be null (treated as 0) or alias out
@@ -589,17 +542,12 @@ void podDevice(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* phi, do
     DevBuf<double> part, dG, dW, dPhi, dNew, dTmp;
     struct { double* p; } dS{dSp};
     dPhi.alloc(n * k);
-    // host <- A^T B (A: n x ka, B: n x kb, column-major on the device)
+    // host <- A^T B (A: n x ka, B: n x kb, column-major on the device), FP64 MFMA
     auto gramTN = [&](const double* A, uint32_t ka, const double* B, uint32_t kb, std::vector<double>& G) {
-      const uint64_t chunk = std::max<uint64_t>(4096, (n + 255) / 256 / GT * GT + GT);
-      const uint32_t nch = (uint32_t)((n + chunk - 1) / chunk);
-      part.alloc((size_t)nch * ka * kb);
-      dG.alloc((size_t)ka * kb);
-      hipLaunchKernelGGL(k_gemm_tn_partial, dim3((ka + GT - 1) / GT, (kb + GT - 1) / GT, nch), dim3(256), 0, 0, A, B,
-                         part.p, n, ka, kb, chunk);
-      hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)(((uint64_t)ka * kb + 255) / 256)), dim3(256), 0, 0, part.p,
-                         dG.p, (uint64_t)ka * kb, nch);
-      SHAW_HIP(hipGetLastError());
+      const uint64_t need = projPartDoubles(n, ka, kb);
+      if (part.n < need) part.alloc(need);
+      if (dG.n < (size_t)ka * kb) dG.alloc((size_t)ka * kb);
+      projectTN(2, 0, 0, n, n, nullptr, A, ka, nullptr, B, kb, part.p, dG.p, 0);
       G.resize((size_t)ka * kb);
       SHAW_HIP(hipMemcpy(G.data(), dG.p, G.size() * 8, hipMemcpyDeviceToHost));
     };
@@ -835,6 +783,52 @@ int shaw_rom_project_energy(shaw_app_t app, const double* phiV, uint64_t kv, con
   });
 }
 
+int shaw_basis_orthonormalize(const double* phi, uint64_t n, uint64_t k, const double* w, double* phiOut,
+                              double* rOut) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(phi && w && phiOut && n > 0 && k > 0 && k <= n && k < (1u << 16), SHAW_ERR_ARG, "bad argument");
+    DevBuf<double> dPhi, dW, part, dG, dR, dOut;
+    dPhi.alloc(n * k); dPhi.upload(phi, n * k);
+    dW.alloc(n); dW.upload(w, n);
+    part.alloc(projPartDoubles(n, k, k));
+    dG.alloc(k * k);
+    // G = Phi^T diag(w) Phi (FP64 MFMA)
+    projectTN(2, 0, 0, n, n, nullptr, dPhi.p, (uint32_t)k, dW.p, dPhi.p, (uint32_t)k, part.p, dG.p, 0);
+    std::vector<double> G(k * k);
+    SHAW_HIP(hipMemcpy(G.data(), dG.p, k * k * sizeof(double), hipMemcpyDeviceToHost));
+    // G = R^T R, R upper triangular (Cholesky by rows of R)
+    std::vector<double> R(k * k, 0.0);
+    for (uint64_t i = 0; i < k; ++i) {
+      double d = G[i * k + i];
+      for (uint64_t q = 0; q < i; ++q) d -= R[q * k + i] * R[q * k + i];
+      SHAW_REQUIRE(d > 0.0, SHAW_ERR_ARG, "basis is rank-deficient in the given weights");
+      R[i * k + i] = std::sqrt(d);
+      for (uint64_t j = i + 1; j < k; ++j) {
+        double x = 0.5 * (G[i * k + j] + G[j * k + i]);
+        for (uint64_t q = 0; q < i; ++q) x -= R[q * k + i] * R[q * k + j];
+        R[i * k + j] = x / R[i * k + i];
+      }
+    }
+    // Rinv (upper triangular), column by column: R Rinv = I
+    std::vector<double> Ri(k * k, 0.0);
+    for (uint64_t c = 0; c < k; ++c)
+      for (uint64_t i = c + 1; i-- > 0;) {
+        double x = i == c ? 1.0 : 0.0;
+        for (uint64_t q = i + 1; q <= c; ++q) x -= R[i * k + q] * Ri[q * k + c];
+        Ri[i * k + c] = x / R[i * k + i];
+      }
+    // Phi_out = Phi Rinv
+    dR.alloc(k * k); dR.upload(Ri.data(), k * k);
+    dOut.alloc(n * k);
+    hipLaunchKernelGGL(k_gemm_nn_rows, dim3((unsigned)((n + 255) / 256), (unsigned)((k + NNC - 1) / NNC)), dim3(256), 0,
+                       0, dPhi.p, dR.p, dOut.p, nullptr, 1.0, n, (uint32_t)k, (uint32_t)k);
+    SHAW_HIP(hipGetLastError());
+    SHAW_HIP(hipMemcpy(phiOut, dOut.p, n * k * sizeof(double), hipMemcpyDeviceToHost));
+    if (rOut) std::copy(R.begin(), R.end(), rOut);
+  });
+}
+
 int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
   using namespace shaw;
   return guarded([&] {
@@ -1036,6 +1030,20 @@ int shaw_rom_info(shaw_rom_t h, uint64_t* kv, uint64_t* ks, uint64_t* r, uint64_
     if (r) *r = h->R;
     if (maxCols) *maxCols = h->maxCols;
     if (maxSamples) *maxSamples = h->maxSamples;
+  });
+}
+
+int shaw_rom_query(shaw_rom_t h, int* idle) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(h && idle, SHAW_ERR_ARG, "null argument");
+    const hipError_t e = hipStreamQuery(h->stream);
+    if (e == hipErrorNotReady) {
+      *idle = 0;
+      return;
+    }
+    SHAW_HIP(e);
+    *idle = 1;
   });
 }
 

@@ -15,11 +15,13 @@
 //      that has waited max_wait seconds is admitted before any other, and
 //      none passes it while it does not fit, so no request waits without
 //      bound under sustained load,
-//   3. advances every engine with active columns by one chunk of steps, all
-//      engines launched (each on its own stream) before any wait, so their
-//      work overlaps on the device,
-//   4. retires the requests whose columns all reached their horizon
-//      (latency = completion - arrival).
+//   3. advances every idle engine with active columns by one chunk of steps
+//      (each engine on its own stream), so the engines' work overlaps on the
+//      device, and
+//   4. when an engine's chunk is done (a non-blocking query), counts its
+//      columns' steps and retires its requests that reached their horizon
+//      (latency = completion - arrival); the engine is then admitted into
+//      and advanced again without waiting for the others.
 // A column's arithmetic does not depend on the others in the batch, so a
 // request's result is the same alone and in the mix.
 // The reference has no serving loop; its ROM entry point runs one problem
@@ -86,8 +88,6 @@ void serve(const shaw_engine* eng, int ne, const shaw_serve_options& o, shaw_req
     SHAW_REQUIRE(r.m > 0 && r.m <= e.capacity, SHAW_ERR_ARG,
                  "request " + std::to_string(r.id) + ": columns exceed the engine capacity");
     SHAW_REQUIRE(r.horizon > 0 && r.b && r.ftab, SHAW_ERR_ARG, "request " + std::to_string(r.id) + ": no inputs");
-    SHAW_REQUIRE(!(e.r && r.every) || r.seis, SHAW_ERR_ARG,
-                 "request " + std::to_string(r.id) + ": no seismogram buffer");
     footprint[i] = r.m * columnBytes(e, r.horizon, r.every);
     SHAW_REQUIRE(footprint[i] <= o.hbm_budget, SHAW_ERR_ARG,
                  "request " + std::to_string(r.id) + ": footprint exceeds the HBM budget");
@@ -99,17 +99,22 @@ void serve(const shaw_engine* eng, int ne, const shaw_serve_options& o, shaw_req
 
   std::deque<uint64_t> queue;
   std::vector<Live> live;
-  std::vector<uint64_t> free(ne);
+  std::vector<uint64_t> free(ne), launched(ne, 0);
+  std::vector<char> busy(ne, 0);
   uint64_t resident = 0, forcingSteps = 0, launches = 0, maxQueue = 0, finished = 0;
   double maxWait = 0.0;
-  std::vector<uint64_t> slotsBuf;
   auto refreshFree = [&](int k) {
     uint64_t f = 0;
     check(eng[k].free_slots(eng[k].ctx, &f), "free_slots");
     free[k] = f;
   };
   for (int k = 0; k < ne; ++k) refreshFree(k);
-  auto fits = [&](uint64_t i) { return free[req[i].engine] >= req[i].m && resident + footprint[i] <= o.hbm_budget; };
+  // admission needs the request's engine idle (an admission waits for the
+  // engine's launches in flight)
+  auto fits = [&](uint64_t i) {
+    const int k = req[i].engine;
+    return !busy[k] && free[k] >= req[i].m && resident + footprint[i] <= o.hbm_budget;
+  };
   auto admitOne = [&](std::deque<uint64_t>::iterator it) {
     const uint64_t i = *it;
     queue.erase(it);
@@ -141,12 +146,52 @@ void serve(const shaw_engine* eng, int ne, const shaw_serve_options& o, shaw_req
       admitOne(best);
     }
   };
+  // an engine whose chunk finished: count the steps its columns took, retire
+  // the requests that reached their horizon
+  auto harvest = [&](int k) {
+    std::vector<Live> keep;
+    for (Live& L : live) {
+      shaw_request& r = req[L.idx];
+      if (r.engine != k) {
+        keep.push_back(std::move(L));
+        continue;
+      }
+      const uint64_t taken = std::min(launched[k], L.left);
+      L.left -= taken;
+      forcingSteps += taken * r.m;
+      if (L.left > 0) {
+        keep.push_back(std::move(L));
+        continue;
+      }
+      const shaw_engine& e = eng[k];
+      const uint64_t T = r.every ? r.horizon / r.every : 0;
+      // seis NULL: the seismograms are not wanted and are not copied out
+      check(e.retire_batch(e.ctx, r.m, L.slots.data(), T, r.seis, nullptr), "retire");
+      r.done_s = rel();
+      resident -= L.bytes;
+      ++finished;
+    }
+    live = std::move(keep);
+    launched[k] = 0;
+    refreshFree(k);
+  };
   uint64_t next = 0;
   while (finished < n) {
     const double t = rel();
     while (next < n && req[order[next]].arrival_s <= t) queue.push_back(order[next++]);
     maxQueue = std::max<uint64_t>(maxQueue, queue.size());
-    if (queue.empty() && live.empty()) {
+    bool progressed = false;
+    for (int k = 0; k < ne; ++k) {
+      if (!busy[k]) continue;
+      int idle = 0;
+      check(eng[k].query ? eng[k].query(eng[k].ctx, &idle) : eng[k].sync(eng[k].ctx), "query");
+      if (!eng[k].query) idle = 1;
+      if (!idle) continue;
+      busy[k] = 0;
+      harvest(k);
+      progressed = true;
+    }
+    if (queue.empty() && live.empty() && finished < n) {
       // idle until the next arrival
       const double wait = req[order[next]].arrival_s - t;
       if (o.wait_until) o.wait_until(o.clock_ctx, start + req[order[next]].arrival_s);
@@ -154,46 +199,20 @@ void serve(const shaw_engine* eng, int ne, const shaw_serve_options& o, shaw_req
       continue;
     }
     admit();
-    if (live.empty()) continue;
-    // advance every engine with active columns by one chunk, launches first
-    std::vector<uint64_t> steps(ne, 0);
-    for (const Live& L : live) steps[req[L.idx].engine] = std::max(steps[req[L.idx].engine], L.left);
-    for (int k = 0; k < ne; ++k)
-      if (steps[k]) {
-        steps[k] = std::min<uint64_t>(steps[k], o.chunk_steps);
-        check(eng[k].advance(eng[k].ctx, steps[k]), "advance");
-        ++launches;
-      }
-    for (int k = 0; k < ne; ++k)
-      if (steps[k]) check(eng[k].sync(eng[k].ctx), "sync");
-    for (Live& L : live) {
-      const uint64_t taken = std::min(steps[req[L.idx].engine], L.left);
-      L.left -= taken;
-      forcingSteps += taken * req[L.idx].m;
+    // every idle engine with active columns advances by one chunk
+    for (int k = 0; k < ne; ++k) {
+      if (busy[k]) continue;
+      uint64_t left = 0;
+      for (const Live& L : live)
+        if (req[L.idx].engine == k) left = std::max(left, L.left);
+      if (!left) continue;
+      launched[k] = std::min<uint64_t>(left, o.chunk_steps);
+      check(eng[k].advance(eng[k].ctx, launched[k]), "advance");
+      busy[k] = 1;
+      ++launches;
+      progressed = true;
     }
-    // retire the requests that reached their horizon
-    std::vector<Live> keep;
-    for (Live& L : live) {
-      if (L.left > 0) {
-        keep.push_back(std::move(L));
-        continue;
-      }
-      shaw_request& r = req[L.idx];
-      const shaw_engine& e = eng[r.engine];
-      const uint64_t T = r.every ? r.horizon / r.every : 0;
-      std::vector<double> sink;
-      double* out = r.seis;
-      if (!out) {
-        sink.resize(std::max<uint64_t>(1, r.m * e.r * T));
-        out = sink.data();
-      }
-      check(e.retire_batch(e.ctx, r.m, L.slots.data(), T, out, nullptr), "retire");
-      refreshFree(r.engine);
-      r.done_s = rel();
-      resident -= L.bytes;
-      ++finished;
-    }
-    live = std::move(keep);
+    if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
     SHAW_REQUIRE(o.wall_limit_s <= 0 || rel() <= o.wall_limit_s, SHAW_ERR_STATE,
                  "serving run exceeded its wall-clock limit");
   }
@@ -231,6 +250,7 @@ int romRetire(void* c, uint64_t n, const uint64_t* slots, uint64_t maxSamples, d
   return shaw_rom_retire_batch(static_cast<shaw_rom_t>(c), n, slots, maxSamples, seis, samples);
 }
 int romFree(void* c, uint64_t* n) { return shaw_rom_free_slots(static_cast<shaw_rom_t>(c), n); }
+int romQuery(void* c, int* idle) { return shaw_rom_query(static_cast<shaw_rom_t>(c), idle); }
 }  // namespace
 
 extern "C" {
@@ -249,6 +269,7 @@ int shaw_engine_from_rom(shaw_rom_t rom, shaw_engine* out) {
     e.sync = romSync;
     e.retire_batch = romRetire;
     e.free_slots = romFree;
+    e.query = romQuery;
     *out = e;
   });
 }

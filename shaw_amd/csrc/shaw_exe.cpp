@@ -337,13 +337,41 @@ class Driver {
     return phi;
   }
 
+  // q <- R^-1 q for the upper-triangular R [k][k] (back substitution)
+  static void backSolve(const std::vector<double>& R, uint64_t k, double* q) {
+    for (uint64_t i = k; i-- > 0;) {
+      double x = q[i];
+      for (uint64_t j = i + 1; j < k; ++j) x -= R[i * k + j] * q[j];
+      q[i] = x / R[i * k + i];
+    }
+  }
+
   void romRun() {
     const uint64_t Kv = p_.vpRomSize, Ks = p_.spRomSize, N = p_.nSteps;
-    const std::vector<double> phiV = loadBasis(p_.vpBasisFile, p_.vpBasisBinary, info_.nvp, Kv, 13718);
-    const std::vector<double> phiS = loadBasis(p_.spBasisFile, p_.spBasisBinary, info_.nsp, Ks, 13719);
+    std::vector<double> phiV = loadBasis(p_.vpBasisFile, p_.vpBasisBinary, info_.nvp, Kv, 13718);
+    std::vector<double> phiS = loadBasis(p_.spBasisFile, p_.spBasisBinary, info_.nsp, Ks, 13719);
     std::vector<double> arV(Kv * Ks, 0.0), arS(Ks * Kv, 0.0);
+    // energy projection: the bases made orthonormal in the FOM energy weights
+    // (same spans), Ar_v = Phi_v^T Mv J_vp Phi_s, Ar_s = -Ar_v^T, reduced
+    // forcing Mv[g] rho^-1[g] Phi_v[g, :]; reduced coordinates are written
+    // back in the files' bases (q = R^-1 q_energy)
+    const bool energy = p_.romEnergy && !p_.dummyBasis;
+    std::vector<double> mv, Rv, Rs;
     const auto t0 = std::chrono::high_resolution_clock::now();
+    if (energy) {
+      std::vector<double> ms(info_.nsp), tmpV(phiV.size()), tmpS(phiS.size());
+      mv.resize(info_.nvp);
+      Rv.resize(Kv * Kv);
+      Rs.resize(Ks * Ks);
+      check(shaw_app_energy_weights(app_, mv.data(), ms.data()));
+      check(shaw_basis_orthonormalize(phiV.data(), info_.nvp, Kv, mv.data(), tmpV.data(), Rv.data()));
+      check(shaw_basis_orthonormalize(phiS.data(), info_.nsp, Ks, ms.data(), tmpS.data(), Rs.data()));
+      phiV.swap(tmpV);
+      phiS.swap(tmpS);
+      std::cout << "rom projection: energy-consistent" << std::endl;
+    }
     if (p_.disableCompRomJacs) std::cout << "The computation of the rom jacobians is disabled" << std::endl;
+    else if (energy) check(shaw_rom_project_energy(app_, phiV.data(), Kv, phiS.data(), Ks, mv.data(), arV.data(), arS.data()));
     else check(shaw_rom_project(app_, phiV.data(), Kv, phiS.data(), Ks, arV.data(), arS.data()));
     std::cout << "romJac time: " << std::fixed << std::setprecision(10)
               << std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count() << std::endl;
@@ -377,7 +405,8 @@ class Driver {
       std::vector<double> b(Kv * fs);
       for (uint64_t j = 0; j < fs; ++j) {
         const uint64_t g = sourceGid(part[j]);
-        for (uint64_t k = 0; k < Kv; ++k) b[k * fs + j] = rhoInv[g] * phiV[g + info_.nvp * k];
+        for (uint64_t k = 0; k < Kv; ++k)
+          b[k * fs + j] = energy ? mv[g] * rhoInv[g] * phiV[g + info_.nvp * k] : rhoInv[g] * phiV[g + info_.nvp * k];
       }
       const std::vector<double> tab = table(part);
       std::vector<uint64_t> hz(fs, N), slots(fs);
@@ -405,6 +434,10 @@ class Driver {
           std::vector<double> qv(Kv), qs(Ks);
           for (uint64_t j = 0; j < fs; ++j) {
             check(shaw_rom_state(rom, slots[j], qv.data(), qs.data(), nullptr));
+            if (energy) {
+              backSolve(Rv, Kv, qv.data());
+              backSolve(Rs, Ks, qs.data());
+            }
             if (step % p_.vpSnapFreq == 0)
               std::copy(qv.begin(), qv.end(), av.begin() + Kv * (step / p_.vpSnapFreq - 1 + Tv * j));
             if (step % p_.spSnapFreq == 0)

@@ -179,6 +179,12 @@ void InputParser::parseAll(const std::string& inputFile) {
       else if (p == "bf16x4") romPrecision = 2;
       else throw std::runtime_error("rom precision must be fp64, bf16x3 or bf16x4");
     }
+    if (r["projection"]) {
+      const std::string p = r["projection"].str();
+      if (p == "galerkin") romEnergy = false;
+      else if (p == "energy") romEnergy = true;
+      else throw std::runtime_error("rom projection must be galerkin or energy");
+    }
     if (vpBasisFile == "empty" || spBasisFile == "empty") {
       std::cout << "You did not set the basisfiles so I am using dummy basis" << std::endl;
       dummyBasis = true;

@@ -2,7 +2,8 @@
 // (src/shared/parser/input_parser.hpp:52-70, composed at types.hpp:64-69)
 // with the same sections, keys, defaults, validation messages and stdout.
 // Extensions: material kind "layered" (radius/density/velocity node lists)
-// and rom.precision ("fp64" | "bf16x3" | "bf16x4").
+// rom.precision ("fp64" | "bf16x3" | "bf16x4") and rom.projection
+// ("galerkin", the reference's, | "energy").
 #pragma once
 
 #include <cstdint>
@@ -47,6 +48,7 @@ struct InputParser {
   std::string vpBasisFile = "empty", spBasisFile = "empty";
   bool vpBasisBinary = true, spBasisBinary = true;
   int romPrecision = 0;  // SHAW_ROM_*: 0 fp64, 1 bf16x3, 2 bf16x4
+  bool romEnergy = false;  // energy-consistent projection of the given bases
 
   // Reads `inputFile` (argc must be 2 as in input_parser.hpp:57-60).
   InputParser(int argc, char** argv);

@@ -27,6 +27,7 @@ _SYNC = C.CFUNCTYPE(C.c_int, C.c_void_p)
 _RETIRE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64), C.c_uint64, C.POINTER(C.c_double),
                       C.POINTER(C.c_uint64))
 _FREE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint64))
+_QUERY = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_int))
 _CLOCK = C.CFUNCTYPE(C.c_double, C.c_void_p)
 _WAIT = C.CFUNCTYPE(None, C.c_void_p, C.c_double)
 
@@ -34,7 +35,7 @@ _WAIT = C.CFUNCTYPE(None, C.c_void_p, C.c_double)
 class ShawEngine(C.Structure):
     _fields_ = [("ctx", C.c_void_p), ("kv", C.c_uint64), ("ks", C.c_uint64), ("r", C.c_uint64),
                 ("capacity", C.c_uint64), ("admit", _ADMIT), ("advance", _ADVANCE), ("sync", _SYNC),
-                ("retire_batch", _RETIRE), ("free_slots", _FREE)]
+                ("retire_batch", _RETIRE), ("free_slots", _FREE), ("query", _QUERY)]
 
 
 class ShawRequest(C.Structure):
@@ -269,7 +270,7 @@ class NumpyEngine:
             width = self.R * int(max_samples)
             for j in range(n):
                 y = self.retire(int(slots[j]))
-                if width:
+                if width and seis:  # NULL: results not wanted
                     out = np.ctypeslib.as_array(seis, (int(n) * width,))
                     out[j * width:j * width + y.size] = y.ravel()
                 if samples:
@@ -280,11 +281,15 @@ class NumpyEngine:
             out[0] = self.free_slots()
             return 0
 
+        def query(ctx, idle):
+            idle[0] = 1  # advance() is synchronous
+            return 0
+
         t = ShawEngine()
         t.kv, t.ks, t.r, t.capacity = self.kv, self.ks, self.R, self.capacity
         t.admit, t.advance, t.sync = _ADMIT(guard(admit)), _ADVANCE(guard(advance)), _SYNC(lambda ctx: 0)
-        t.retire_batch, t.free_slots = _RETIRE(guard(retire)), _FREE(guard(free))
-        self._keep = (t.admit, t.advance, t.sync, t.retire_batch, t.free_slots)
+        t.retire_batch, t.free_slots, t.query = _RETIRE(guard(retire)), _FREE(guard(free)), _QUERY(guard(query))
+        self._keep = (t.admit, t.advance, t.sync, t.retire_batch, t.free_slots, t.query)
         return t
 
 

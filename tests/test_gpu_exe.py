@@ -144,3 +144,50 @@ def test_exe_binary_outputs_and_rom(tmp_path):
     assert res.returncode == 0, res.stderr + res.stdout[-3000:]
     seis = read_ascii(str(rdir / "seismogram_0"), False)
     assert rel_max(seis, ref["seismogram"][0].T) <= 1e-12
+
+
+def test_basis_orthonormalize_vs_numpy():
+    """phi_out = phi R^-1 with R^T R = phi^T diag(w) phi: phi_out is
+    w-orthonormal to 1e-12, spans phi (phi_out R == phi), and R equals
+    numpy's Cholesky factor of the weighted Gram matrix."""
+    rng = np.random.default_rng(3)
+    n, k = 5000, 40
+    phi = rng.standard_normal((n, k))
+    w = rng.uniform(0.5, 2.0, n)
+    out, R = shaw_amd.basis_orthonormalize(phi, w)
+    assert np.abs(out.T @ (w[:, None] * out) - np.eye(k)).max() <= 1e-12
+    assert np.abs(out @ R - phi).max() <= 1e-12 * np.abs(phi).max()
+    L = np.linalg.cholesky(phi.T @ (w[:, None] * phi))
+    assert np.abs(R - L.T).max() <= 1e-11 * np.abs(L).max()
+
+
+def test_exe_energy_projection_full_basis_equals_fom(tmp_path):
+    """rom.projection: energy on identity bases of the 4 x 5 mesh: the bases
+    become diag(Mv)^-1/2 and diag(Ms)^-1/2, the reduced leapfrog is the FOM's
+    in rescaled coordinates, and the ROM seismogram equals the FOM's within
+    1e-12 (the axis couplings where the energy pair differs from J_sp act on
+    velocities that stay zero)."""
+    mesh = os.path.join(MESHES, "fullMesh4x5")
+    mat = {"kind": "unilayer", "layer": {"density": [2000.0, 0.0], "velocity": [5000.0, 0.0]}}
+    base = {"general": {"meshDir": mesh, "dt": 1.0, "finalTime": 60.0, "checkNumericalDispersion": False,
+                        "checkCfl": False},
+            "io": {"seismogram": {"binary": False, "freq": 2, "receivers": [45.0, 90.0, 135.0]}},
+            "source": {"signal": {"kind": "ricker", "depth": 900.0, "angle": 45.0, "period": 40.0, "delay": 10.0}},
+            "material": mat}
+    ora = oracle.App(mesh, mat)
+    ref = ora.fom_run(1.0, 60, [(900.0, 45.0, 40.0, 10.0)], False, [45.0, 90.0, 135.0], 2)
+    for dof, n in (("vp", ora.nVp), ("sp", ora.nSp)):
+        with open(tmp_path / f"lsv_{dof}", "wb") as f:
+            np.array([n, n], dtype=np.uint64).tofile(f)
+            np.eye(n).T.ravel().tofile(f)
+    rom = dict(base)
+    rom["rom"] = {"projection": "energy",
+                  "velocity": {"numModes": ora.nVp, "modesFile": str(tmp_path / "lsv_vp")},
+                  "stress": {"numModes": ora.nSp, "modesFile": str(tmp_path / "lsv_sp")}}
+    rdir = tmp_path / "rom"
+    rdir.mkdir()
+    res = run_exe(rom, rdir)
+    assert res.returncode == 0, res.stderr + res.stdout[-3000:]
+    assert "energy-consistent" in res.stdout
+    seis = read_ascii(str(rdir / "seismogram_0"), False)
+    assert rel_max(seis, ref["seismogram"][0].T) <= 1e-12

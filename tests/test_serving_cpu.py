@@ -189,3 +189,19 @@ def test_least_work_wait_is_bounded_under_sustained_overload():
 def test_unknown_policy_is_refused():
     with pytest.raises(ValueError, match="admission policy"):
         serving.Server({8: make_engine(8, 6)}, hbm_budget=1e12, policy="lifo")
+
+
+def test_unkept_seismograms_are_discarded():
+    """keep(r) False: the loop retires into a scratch buffer (seis NULL) and
+    returns no seismograms for that request; kept requests are unaffected."""
+    clk = serving.SimClock()
+    engines = {8: make_engine(8, 16, clock=clk)}
+    reqs = [req(0, 8, 2, 20, 0.0, 1), req(1, 8, 3, 25, 0.0, 2)]
+    serve_all = serving.Server(engines, clock=clk, hbm_budget=1e12, chunk=5)
+    rep = serve_all.run(reqs, keep=lambda r: r.rid == 1)
+    assert rep["requests"] == 2 and reqs[0].seismograms == [] and len(reqs[1].seismograms) == 3
+    c2 = serving.SimClock()
+    alone = req(1, 8, 3, 25, 0.0, 2)
+    serving.Server({8: make_engine(8, 16, clock=c2)}, clock=c2, hbm_budget=1e12, chunk=5).run([alone])
+    for x, y in zip(reqs[1].seismograms, alone.seismograms):
+        np.testing.assert_array_equal(x, y)
