@@ -137,11 +137,13 @@ typedef struct {
                            or 4 steps: 1 or 2)                                        */
   int steps_per_launch; /* 0 = default: 6 on one-wavefront strips (cols_per_thread =
                            col_block <= 2, strip 64) when a launch fills the GPU, else 4
-                           (loaded two rows ahead on those strips); fused kernel: 1..6
+                           (loaded two rows ahead on those strips); fused kernel: 1..7
                            leapfrog steps per launch, the intermediate levels kept on
-                           chip (strip >= 2 x steps; 5 and 6 only on one-wavefront
+                           chip (strip >= 2 x steps; 5 to 7 only on one-wavefront
                            strips); a pack with a snapshot due at an intermediate step
-                           falls back to fewer steps per launch                       */
+                           falls back to fewer steps per launch; the last steps of a
+                           run (up to two 7-step launches on one-wavefront strips)
+                           go in as few, evenly split launches                         */
   int graph_steps;      /* 0 = default (64, rounded up to an even number of launches);
                            < 0: no graphs. Fused kernel without snapshots: whole chunks
                            of this many steps replay one captured HIP graph (kernels
@@ -401,7 +403,8 @@ int shaw_serve(const shaw_engine* engines, int n_engines, const shaw_serve_optio
 /* POD basis: the k leading left singular vectors of the snapshot matrix
  * S LayoutLeft [n x ns] (the lsv_<dof> file content) and ns singular values,
  * descending, in FP64 on the GPU by a deflated method of snapshots: each pass
- * eigen-decomposes the Gram matrix of the residual snapshots (Jacobi), keeps
+ * eigen-decomposes the Gram matrix of the residual snapshots (Householder
+ * tridiagonalization + implicit symmetric QR, on the host), keeps
  * the modes it resolves (sigma > 1e-3 of the residual's largest), orthogonal-
  * izes them (Cholesky) and projects them out of the residual. Fails if k
  * exceeds the numerical rank. Replaces computeThinSVD

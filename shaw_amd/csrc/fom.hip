@@ -1373,11 +1373,11 @@ class FomEngine final : public shaw_fom {
       } else {
         spl_ = pf_ != 1 ? 1 : (P_ >= 8 && fits32() ? 4 : (P_ >= 4 ? 2 : 1));
       }
-      SHAW_REQUIRE(spl_ >= 1 && spl_ <= kMaxSpl, SHAW_ERR_ARG, "steps_per_launch must be 1 to 6");
+      SHAW_REQUIRE(spl_ >= 1 && spl_ <= kMaxSpl, SHAW_ERR_ARG, "steps_per_launch must be 1 to 7");
       SHAW_REQUIRE(spl_ == 1 || P_ >= 2u * spl_, SHAW_ERR_ARG,
                    "k steps per launch need a strip of at least 2k points");
       SHAW_REQUIRE(spl_ <= 4 || (waveStrip(cpt_, P_) && pf_ == 1), SHAW_ERR_ARG,
-                   "5 or 6 steps per launch need one thread per point (cols_per_thread = col_block <= 2), "
+                   "5 to 7 steps per launch need one thread per point (cols_per_thread = col_block <= 2), "
                    "64-point strips and prefetch_rows 1");
       graphStepsOpt_ = o.graph_steps;
       graphSteps_ = graphStepsFor(spl_);
@@ -1517,22 +1517,29 @@ class FomEngine final : public shaw_fom {
     if (graphs) {
       ensureGraph();
       hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, stream_, stepDev_.p, step_);
-      for (; k + graphSteps_ <= nsteps; k += graphSteps_) {
+      // leave the last chunk to the eager packs when the remainder alone
+      // would be a pack of one or two steps
+      const uint64_t rem = nsteps % graphSteps_;
+      const uint64_t whole = nsteps / graphSteps_ - (rem > 0 && rem < 3 && nsteps / graphSteps_ > 0 ? 1 : 0);
+      for (; k + graphSteps_ <= whole * graphSteps_; k += graphSteps_) {
         SHAW_HIP(hipGraphLaunch(gexec_, stream_));
         step_ += graphSteps_;
       }
     }
-    // eager launches: the widest pack (up to spl_ steps) that fits the
-    // remaining steps and has no snapshot due at an intermediate step. A
-    // launch of the multi-step kernels costs about the same whatever its
-    // steps (its time follows the bytes it moves), so when the tail would
-    // end in a pack of one or two steps (which run the pair or single-step
-    // kernel, shaped for wide column blocks) the last two packs split the
-    // tail evenly instead: 8 steps at six per launch run 4 + 4, not 6 + 2.
+    // eager launches of spl_ steps; a launch of the multi-step kernels costs
+    // about the same whatever its steps (its time follows the bytes it
+    // moves: 1.50-1.57 ms at 1024 x 4096 for 3 to 6 steps), so the last
+    // steps (at most two of the longest packs) go in as few launches as
+    // possible, split evenly: 20 steps at six per launch run 6 + 7 + 7, 8
+    // run 4 + 4. No pack has a snapshot due at an intermediate step.
     for (; k < nsteps;) {
       const uint64_t left = nsteps - k;
       int ns = spl_;
-      if (spl_ >= 3 && left > (uint64_t)spl_ && left < (uint64_t)spl_ + 3) ns = (int)((left + 1) / 2);
+      if (spl_ >= 3) {
+        const uint64_t top = longestPack();
+        if (left <= top) ns = (int)left;                           // one launch
+        else if (left <= 2 * top) ns = (int)((left + 1) / 2);      // two, balanced
+      }
       while (ns > 1 && (k + ns > nsteps || packBlocked(ns))) --ns;
       switch (ns) {
         case 1: launchStep(); break;
@@ -1540,11 +1547,18 @@ class FomEngine final : public shaw_fom {
         case 3: launchPackN<3>(); break;
         case 4: launchPackN<4>(); break;
         case 5: launchPackN<5>(); break;
-        default: launchPackN<6>(); break;
+        case 6: launchPackN<6>(); break;
+        default: launchPackN<7>(); break;
       }
       k += ns;
     }
     SHAW_HIP(hipGetLastError());
+  }
+  // the most steps one launch of the current shape takes: at six steps per
+  // launch the wave kernel (one-wavefront strips, one row loaded ahead) also
+  // runs seven-step packs for the last steps of a run
+  uint64_t longestPack() const {
+    return (spl_ >= 6 && waveStrip(cpt_, P_) && pf_ == 1) ? (uint64_t)kMaxSpl : (uint64_t)spl_;
   }
   // a snapshot due at one of the intermediate steps of a pack of ns steps
   bool packBlocked(int ns) const {
@@ -1575,7 +1589,8 @@ class FomEngine final : public shaw_fom {
           case 3: captureN<3>(k); break;
           case 4: captureN<4>(k); break;
           case 5: captureN<5>(k); break;
-          default: captureN<6>(k); break;
+          case 6: captureN<6>(k); break;
+          default: captureN<7>(k); break;
         }
       } else {
         StepArgs<T> a = deviceStepArgs(k + 1);
@@ -1839,7 +1854,7 @@ class FomEngine final : public shaw_fom {
             for (int spl = 1; spl <= kMaxSpl; ++spl) {
               // the multi-step kernels load one row ahead, k_fusedW one or two
               if (spl >= 2 && pf == 2 && !(spl >= 3 && spl <= 4 && waveStrip(cpt, P))) continue;
-              if (spl >= 5 && !waveStrip(cpt, P)) continue;  // 5 and 6 steps: one-wavefront strips only
+              if (spl >= 5 && !waveStrip(cpt, P)) continue;  // 5 to 7 steps: one-wavefront strips only
               if (spl >= 2 && P < 2u * spl) continue;
               if (step_ + spl > ftabSteps_) continue;  // a trial reads the forcing of its steps
               if (graphStepsFor(spl) % (2 * spl)) continue;  // a requested graph length must fit
@@ -1923,7 +1938,8 @@ class FomEngine final : public shaw_fom {
         case 3: trialPackN<3>(); break;
         case 4: trialPackN<4>(); break;
         case 5: trialPackN<5>(); break;
-        default: trialPackN<6>(); break;
+        case 6: trialPackN<6>(); break;
+        default: trialPackN<7>(); break;
       }
     } else if (spl_ == 2) {
       StepArgs2<T> aa;
@@ -2026,7 +2042,7 @@ class FomEngine final : public shaw_fom {
       hipLaunchKernelGGL((k_fusedN<T, CPT, NS>), dim3(blocks), dim3(P_ * tpp), ldsBytesN(P_, NS), stream_, aa);
     } else {
       (void)tpp;
-      throw Error(SHAW_ERR_STATE, "5 or 6 steps per launch run only on one-wavefront strips");
+      throw Error(SHAW_ERR_STATE, "5 to 7 steps per launch run only on one-wavefront strips");
     }
     cur_ ^= 1;
     if (observe) {
@@ -2130,8 +2146,8 @@ class FomEngine final : public shaw_fom {
   hipGraphExec_t gexec_ = nullptr;
   int graphKey_[7] = {0, 0, 0, 0, 0, -1, 0};
   int graphSteps_ = 64, graphStepsOpt_ = 0;
-  uint32_t nStripsK_[7] = {0, 0, 0, 0, 0, 0, 0};  // strips of the k-step kernels (k = 2..6)
-  static constexpr int kMaxSpl = 6;
+  uint32_t nStripsK_[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // strips of the k-step kernels (k = 2..7)
+  static constexpr int kMaxSpl = 7;
   static constexpr size_t kMaxLds = 160 * 1024;
   static constexpr uint64_t kMinWaves = 1536;  // 6 per CU of the 256
   std::vector<double> rhoInv_;

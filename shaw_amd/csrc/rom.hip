@@ -9,7 +9,7 @@
 // torevise/run_rom_rank_two_forcing.hpp:149-188), for resident column slots:
 //   q_v += dt * (Ar_v q_s) ; q_v += (f dt) b ; y = C q_v (sample steps) ;
 //   q_s += dt * (Ar_s q_v)
-// A persistent kernel: one workgroup (8 waves) owns 16 column slots, keeps
+// A persistent kernel: one workgroup (8 or 16 waves) owns 16 column slots, keeps
 // their q_v, q_s in LDS in FP64 for the whole launch and runs many steps.
 // Each wave owns up to 4 row tiles of 16 rows of each operator; operators are
 // pre-packed in MFMA operand order as 32-byte lane items and stream from L2
@@ -49,7 +49,7 @@ typedef short shrt4 __attribute__((ext_vector_type(4)));
 typedef short shrt16 __attribute__((ext_vector_type(16)));
 
 constexpr int kTileCols = 16;  // column slots per workgroup
-constexpr int kWaves = 8;      // waves per ROM workgroup
+constexpr int kWaves = 8;      // waves per ROM workgroup (16 at 9..16 row tiles)
 
 // ------------------------------------------------------------ bf16 helpers
 __host__ __device__ inline unsigned short f32ToBf16(float x) {
@@ -286,16 +286,16 @@ __device__ __forceinline__ void tileProducts(const void* __restrict__ Araw, int 
 __device__ __forceinline__ int accRow(int parts, int lane, int r) { return parts ? rowBf16(lane, r) : rowF64(lane, r); }
 
 // Y[rows][cols] += dt * (A X) (+ (f dt) b for the velocity half), active
-// columns only. Row tiles rt0 + t * kWaves, t < TPW, belong to this wave.
-template <int PARTS, int TPW>
+// columns only. Row tiles rt0 + t * WAVES, t < TPW, belong to this wave.
+template <int PARTS, int TPW, int WAVES>
 __device__ __forceinline__ void halfStep(const RomArgs& a, const void* A, int rowTiles, int kGroups, const void* Xs,
                                          int loOff, double* Ys, int wave, int lane, bool active, double fdt,
                                          const double* bcol, int slot0) {
   const int col = lane % 16;
-  for (int rt0 = wave; rt0 < rowTiles; rt0 += kWaves * TPW) {
+  for (int rt0 = wave; rt0 < rowTiles; rt0 += WAVES * TPW) {
     int rt[TPW];
 #pragma unroll
-    for (int t = 0; t < TPW; ++t) rt[t] = rt0 + t * kWaves < rowTiles ? rt0 + t * kWaves : rt0;  // spare: recompute
+    for (int t = 0; t < TPW; ++t) rt[t] = rt0 + t * WAVES < rowTiles ? rt0 + t * WAVES : rt0;  // spare: recompute
     double bv[TPW][4];
     if (bcol)  // issued before the products, consumed after them
 #pragma unroll
@@ -306,7 +306,7 @@ __device__ __forceinline__ void halfStep(const RomArgs& a, const void* A, int ro
     tileProducts<PARTS, TPW>(A, kGroups, rt, Xs, loOff, lane, acc);
 #pragma unroll
     for (int t = 0; t < TPW; ++t)
-      if (rt0 + t * kWaves < rowTiles && active)
+      if (rt0 + t * WAVES < rowTiles && active)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = rt[t] * 16 + accRow(PARTS, lane, r);
@@ -317,20 +317,20 @@ __device__ __forceinline__ void halfStep(const RomArgs& a, const void* A, int ro
   }
 }
 
-template <int PARTS, int TPW>
+template <int PARTS, int TPW, int WAVES>
 __device__ __forceinline__ void receivers(const RomArgs& a, const void* Xv, int loOff, int wave, int lane, bool sample,
                                           uint32_t sampleIdx, int slot0) {
   const int rowTiles = a.Rp / 16, kGroups = PARTS ? a.Kvp / 32 : a.Kvp / 16;
-  for (int rt0 = wave; rt0 < rowTiles; rt0 += kWaves * TPW) {
+  for (int rt0 = wave; rt0 < rowTiles; rt0 += WAVES * TPW) {
     int rt[TPW];
 #pragma unroll
-    for (int t = 0; t < TPW; ++t) rt[t] = rt0 + t * kWaves < rowTiles ? rt0 + t * kWaves : rt0;
+    for (int t = 0; t < TPW; ++t) rt[t] = rt0 + t * WAVES < rowTiles ? rt0 + t * WAVES : rt0;
     typename Prec<PARTS>::Acc acc[TPW];
     tileProducts<PARTS, TPW>(a.c, kGroups, rt, Xv, loOff, lane, acc);
     const int col = lane % 16;
 #pragma unroll
     for (int t = 0; t < TPW; ++t)
-      if (rt0 + t * kWaves < rowTiles && sample)
+      if (rt0 + t * WAVES < rowTiles && sample)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = rt[t] * 16 + accRow(PARTS, lane, r);
@@ -341,7 +341,9 @@ __device__ __forceinline__ void receivers(const RomArgs& a, const void* Xv, int 
   }
 }
 
-// One workgroup (kWaves waves) owns kTileCols column slots and advances them
+// One workgroup (WAVES waves: 16 when the operators have 9 to 16 row tiles,
+// one tile per wave, which doubles the loads in flight per CU for the same
+// operator stream; else 8) owns kTileCols column slots and advances them
 // a.nsteps steps, states resident in LDS; each wave owns up to TPW row tiles
 // of each operator. Lane l works on column l % 16 in every product, so each
 // lane keeps its column's step counter, horizon, sampling period and next
@@ -349,8 +351,8 @@ __device__ __forceinline__ void receivers(const RomArgs& a, const void* Xv, int 
 // step s); the block-uniform decisions (stop, sample) come from values every
 // wave computes identically, so a step needs only the two barriers between
 // the half-steps.
-template <int PARTS, int TPW>
-__global__ void __launch_bounds__(kWaves * 64) k_rom_steps(RomArgs a) {
+template <int PARTS, int TPW, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) k_rom_steps(RomArgs a) {
   extern __shared__ double sm[];
   double* Qv = sm;                                   // [Kvp][16]
   double* Qs = Qv + (size_t)a.Kvp * kTileCols;       // [Ksp][16]
@@ -401,16 +403,16 @@ __global__ void __launch_bounds__(kWaves * 64) k_rom_steps(RomArgs a) {
     const bool anySample = (__ballot(smp) & 0xffffull) != 0;
     // velocity: q_v += dt Ar_v q_s ; q_v += (f dt) b
     const void* Xs = PARTS ? static_cast<const void*>(S) : static_cast<const void*>(Qs);
-    halfStep<PARTS, TPW>(a, a.av, kvTiles, kGroupsS, Xs, loOff, Qv, wave, lane, act, fdt, a.b, slot0);
+    halfStep<PARTS, TPW, WAVES>(a, a.av, kvTiles, kGroupsS, Xs, loOff, Qv, wave, lane, act, fdt, a.b, slot0);
     __syncthreads();
     if constexpr (PARTS > 0) {  // every read of the split of q_s is done
       splitState(Qv, (int)a.Kvp, S, loOff, tid, blockDim.x);
       __syncthreads();
     }
     const void* Xv = PARTS ? static_cast<const void*>(S) : static_cast<const void*>(Qv);
-    if (anySample) receivers<PARTS, TPW>(a, Xv, loOff, wave, lane, smp, nextIdx - 1, slot0);
+    if (anySample) receivers<PARTS, TPW, WAVES>(a, Xv, loOff, wave, lane, smp, nextIdx - 1, slot0);
     // stress: q_s += dt Ar_s q_v
-    halfStep<PARTS, TPW>(a, a.as, ksTiles, kGroupsV, Xv, loOff, Qs, wave, lane, act, 0.0, nullptr, slot0);
+    halfStep<PARTS, TPW, WAVES>(a, a.as, ksTiles, kGroupsV, Xv, loOff, Qs, wave, lane, act, 0.0, nullptr, slot0);
     __syncthreads();
     if constexpr (PARTS > 0) {
       if (it + 1 < steps) {
@@ -517,15 +519,16 @@ std::vector<double> packBf16(const double* A, uint64_t rows, uint64_t cols, uint
 
 template <int PARTS>
 void launchRomSteps(int tpw, dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a) {
-  auto go = [&](auto kernel) {
+  auto go = [&](auto kernel, int waves) {
     (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(kernel, grid, dim3(kWaves * 64), lds, stream, a);
+    hipLaunchKernelGGL(kernel, grid, dim3(waves * 64), lds, stream, a);
   };
   switch (tpw) {
-    case 1: go(k_rom_steps<PARTS, 1>); break;
-    case 2: go(k_rom_steps<PARTS, 2>); break;
-    case 3: go(k_rom_steps<PARTS, 3>); break;
-    default: go(k_rom_steps<PARTS, 4>); break;
+    case 0: go(k_rom_steps<PARTS, 1, 16>, 16); break;  // 9..16 row tiles: one per wave of 16
+    case 1: go(k_rom_steps<PARTS, 1, kWaves>, kWaves); break;
+    case 2: go(k_rom_steps<PARTS, 2, kWaves>, kWaves); break;
+    case 3: go(k_rom_steps<PARTS, 3, kWaves>, kWaves); break;
+    default: go(k_rom_steps<PARTS, 4, kWaves>, kWaves); break;
   }
 }
 
@@ -852,7 +855,10 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
     h->maxCols = roundUp(d->max_cols, kTileCols);
     h->maxSamples = d->max_samples;
     SHAW_REQUIRE(h->kvp <= 512 && h->ksp <= 512, SHAW_ERR_ARG, "ROM size must be at most 512 modes per dof");
-    h->tpw = (int)((std::max(h->kvp, h->ksp) / 16 + kWaves - 1) / kWaves);
+    {
+      const uint64_t tiles = std::max(h->kvp, h->ksp) / 16;
+      h->tpw = tiles > (uint64_t)kWaves && tiles <= 16 ? 0 : (int)((tiles + kWaves - 1) / kWaves);
+    }
     SHAW_REQUIRE(h->ldsBytes() <= 160 * 1024, SHAW_ERR_ARG, "ROM size too large for the LDS-resident state tile");
     SHAW_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     SHAW_HIP(hipEventCreate(&h->ev0));

@@ -36,10 +36,10 @@ def create(app, m, **kw):
     (4, dict(col_block=2, cols_per_thread=4), "cols_per_thread must divide the column block width"),
     (32, dict(col_block=8, cols_per_thread=1, strip=128), "strip too wide for a 512-thread block"),
     (4, dict(prefetch_rows=3), "prefetch_rows must be 1 or 2"),
-    (4, dict(steps_per_launch=7), "steps_per_launch must be 1 to 6"),
+    (4, dict(steps_per_launch=8), "steps_per_launch must be 1 to 7"),
     (4, dict(steps_per_launch=4, strip=6), "k steps per launch need a strip of at least 2k points"),
-    (32, dict(col_block=8, steps_per_launch=5), "5 or 6 steps per launch need one thread per point"),
-    (4, dict(steps_per_launch=6, strip=32), "5 or 6 steps per launch need one thread per point"),
+    (32, dict(col_block=8, steps_per_launch=5), "5 to 7 steps per launch need one thread per point"),
+    (4, dict(steps_per_launch=6, strip=32), "5 to 7 steps per launch need one thread per point"),
     (4, dict(steps_per_launch=3, graph_steps=8), "graph_steps must be a multiple of 2 x steps per launch"),
     (32, dict(col_block=32, cols_per_thread=8, strip=128, steps_per_launch=4), "tile needs more LDS"),
 ])

@@ -128,6 +128,8 @@ def mesh64x256(tmp_path_factory):
                                   (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=5)),
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=6, col_block=1, segment=9)),
                                   (40, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=6, segment=30)),
+                                  (32, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=7)),
+                                  (40, dict(kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=7, col_block=1, segment=9)),
                                   (40, dict(kernel=shaw_amd.KERNEL_SPLIT, cols_per_thread=2))])
 def test_random_rank2_bit_exact(mesh64x256, M, kw):
     say(f"[gpu] random rank-2 M={M} {kw}")
@@ -266,7 +268,28 @@ def test_autotune_keeps_state_and_parity(mesh64x256):
         np.testing.assert_array_equal(a[key], b[key], err_msg=key)
 
 
-@pytest.mark.parametrize("spl,gsteps", [(1, 8), (2, 8), (3, 12), (4, 8), (5, 10), (6, 12), (6, 0)])
+@pytest.mark.parametrize("n,snaps", [(n, False) for n in (1, 2, 7, 8, 13, 14, 15, 20, 27)] +
+                         [(n, True) for n in (13, 14, 20, 27)])
+def test_last_steps_pack_evenly_bit_exact(mesh64x256, n, snaps):
+    """Six steps per launch on one-wavefront strips: a run's last steps (up
+    to 14) go in one or two evenly split launches of up to seven steps
+    (20 = 6 + 7 + 7, 8 = 4 + 4); packs still never cross a snapshot. Equal
+    to the oracle bit for bit."""
+    say(f"[gpu] last-step packs n={n} snapshots={snaps}")
+    rng = np.random.default_rng(n)
+    app = shaw_amd.App.from_mesh(mesh64x256, LAYERED)
+    ora = oracle.App(mesh64x256, LAYERED)
+    src = random_sources(rng, 32)
+    rec = list(np.arange(0, 180, 7.5))
+    ev = (1, 5, 9) if snaps else (1, 0, 0)
+    a = shaw_amd.fom_run(app, 0.5, n, src, True, rec, *ev, kernel=shaw_amd.KERNEL_FUSED, steps_per_launch=6,
+                         graph_steps=-1)
+    b = ora.fom_run(0.5, n, src, True, rec, *ev)
+    for key in b:
+        np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+
+
+@pytest.mark.parametrize("spl,gsteps", [(1, 8), (2, 8), (3, 12), (4, 8), (5, 10), (6, 12), (6, 0), (7, 14)])
 def test_graph_replays_equal_eager(mesh64x256, spl, gsteps):
     """Chunks of gsteps steps (0: the default length) replay one captured HIP
     graph whose kernels read their step from a device counter. Checked against
@@ -289,7 +312,9 @@ def test_graph_replays_equal_eager(mesh64x256, spl, gsteps):
         return f
 
     g, e = make(gsteps), make(-1)
-    for chunk in (80 if gsteps == 0 else 30, 17, 8, 41):  # the default chunk is 64 steps or more
+    # the default chunk is 64 steps or more; 25 and 26 leave one or two
+    # steps over whole chunks of 12, which then go to the eager packs
+    for chunk in (80 if gsteps == 0 else 30, 17, 8, 41, 25, 26):
         g.advance(chunk)
         e.advance(chunk)
         for x, y in zip(g.state(), e.state()):
@@ -348,7 +373,8 @@ def test_default_segment_fills_the_chip():
 
 @pytest.mark.parametrize("kw", [dict(steps_per_launch=2), dict(steps_per_launch=3, col_block=32),
                                 dict(steps_per_launch=4), dict(steps_per_launch=4, cols_per_thread=2, strip=32),
-                                dict(steps_per_launch=6), dict(steps_per_launch=5, col_block=1, segment=9)])
+                                dict(steps_per_launch=6), dict(steps_per_launch=5, col_block=1, segment=9),
+                                dict(steps_per_launch=7)])
 def test_fp32_packs_equal_single_steps(mesh64x256, kw):
     """The FP32 state variant: k steps per launch == k single steps bit for bit
     (the same single-precision expressions in the same order)."""
