@@ -397,7 +397,7 @@ def config5_section(args) -> dict:
 
 
 def rom_kernel_only(args) -> int:
-    """The batched reduced leapfrog alone (k_rom_steps), for profiling: K x K
+    """The batched reduced leapfrog alone (k_rom_steps, k_rom_steps_bf), for profiling: K x K
     skew operator pairs (random, seeded), M columns, 180 receivers every 10
     steps, --rom-steps steps timed per precision with HIP events on the
     engine's stream. Prints one JSON line of forcing-steps/s and MFMA TFLOP/s."""

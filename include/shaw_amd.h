@@ -431,6 +431,8 @@ int shaw_reconstruct(const double* phi, uint64_t n, uint64_t k, const double* q,
  * checking the operand/result lane layouts the ROM kernels assume.
  * kind 0: v_mfma_f64_16x16x4f64, a[64], b[64] -> d[64][4];
  * kind 1: v_mfma_f32_16x16x16bf16_1k, a[64][4], b[64][4] (rounded to bf16)
+ *         -> d[64][4];
+ * kind 2: v_mfma_f32_16x16x32_bf16, a[64][8], b[64][8] (rounded to bf16)
  *         -> d[64][4]. No reference counterpart.                          */
 int shaw_debug_mfma(int kind, const double* a, const double* b, double* d);
 /* Host symmetric eigen-decomposition used by the POD (Householder

@@ -9,15 +9,16 @@
 // torevise/run_rom_rank_two_forcing.hpp:149-188), for resident column slots:
 //   q_v += dt * (Ar_v q_s) ; q_v += (f dt) b ; y = C q_v (sample steps) ;
 //   q_s += dt * (Ar_s q_v)
-// A persistent kernel: one workgroup (8 or 16 waves) owns 16 column slots, keeps
-// their q_v, q_s in LDS in FP64 for the whole launch and runs many steps.
-// Each wave owns up to 4 row tiles of 16 rows of each operator; operators are
-// pre-packed in MFMA operand order as 32-byte lane items and stream from L2
-// through a 4-deep register ring per row tile, so the loads of later k-steps
-// are in flight while the MFMAs of earlier ones run.
+// A persistent kernel: one workgroup (8 or 16 waves) owns 16 column slots and
+// runs many steps, their q_v, q_s kept in FP64 for the whole launch (in LDS
+// for FP64; for BF16 in the registers of the waves that update them, with
+// their bf16 splits in LDS). Each wave owns up to 4 row tiles of 16 rows of
+// each operator; operators are pre-packed in MFMA operand order as lane
+// items and stream from L2 through a register ring per row tile, so the
+// loads of later k-steps are in flight while the MFMAs of earlier ones run.
 //   FP64:   v_mfma_f64_16x16x4f64, FP64 accumulation.
 //   BF16x3: Ar = Ar_hi + Ar_lo and q = q_hi + q_lo in bf16; the product uses
-//           Ar_hi q_hi + Ar_hi q_lo + Ar_lo q_hi on v_mfma_f32_16x16x16bf16_1k
+//           Ar_hi q_hi + Ar_hi q_lo + Ar_lo q_hi on v_mfma_f32_16x16x32_bf16
 //           with FP32 accumulation; the state update q += dt * acc is FP64.
 //   BF16x4: as BF16x3 plus Ar_res q_hi, Ar_res = bf16(Ar - Ar_hi - Ar_lo):
 //           the operator is represented to ~2^-24 instead of ~2^-17. Over
@@ -113,9 +114,22 @@ __global__ void __launch_bounds__(256) k_gemm_nn_rows(const double* __restrict__
 //   A: lane l holds A[l % 16][4 * (l / 16) + e], e = 0..3
 //   B: lane l holds B[4 * (l / 16) + e][l % 16]
 //   D: lane l, register r holds D[4 * (l / 16) + r][l % 16]
+// v_mfma_f32_16x16x32_bf16 (gfx950; twice the bf16 products per instruction
+// of the 16x16x16 form):
+//   A: lane l holds A[l % 16][8 * (l / 16) + e], e = 0..7
+//   B: lane l holds B[8 * (l / 16) + e][l % 16]
+//   D: as 16x16x16
+// The ROM kernels rely only on A's value e of lane group l / 16 meeting B's
+// value e of the same lane group, and on D's layout.
 // These are checked on the device by shaw_debug_mfma (tests/test_gpu_rom.py).
+typedef int int4v __attribute__((ext_vector_type(4)));
+typedef short shrt8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 __device__ inline dbl4 mfmaF64(double a, double b, dbl4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ inline flt4 mfmaBf16(shrt4 a, shrt4 b, flt4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+__device__ inline flt4 mfmaBf16x32(int4v a, int4v b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
 // result row of register r in lane l
 __device__ inline int rowF64(int lane, int r) { return lane / 16 + 4 * r; }
 __device__ inline int rowBf16(int lane, int r) { return 4 * (lane / 16) + r; }
@@ -126,7 +140,7 @@ __global__ void k_mfma_probe(int kind, const double* __restrict__ a, const doubl
     dbl4 acc = {0, 0, 0, 0};
     acc = mfmaF64(a[l], b[l], acc);
     for (int r = 0; r < 4; ++r) d[4 * l + r] = acc[r];
-  } else {
+  } else if (kind == 1) {
     shrt4 x, y;
     for (int e = 0; e < 4; ++e) {
       x[e] = (short)f32ToBf16((float)a[4 * l + e]);
@@ -134,6 +148,15 @@ __global__ void k_mfma_probe(int kind, const double* __restrict__ a, const doubl
     }
     flt4 acc = {0, 0, 0, 0};
     acc = mfmaBf16(x, y, acc);
+    for (int r = 0; r < 4; ++r) d[4 * l + r] = acc[r];
+  } else {
+    shrt8 x, y;
+    for (int e = 0; e < 8; ++e) {
+      x[e] = (short)f32ToBf16((float)a[8 * l + e]);
+      y[e] = (short)f32ToBf16((float)b[8 * l + e]);
+    }
+    flt4 acc = {0, 0, 0, 0};
+    acc = mfmaBf16x32(__builtin_bit_cast(int4v, x), __builtin_bit_cast(int4v, y), acc);
     for (int r = 0; r < 4; ++r) d[4 * l + r] = acc[r];
   }
 }
@@ -144,9 +167,10 @@ __global__ void k_mfma_probe(int kind, const double* __restrict__ a, const doubl
 //   FP64:   32-byte items; group g of row tile t covers k-steps 4g..4g+3 of
 //           16 x 16 x 4; lane l holds A[16t + l%16][16g + 4s + l/16], s < 4.
 //   BF16x3 / BF16x4: 2 / 3 parts of 16 bytes (hi, lo, residual bf16 parts);
-//           group g covers k-steps 2g, 2g+1 of 16 x 16 x 16; part p of lane
-//           l holds that part of A[16t + l%16][32g + 16s + 4(l/16) + e] at
-//           position 4s + e, s < 2, e < 4.
+//           group g covers k = 32g .. 32g + 31, one 16 x 16 x 32 product per
+//           part; part p of lane l holds that part of
+//           A[16t + l%16][32g + 16s + 4(l/16) + e] at position 4s + e,
+//           s < 2, e < 4 (the state's split uses the same k order).
 // Padded sizes are multiples of kRing groups (64 for FP64, 128 for BF16x3).
 struct RomArgs {
   const void* av;       // Ar_v: rows Kvp, k-dim Ksp
@@ -169,9 +193,7 @@ constexpr int kRing = 4;  // operator groups kept in flight per row tile
 
 // PARTS = 0: FP64 operators; 2: bf16 hi + lo (BF16x3); 3: bf16 hi + lo +
 // residual (BF16x4). Ring items: a dbl4 (FP64) or PARTS 16-byte parts, part p
-// holding that part's 4 bf16 values of k-steps 2g and 2g+1 (32 B or 48 B).
-typedef int int4v __attribute__((ext_vector_type(4)));
-typedef short shrt8 __attribute__((ext_vector_type(8)));
+// holding that part's 8 bf16 values of group g (32 B or 48 B).
 template <int PARTS>
 struct BfItem {
   int4v p[PARTS];
@@ -188,9 +210,9 @@ struct Prec<0> {
 };
 
 // B operand of k-step group g. FP64: X is the state [K][16] in LDS. BF16:
-// X is the state's bf16 split (splitState): hi parts, then lo parts, each
-// [K/4][16] items of 4 bf16 values (k = 4 kq .. 4 kq + 3 of one column), so
-// a lane's operand of k-step s is one 8-byte read.
+// X is the state's bf16 split (writeSplitRows): hi parts, then lo parts, each
+// [K/32][4][16] items of 8 bf16 values in the operator's k order, so a
+// lane's operand of a group is one 16-byte read.
 template <int PARTS, int TPW>
 __device__ __forceinline__ void groupProducts(const typename Prec<PARTS>::Item (&ag)[TPW], int g, const void* X,
                                               int loOff, int lane, typename Prec<PARTS>::Acc (&acc)[TPW]) {
@@ -203,84 +225,78 @@ __device__ __forceinline__ void groupProducts(const typename Prec<PARTS>::Item (
       for (int t = 0; t < TPW; ++t) acc[t] = mfmaF64(ag[t][s], xb, acc[t]);
     }
   } else {
-    const shrt4* Xh = static_cast<const shrt4*>(X);
+    const int4v* Xh = static_cast<const int4v*>(X);
+    const int item = (4 * g + lane / 16) * kTileCols + lane % 16;
+    const int4v xh = Xh[item], xl = Xh[loOff + item];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int item = (4 * (2 * g + s) + lane / 16) * kTileCols + lane % 16;
-      const shrt4 xh = Xh[item], xl = Xh[loOff + item];
-#pragma unroll
-      for (int t = 0; t < TPW; ++t) {
-        const shrt8 w0 = __builtin_bit_cast(shrt8, ag[t].p[0]);
-        const shrt8 w1 = __builtin_bit_cast(shrt8, ag[t].p[1]);
-        const shrt4 ah = {w0[4 * s + 0], w0[4 * s + 1], w0[4 * s + 2], w0[4 * s + 3]};
-        const shrt4 al = {w1[4 * s + 0], w1[4 * s + 1], w1[4 * s + 2], w1[4 * s + 3]};
-        acc[t] = mfmaBf16(ah, xh, acc[t]);
-        acc[t] = mfmaBf16(ah, xl, acc[t]);
-        acc[t] = mfmaBf16(al, xh, acc[t]);
-        if constexpr (PARTS == 3) {
-          const shrt8 w2 = __builtin_bit_cast(shrt8, ag[t].p[2]);
-          const shrt4 ar = {w2[4 * s + 0], w2[4 * s + 1], w2[4 * s + 2], w2[4 * s + 3]};
-          acc[t] = mfmaBf16(ar, xh, acc[t]);
-        }
-      }
+    for (int t = 0; t < TPW; ++t) {
+      acc[t] = mfmaBf16x32(ag[t].p[0], xh, acc[t]);
+      acc[t] = mfmaBf16x32(ag[t].p[0], xl, acc[t]);
+      acc[t] = mfmaBf16x32(ag[t].p[1], xh, acc[t]);
+      if constexpr (PARTS == 3) acc[t] = mfmaBf16x32(ag[t].p[2], xh, acc[t]);
     }
   }
 }
 
-// the bf16 split of the state X [Kp][16] (FP64, LDS) into S: x = float(X),
-// hi = bf16(x), lo = bf16(x - hi), both rounded to nearest even; item
-// (k / 4) * 16 + c holds k % 4 = 0..3 of column c (layout of groupProducts)
-__device__ __forceinline__ void splitState(const double* X, int Kp, shrt4* S, int loOff, int tid, int nthreads) {
-  for (int e = tid; e < Kp * kTileCols / 4; e += nthreads) {
-    const int kq = e / kTileCols, c = e % kTileCols;
-    shrt4 h, l;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const float x = (float)X[(4 * kq + r) * kTileCols + c];
-      const unsigned short hb = f32ToBf16(x);
-      h[r] = (short)hb;
-      l[r] = (short)f32ToBf16(x - bf16ToF32(hb));
-    }
-    S[e] = h;
-    S[loOff + e] = l;
-  }
+// operator groups kept in flight per row tile: kRing, or 2 where the ring
+// of all the wave's tiles would not fit the registers (register-resident
+// states: beside one tile of 8 waves)
+template <int PARTS, int TPW, int WAVES, bool RESIDENT>
+constexpr int ringDepth() {
+  return RESIDENT ? (TPW == 1 && WAVES <= 8 ? kRing : 2) : (PARTS * TPW > 9 ? 2 : kRing);
 }
 
-template <int PARTS, int TPW>
-__device__ __forceinline__ void tileProducts(const void* __restrict__ Araw, int kGroups, const int (&rt)[TPW],
-                                             const void* X, int loOff, int lane, typename Prec<PARTS>::Acc (&acc)[TPW]) {
+// issue the loads of groups 0 .. D-1 of the wave's row tiles
+template <int PARTS, int TPW, int D>
+__device__ __forceinline__ void preloadRing(const void* __restrict__ Araw, int kGroups, const int (&rt)[TPW], int lane,
+                                            typename Prec<PARTS>::Item (&ring)[D][TPW]) {
   using Item = typename Prec<PARTS>::Item;
-  // BF16x4 with 4 tiles per wave: a 2-deep ring keeps the registers within
-  // the 256 of a 2-waves-per-SIMD workgroup (kGroups stays a multiple of it)
-  constexpr int depth = PARTS * TPW > 9 ? 2 : kRing;
   const Item* A = reinterpret_cast<const Item*>(Araw);
-  Item ring[depth][TPW];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) ring[d][t] = A[((size_t)rt[t] * kGroups + d) * 64 + lane];
+    __builtin_amdgcn_sched_barrier(0);  // issue order = consumption order
+  }
+}
+
+// acc[t] = (row tile rt[t] of A) X over all kGroups groups; the ring holds
+// groups 0 .. D-1 on entry (preloadRing) and is refilled D groups ahead
+template <int PARTS, int TPW, int D>
+__device__ __forceinline__ void consumeRing(const void* __restrict__ Araw, int kGroups, const int (&rt)[TPW],
+                                            const void* X, int loOff, int lane,
+                                            typename Prec<PARTS>::Item (&ring)[D][TPW],
+                                            typename Prec<PARTS>::Acc (&acc)[TPW]) {
+  using Item = typename Prec<PARTS>::Item;
+  const Item* A = reinterpret_cast<const Item*>(Araw);
   const Item* row[TPW];
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
     acc[t] = typename Prec<PARTS>::Acc{0, 0, 0, 0};
     row[t] = A + (size_t)rt[t] * kGroups * 64 + lane;
   }
-#pragma unroll
-  for (int d = 0; d < depth; ++d) {
-#pragma unroll
-    for (int t = 0; t < TPW; ++t) ring[d][t] = row[t][d * 64];
-    __builtin_amdgcn_sched_barrier(0);  // issue order = consumption order
-  }
   int g0 = 0;
-  for (; g0 + depth < kGroups; g0 += depth) {
+  for (; g0 + D < kGroups; g0 += D) {
 #pragma unroll
-    for (int d = 0; d < depth; ++d) {
+    for (int d = 0; d < D; ++d) {
       groupProducts<PARTS, TPW>(ring[d], g0 + d, X, loOff, lane, acc);
 #pragma unroll
-      for (int t = 0; t < TPW; ++t) ring[d][t] = row[t][(g0 + d + depth) * 64];
+      for (int t = 0; t < TPW; ++t) ring[d][t] = row[t][(g0 + d + D) * 64];
       // keep this group's refill here: the scheduler would otherwise sink all
       // refills of the round to its end, shortening the prefetch distance
       __builtin_amdgcn_sched_barrier(0);
     }
   }
 #pragma unroll
-  for (int d = 0; d < depth; ++d) groupProducts<PARTS, TPW>(ring[d], g0 + d, X, loOff, lane, acc);
+  for (int d = 0; d < D; ++d) groupProducts<PARTS, TPW>(ring[d], g0 + d, X, loOff, lane, acc);
+}
+
+template <int PARTS, int TPW, int D = ringDepth<PARTS, TPW, kWaves, false>()>
+__device__ __forceinline__ void tileProducts(const void* __restrict__ Araw, int kGroups, const int (&rt)[TPW],
+                                             const void* X, int loOff, int lane, typename Prec<PARTS>::Acc (&acc)[TPW]) {
+  typename Prec<PARTS>::Item ring[D][TPW];
+  preloadRing<PARTS, TPW, D>(Araw, kGroups, rt, lane, ring);
+  consumeRing<PARTS, TPW, D>(Araw, kGroups, rt, X, loOff, lane, ring, acc);
 }
 
 __device__ __forceinline__ int accRow(int parts, int lane, int r) { return parts ? rowBf16(lane, r) : rowF64(lane, r); }
@@ -317,7 +333,7 @@ __device__ __forceinline__ void halfStep(const RomArgs& a, const void* A, int ro
   }
 }
 
-template <int PARTS, int TPW, int WAVES>
+template <int PARTS, int TPW, int WAVES, int D = ringDepth<PARTS, TPW, kWaves, false>()>
 __device__ __forceinline__ void receivers(const RomArgs& a, const void* Xv, int loOff, int wave, int lane, bool sample,
                                           uint32_t sampleIdx, int slot0) {
   const int rowTiles = a.Rp / 16, kGroups = PARTS ? a.Kvp / 32 : a.Kvp / 16;
@@ -326,7 +342,7 @@ __device__ __forceinline__ void receivers(const RomArgs& a, const void* Xv, int 
 #pragma unroll
     for (int t = 0; t < TPW; ++t) rt[t] = rt0 + t * WAVES < rowTiles ? rt0 + t * WAVES : rt0;
     typename Prec<PARTS>::Acc acc[TPW];
-    tileProducts<PARTS, TPW>(a.c, kGroups, rt, Xv, loOff, lane, acc);
+    tileProducts<PARTS, TPW, D>(a.c, kGroups, rt, Xv, loOff, lane, acc);
     const int col = lane % 16;
 #pragma unroll
     for (int t = 0; t < TPW; ++t)
@@ -341,11 +357,11 @@ __device__ __forceinline__ void receivers(const RomArgs& a, const void* Xv, int 
   }
 }
 
-// One workgroup (WAVES waves: 16 when the operators have 9 to 16 row tiles,
-// one tile per wave, which doubles the loads in flight per CU for the same
-// operator stream; else 8) owns kTileCols column slots and advances them
-// a.nsteps steps, states resident in LDS; each wave owns up to TPW row tiles
-// of each operator. Lane l works on column l % 16 in every product, so each
+// The FP64 reduced leapfrog. One workgroup (WAVES waves: 16 when the
+// operators have 9 to 16 row tiles, one tile per wave, which doubles the
+// loads in flight per CU for the same operator stream; else 8) owns
+// kTileCols column slots and advances them a.nsteps steps, states resident
+// in LDS; each wave owns up to TPW row tiles of each operator. Lane l works on column l % 16 in every product, so each
 // lane keeps its column's step counter, horizon, sampling period and next
 // forcing value in registers (the forcing of step s + 1 is loaded during
 // step s); the block-uniform decisions (stop, sample) come from values every
@@ -353,12 +369,10 @@ __device__ __forceinline__ void receivers(const RomArgs& a, const void* Xv, int 
 // the half-steps.
 template <int PARTS, int TPW, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64) k_rom_steps(RomArgs a) {
+  static_assert(PARTS == 0, "the bf16 variants run k_rom_steps_bf");
   extern __shared__ double sm[];
   double* Qv = sm;                                   // [Kvp][16]
   double* Qs = Qv + (size_t)a.Kvp * kTileCols;       // [Ksp][16]
-  // BF16: the bf16 split of the half-step's input state (hi items, then lo)
-  shrt4* S = reinterpret_cast<shrt4*>(Qs + (size_t)a.Ksp * kTileCols);
-  const int loOff = (int)(a.Kvp > a.Ksp ? a.Kvp : a.Ksp) * kTileCols / 4;
   const int slot0 = blockIdx.x * kTileCols;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64, col = lane % 16;
   for (int e = tid; e < (int)a.Kvp * kTileCols; e += blockDim.x) {
@@ -384,12 +398,8 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_steps(RomArgs a) {
   }
   if ((uint64_t)steps > a.nsteps) steps = (uint32_t)a.nsteps;
   __syncthreads();  // the states are in LDS
-  if constexpr (PARTS > 0) {
-    splitState(Qs, (int)a.Ksp, S, loOff, tid, blockDim.x);
-    __syncthreads();
-  }
   const int kvTiles = a.Kvp / 16, ksTiles = a.Ksp / 16;
-  const int kGroupsS = PARTS ? a.Ksp / 32 : a.Ksp / 16, kGroupsV = PARTS ? a.Kvp / 32 : a.Kvp / 16;
+  const int kGroupsS = a.Ksp / 16, kGroupsV = a.Kvp / 16;
   double fcur = s0 < hz ? fcol[s0] : 0.0;
   uint32_t phase = ev ? (s0 + 1) % ev : 0;     // (s + 1) mod every
   uint32_t nextIdx = ev ? (s0 + 1) / ev : 0;   // index of the next sample
@@ -402,24 +412,12 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_steps(RomArgs a) {
     // lanes 0..15 of every wave hold columns 0..15: the same answer in all waves
     const bool anySample = (__ballot(smp) & 0xffffull) != 0;
     // velocity: q_v += dt Ar_v q_s ; q_v += (f dt) b
-    const void* Xs = PARTS ? static_cast<const void*>(S) : static_cast<const void*>(Qs);
-    halfStep<PARTS, TPW, WAVES>(a, a.av, kvTiles, kGroupsS, Xs, loOff, Qv, wave, lane, act, fdt, a.b, slot0);
+    halfStep<PARTS, TPW, WAVES>(a, a.av, kvTiles, kGroupsS, Qs, 0, Qv, wave, lane, act, fdt, a.b, slot0);
     __syncthreads();
-    if constexpr (PARTS > 0) {  // every read of the split of q_s is done
-      splitState(Qv, (int)a.Kvp, S, loOff, tid, blockDim.x);
-      __syncthreads();
-    }
-    const void* Xv = PARTS ? static_cast<const void*>(S) : static_cast<const void*>(Qv);
-    if (anySample) receivers<PARTS, TPW, WAVES>(a, Xv, loOff, wave, lane, smp, nextIdx - 1, slot0);
+    if (anySample) receivers<PARTS, TPW, WAVES>(a, Qv, 0, wave, lane, smp, nextIdx - 1, slot0);
     // stress: q_s += dt Ar_s q_v
-    halfStep<PARTS, TPW, WAVES>(a, a.as, ksTiles, kGroupsV, Xv, loOff, Qs, wave, lane, act, 0.0, nullptr, slot0);
+    halfStep<PARTS, TPW, WAVES>(a, a.as, ksTiles, kGroupsV, Qv, 0, Qs, wave, lane, act, 0.0, nullptr, slot0);
     __syncthreads();
-    if constexpr (PARTS > 0) {
-      if (it + 1 < steps) {
-        splitState(Qs, (int)a.Ksp, S, loOff, tid, blockDim.x);
-        __syncthreads();
-      }
-    }
     fcur = fnext;
     if (ev && act) {
       phase = phase + 1 == ev ? 0 : phase + 1;
@@ -434,6 +432,186 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_steps(RomArgs a) {
     const int row = e / kTileCols, c = e % kTileCols;
     a.qs[(uint64_t)(slot0 + c) * a.Ksp + row] = Qs[e];
   }
+  if (tid < kTileCols) {
+    const uint64_t h = a.horizon[slot0 + tid];
+    const uint64_t st = a.stepCol[slot0 + tid], s = st + a.nsteps;
+    a.stepCol[slot0 + tid] = s < h ? s : (st < h ? h : st);
+  }
+}
+
+// the bf16 split of the 4 state rows a lane holds after the products of row
+// tile rt (rows 16 rt + 4 (lane / 16) + r, column lane % 16): x = float(q),
+// hi = bf16(x), lo = bf16(x - hi), both rounded to nearest even. Item
+// (4 g + q') * 16 + c of the split holds, at position 4 s + e, k = 32 g +
+// 16 s + 4 q' + e of column c (the operator's k order, see the packing
+// above); here g = rt / 2, s = rt % 2, q' = lane / 16, e = r
+__device__ __forceinline__ void writeSplitRows(int4v* S, int loOff, int rt, int lane, const double (&q)[4]) {
+  shrt4 h, l;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const float x = (float)q[r];
+    const unsigned short hb = f32ToBf16(x);
+    h[r] = (short)hb;
+    l[r] = (short)f32ToBf16(x - bf16ToF32(hb));
+  }
+  const int item = (4 * (rt / 2) + lane / 16) * kTileCols + lane % 16;
+  *(reinterpret_cast<shrt4*>(S + item) + rt % 2) = h;
+  *(reinterpret_cast<shrt4*>(S + loOff + item) + rt % 2) = l;
+}
+
+// The BF16x3 / BF16x4 reduced leapfrog with register-resident states. As
+// k_rom_steps, one workgroup advances kTileCols column slots; here the FP64
+// states live in the registers of the wave that computes them (lane l holds
+// rows 16 rt + 4 (l / 16) + r of column l % 16 of each of its row tiles, the
+// MFMA result layout), and LDS holds only the bf16 splits of q_v and q_s,
+// each written by its owning waves right after their update. A step is then
+// two barriers: velocity products (read the split of q_s) -> update q_v,
+// write its split -> barrier -> stress products (read the split of q_v) ->
+// update q_s, write its split -> barrier. The first operator groups of the
+// next half-step are loaded before each barrier, so the barrier does not
+// drain the operator stream. Every value equals k_rom_steps' (the same
+// splits, products and FP64 updates in the same order).
+template <int PARTS, int TPW, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) k_rom_steps_bf(RomArgs a) {
+  // tiles per pass: at three or four tiles a wave the products run in passes
+  // of one or two tiles, whose rings fit the registers beside the states
+  // (K = 384 and 512, M = 4096: BF16x4 1.77e8 -> 2.06e8 and 7.6e7 -> 1.19e8,
+  // BF16x3 3.02e8 -> 3.11e8 and 1.55e8 -> 1.72e8 forcing-steps/s)
+  constexpr int TPP = TPW == 4 ? 2 : (TPW == 3 ? 1 : TPW), NP = TPW / TPP;
+  constexpr int D = ringDepth<PARTS, TPP, WAVES, true>();
+  using Item = typename Prec<PARTS>::Item;
+  using Acc = typename Prec<PARTS>::Acc;
+  extern __shared__ double sm[];
+  // splits: hi items, then lo items, 2 K items of 16 bytes each per state
+  int4v* Sv = reinterpret_cast<int4v*>(sm);
+  int4v* Ss = Sv + 4 * (size_t)a.Kvp;
+  const int loV = 2 * (int)a.Kvp, loS = 2 * (int)a.Ksp;
+  const int slot0 = blockIdx.x * kTileCols;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64, col = lane % 16;
+  const int kvTiles = a.Kvp / 16, ksTiles = a.Ksp / 16;
+  const int kGroupsS = a.Ksp / 32, kGroupsV = a.Kvp / 32;
+  // this wave's row tiles (the host launches WAVES * TPW >= both tile counts);
+  // a spare slot repeats tile `wave` and neither stores nor writes a split
+  int rtV[TPW], rtS[TPW];
+  bool okV[TPW], okS[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    okV[t] = wave + t * WAVES < kvTiles;
+    okS[t] = wave + t * WAVES < ksTiles;
+    rtV[t] = okV[t] ? wave + t * WAVES : (wave < kvTiles ? wave : 0);
+    rtS[t] = okS[t] ? wave + t * WAVES : (wave < ksTiles ? wave : 0);
+  }
+  int rtVp[NP][TPP], rtSp[NP][TPP];  // the tiles of each pass
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int j = 0; j < TPP; ++j) {
+      rtVp[p][j] = rtV[p * TPP + j];
+      rtSp[p][j] = rtS[p * TPP + j];
+    }
+  const uint64_t colV = (uint64_t)(slot0 + col) * a.Kvp, colS = (uint64_t)(slot0 + col) * a.Ksp;
+  double qv[TPW][4], qs[TPW][4];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rowV = rtV[t] * 16 + rowBf16(lane, r), rowS = rtS[t] * 16 + rowBf16(lane, r);
+      qv[t][r] = okV[t] ? a.qv[colV + rowV] : 0.0;
+      qs[t][r] = okS[t] ? a.qs[colS + rowS] : 0.0;
+    }
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    if (okV[t]) writeSplitRows(Sv, loV, rtV[t], lane, qv[t]);
+    if (okS[t]) writeSplitRows(Ss, loS, rtS[t], lane, qs[t]);
+  }
+  const uint32_t s0 = (uint32_t)a.stepCol[slot0 + col];
+  const uint32_t hz = (uint32_t)a.horizon[slot0 + col];
+  const uint32_t ev = (uint32_t)a.every[slot0 + col];
+  const double* fcol = a.ftab + (uint64_t)(slot0 + col) * a.maxSteps;
+  uint32_t steps = hz > s0 ? hz - s0 : 0;
+#pragma unroll
+  for (int off = 32; off >= 1; off /= 2) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)steps, off);
+    steps = o > steps ? o : steps;
+  }
+  if ((uint64_t)steps > a.nsteps) steps = (uint32_t)a.nsteps;
+  Item ring[D][TPP];
+  Acc acc[TPP];
+  if (steps) preloadRing<PARTS, TPP, D>(a.av, kGroupsS, rtVp[0], lane, ring);
+  __syncthreads();  // both splits are in LDS
+  double fcur = s0 < hz ? fcol[s0] : 0.0;
+  uint32_t phase = ev ? (s0 + 1) % ev : 0;
+  uint32_t nextIdx = ev ? (s0 + 1) / ev : 0;
+  for (uint32_t it = 0; it < steps; ++it) {
+    const uint32_t s = s0 + it;
+    const bool act = s < hz;
+    const double fnext = act && s + 1 < hz ? fcol[s + 1] : 0.0;
+    const double fdt = act ? fcur * a.dt : 0.0;
+    const bool smp = act && ev > 0 && phase == 0;
+    const bool anySample = (__ballot(smp) & 0xffffull) != 0;
+    // velocity: q_v += dt Ar_v q_s ; q_v += (f dt) b
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      consumeRing<PARTS, TPP, D>(a.av, kGroupsS, rtVp[p], Ss, loS, lane, ring, acc);
+      double bv[TPP][4];  // b stays in L1/L2; loaded here to keep it out of the products' registers
+#pragma unroll
+      for (int j = 0; j < TPP; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bv[j][r] = a.b[colV + rtVp[p][j] * 16 + rowBf16(lane, r)];
+      if (p + 1 < NP) preloadRing<PARTS, TPP, D>(a.av, kGroupsS, rtVp[p + 1 < NP ? p + 1 : 0], lane, ring);
+      else preloadRing<PARTS, TPP, D>(a.as, kGroupsV, rtSp[0], lane, ring);
+#pragma unroll
+      for (int j = 0; j < TPP; ++j) {
+        const int t = p * TPP + j;
+        if (okV[t]) {
+          if (act)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              double y = qv[t][r] + a.dt * (double)acc[j][r];
+              qv[t][r] = y + fdt * bv[j][r];
+            }
+          writeSplitRows(Sv, loV, rtV[t], lane, qv[t]);
+        }
+      }
+    }
+    __syncthreads();
+    // stress: q_s += dt Ar_s q_v
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      consumeRing<PARTS, TPP, D>(a.as, kGroupsV, rtSp[p], Sv, loV, lane, ring, acc);
+      if (p + 1 < NP) {
+        preloadRing<PARTS, TPP, D>(a.as, kGroupsV, rtSp[p + 1 < NP ? p + 1 : 0], lane, ring);
+      } else {
+        // one receiver tile at a time with a short ring: sample steps are
+        // rare, and the states stay in registers across them
+        if (anySample) receivers<PARTS, 1, WAVES, 2>(a, Sv, loV, wave, lane, smp, nextIdx - 1, slot0);
+        if (it + 1 < steps) preloadRing<PARTS, TPP, D>(a.av, kGroupsS, rtVp[0], lane, ring);
+      }
+#pragma unroll
+      for (int j = 0; j < TPP; ++j) {
+        const int t = p * TPP + j;
+        if (okS[t]) {
+          if (act)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) qs[t][r] = qs[t][r] + a.dt * (double)acc[j][r];
+          writeSplitRows(Ss, loS, rtS[t], lane, qs[t]);
+        }
+      }
+    }
+    __syncthreads();
+    fcur = fnext;
+    if (ev && act) {
+      phase = phase + 1 == ev ? 0 : phase + 1;
+      if (phase == 0) ++nextIdx;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (okV[t]) a.qv[colV + rtV[t] * 16 + rowBf16(lane, r)] = qv[t][r];
+      if (okS[t]) a.qs[colS + rtS[t] * 16 + rowBf16(lane, r)] = qs[t][r];
+    }
   if (tid < kTileCols) {
     const uint64_t h = a.horizon[slot0 + tid];
     const uint64_t st = a.stepCol[slot0 + tid], s = st + a.nsteps;
@@ -517,18 +695,24 @@ std::vector<double> packBf16(const double* A, uint64_t rows, uint64_t cols, uint
   return out;
 }
 
+template <int PARTS, int TPW, int WAVES>
+void launchRomKernel(dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a) {
+  auto go = [&](auto kernel) {
+    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kernel, grid, dim3(WAVES * 64), lds, stream, a);
+  };
+  if constexpr (PARTS == 0) go(k_rom_steps<0, TPW, WAVES>);
+  else go(k_rom_steps_bf<PARTS, TPW, WAVES>);
+}
+
 template <int PARTS>
 void launchRomSteps(int tpw, dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a) {
-  auto go = [&](auto kernel, int waves) {
-    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(kernel, grid, dim3(waves * 64), lds, stream, a);
-  };
   switch (tpw) {
-    case 0: go(k_rom_steps<PARTS, 1, 16>, 16); break;  // 9..16 row tiles: one per wave of 16
-    case 1: go(k_rom_steps<PARTS, 1, kWaves>, kWaves); break;
-    case 2: go(k_rom_steps<PARTS, 2, kWaves>, kWaves); break;
-    case 3: go(k_rom_steps<PARTS, 3, kWaves>, kWaves); break;
-    default: go(k_rom_steps<PARTS, 4, kWaves>, kWaves); break;
+    case 0: launchRomKernel<PARTS, 1, 16>(grid, lds, stream, a); break;  // 9..16 row tiles: one per wave of 16
+    case 1: launchRomKernel<PARTS, 1, kWaves>(grid, lds, stream, a); break;
+    case 2: launchRomKernel<PARTS, 2, kWaves>(grid, lds, stream, a); break;
+    case 3: launchRomKernel<PARTS, 3, kWaves>(grid, lds, stream, a); break;
+    default: launchRomKernel<PARTS, 4, kWaves>(grid, lds, stream, a); break;
   }
 }
 
@@ -706,11 +890,11 @@ struct shaw_rom {
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
   }
-  // LDS of one workgroup: the FP64 states, plus for BF16 the hi/lo split of
-  // the larger state (4 bytes an element)
+  // LDS of one workgroup: FP64, the states (8 bytes an element); BF16, the
+  // hi/lo splits of both states (4 bytes an element; the FP64 states are in
+  // registers)
   size_t ldsBytes() const {
-    const size_t st = (size_t)(kvp + ksp) * shaw::kTileCols * sizeof(double);
-    return precision == SHAW_ROM_FP64 ? st : st + (size_t)std::max(kvp, ksp) * shaw::kTileCols * 4;
+    return (size_t)(kvp + ksp) * shaw::kTileCols * (precision == SHAW_ROM_FP64 ? sizeof(double) : 4);
   }
   void launch(uint64_t nsteps) {
     using namespace shaw;
@@ -1147,8 +1331,8 @@ int shaw_debug_symeig(int n, const double* a, double* w, double* v) {
 int shaw_debug_mfma(int kind, const double* a, const double* b, double* d) {
   using namespace shaw;
   return guarded([&] {
-    SHAW_REQUIRE(a && b && d && (kind == 0 || kind == 1), SHAW_ERR_ARG, "bad argument");
-    const size_t na = kind == 0 ? 64 : 256;
+    SHAW_REQUIRE(a && b && d && kind >= 0 && kind <= 2, SHAW_ERR_ARG, "bad argument");
+    const size_t na = kind == 0 ? 64 : kind == 1 ? 256 : 512;
     DevBuf<double> da, db, dd;
     da.alloc(na); db.alloc(na); dd.alloc(256);
     da.upload(a, na); db.upload(b, na);

@@ -57,6 +57,22 @@ def test_mfma_bf16_layout():
     np.testing.assert_allclose(d, exp, rtol=1e-5, atol=1e-5)
 
 
+def test_mfma_bf16_k32_layout():
+    """v_mfma_f32_16x16x32_bf16: lane l holds A[l % 16][8 (l / 16) + e] and
+    B[8 (l / 16) + e][l % 16], e < 8; D as the 16x16x16 form."""
+    rng = np.random.default_rng(2)
+    a = bf16_round(rng.standard_normal((64, 8)))
+    b = bf16_round(rng.standard_normal((64, 8)))
+    d = shaw_amd.debug_mfma(2, a, b)
+    A = np.array([[a[i + 16 * (k // 8), k % 8] for k in range(32)] for i in range(16)])
+    B = np.array([[b[j + 16 * (k // 8), k % 8] for j in range(16)] for k in range(32)])
+    D = A @ B
+    exp = np.array([[D[4 * (l // 16) + r, l % 16] for r in range(4)] for l in range(64)])
+    if not np.allclose(d, exp, rtol=1e-5, atol=1e-5):
+        say(f"[gpu] bf16 k32 mfma raw output:\n{d}")
+    np.testing.assert_allclose(d, exp, rtol=1e-5, atol=1e-5)
+
+
 @pytest.fixture(scope="module")
 def pod21():
     """A POD basis of the 21 x 51 mesh from oracle FOM snapshots (numpy SVD)."""
