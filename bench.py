@@ -115,7 +115,7 @@ def rom_section(app, w, fom_value, args) -> dict:
         arv, ars = rw.operators(app, basis, 256)
         t2 = time.time()
         B, ftab = rw.basis_forcing_inputs(app, basis, 256, test, w.dt, n)
-        C = rw.receiver_rows(app, basis.phi_v, workloads.RECEIVERS)
+        C = rw.receiver_rows(app, basis, workloads.RECEIVERS)
         t3 = time.time()
         res, rates = {}, {}
         for prec, name in ((shaw_amd.ROM_FP64, "fp64"), (shaw_amd.ROM_BF16X3, "bf16x3"),
@@ -185,7 +185,7 @@ def rom_err_section(app, w, fom_value, args):
     periods = np.random.default_rng(640).uniform(20.0, 70.0, 1024)
     test = [(depth, 0.0, float(pp), workloads.DELAY_S) for pp in periods]
     B, ftab = rw.basis_forcing_inputs(app, basis, K, test, w.dt, n)
-    C = rw.receiver_rows(app, basis.phi_v[:, :K], workloads.RECEIVERS)
+    C = rw.receiver_rows(app, basis, workloads.RECEIVERS, K)
     setup = time.time() - t0
     res, rates = {}, {}
     for prec, name in ((shaw_amd.ROM_FP64, "fp64"), (shaw_amd.ROM_BF16X3, "bf16x3"), (shaw_amd.ROM_BF16X4, "bf16x4")):
@@ -234,14 +234,14 @@ def serving_section(app, w, basis, args) -> dict:
     depth = 640.0
     _, _, rho_inv, _ = app.operators()
     g = app.map_source(depth, 0.0)
-    C512 = rw.receiver_rows(app, basis.phi_v, workloads.RECEIVERS)
+    C512 = rw.receiver_rows(app, basis, workloads.RECEIVERS)
     ops, engines = {}, {}
     for k in (128, 256, 512):
         ops[k] = rw.operators(app, basis, k)
         rom = shaw_amd.Rom(ops[k][0], ops[k][1], w.dt, shaw_amd.ROM_BF16X4, C512[:, :k], max_cols=4096,
                            max_samples=2000)
         engines[k] = serving.GpuEngine(rom, 4096)
-    bcol = basis.mv[g] * rho_inv[g] * basis.phi_v[g, :]
+    bcol = basis.mv[g] * rho_inv[g] * basis.dev.rows("vp", [g])[0]
     setup = time.time() - t0
     mean_work = 128.5 * 12500.0  # E[M] E[horizon] forcing-steps per request
 

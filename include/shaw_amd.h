@@ -285,6 +285,37 @@ int shaw_rom_project_noalloc(uint64_t nr, uint64_t nth, const double* coef_vp, c
 int shaw_basis_orthonormalize(const double* phi, uint64_t n, uint64_t k, const double* w, double* phi_out,
                               double* r_out);
 
+/* Device-resident POD bases. The reference's offline chain moves each basis
+ * through files (computeThinSVD writes lsv_<dof>, rom_load_basis.hpp:56-101
+ * reads it back, rom_compute_jacobians.hpp:58-109 projects it); here the
+ * bases of both dofs stay in HBM from the POD to the projection, and only
+ * what the ROM needs (Ar, the rows at source and receiver gids) comes to
+ * the host. At config 3's size a basis pair is 26 GB, which a host round
+ * trip moves twice over PCIe.
+ * shaw_fom_pod_basis: kv and ks modes of the FOM handle's snapshot buffers
+ * (both consumed), Euclidean as shaw_fom_pod, or with wv [nvp], ws [nsp]
+ * (both or neither) as shaw_fom_pod_weighted; equal to those bit for bit.
+ * The first j columns of a basis are its j-mode basis (POD bases nest). */
+typedef struct shaw_basis* shaw_basis_t;
+int shaw_fom_pod_basis(shaw_fom_t h, uint64_t kv, uint64_t ks, const double* wv, const double* ws,
+                       shaw_basis_t* out);
+int shaw_basis_info(shaw_basis_t b, uint64_t* n_vp, uint64_t* n_sp, uint64_t* kv, uint64_t* ks,
+                    uint64_t* nsnap_vp, uint64_t* nsnap_sp);
+/* the snapshot matrix's singular values of dof (0 vp, 1 sp): nsnap values */
+int shaw_basis_sigma(shaw_basis_t b, int dof, double* sigma);
+/* out [m][k] row-major: the first k modes at grid points gids[m] (the rows
+ * rom_problem_rank_one.hpp:133-147 and main_reconstruct_seismo.cc:13-41
+ * take from Phi_v for the forcing vector and the receivers)               */
+int shaw_basis_rows(shaw_basis_t b, int dof, const uint64_t* gids, uint64_t m, uint64_t k, double* out);
+/* phi LayoutLeft [n x k]: the first k modes (the lsv_<dof> layout)       */
+int shaw_basis_download(shaw_basis_t b, int dof, uint64_t k, double* phi);
+/* the operators of the first kv / ks modes: mv NULL, as shaw_rom_project;
+ * else the energy pair of shaw_rom_project_energy. Equal bit for bit to
+ * those calls on the downloaded bases.                                    */
+int shaw_basis_project(shaw_app_t app, shaw_basis_t b, uint64_t kv, uint64_t ks, const double* mv,
+                       double* ar_v, double* ar_s);
+void shaw_basis_destroy(shaw_basis_t b);
+
 typedef struct {
   uint64_t kv, ks;
   double dt;

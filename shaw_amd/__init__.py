@@ -113,6 +113,13 @@ def lib():
         "shaw_fom_destroy": (None, [vp]),
         "shaw_fom_pod": (i32, [vp, i32, u64, vp, vp]),
         "shaw_fom_pod_weighted": (i32, [vp, i32, u64, vp, vp, vp]),
+        "shaw_fom_pod_basis": (i32, [vp, u64, u64, vp, vp, P(vp)]),
+        "shaw_basis_info": (i32, [vp, P(u64), P(u64), P(u64), P(u64), P(u64), P(u64)]),
+        "shaw_basis_sigma": (i32, [vp, i32, vp]),
+        "shaw_basis_rows": (i32, [vp, i32, vp, u64, u64, vp]),
+        "shaw_basis_download": (i32, [vp, i32, u64, vp]),
+        "shaw_basis_project": (i32, [vp, vp, u64, u64, vp, vp, vp]),
+        "shaw_basis_destroy": (None, [vp]),
     }
     optional = {
         "shaw_rom_project": (i32, [vp, vp, u64, vp, u64, vp, vp]),
@@ -399,6 +406,21 @@ class Fom:
             _check(lib().shaw_fom_pod_weighted(self._h, d, k, _p(w), _p(phi), _p(sig)))
         return phi.T, sig
 
+    def pod_basis(self, kv: int, ks: int, wv=None, ws=None) -> "Basis":
+        """POD of both dofs' snapshots (consumed) kept on the device
+        (shaw_fom_pod_basis): Euclidean, or in the inner products diag(wv),
+        diag(ws)."""
+        if (wv is None) != (ws is None):
+            raise ShawError("pod_basis: give both weight vectors or neither")
+        w1 = None if wv is None else _f64(wv).ravel()
+        w2 = None if ws is None else _f64(ws).ravel()
+        if w1 is not None and (w1.size != self.app.nVp or w2.size != self.app.nSp):
+            raise ShawError("pod_basis: weights must have nVp and nSp values")
+        h = C.c_void_p()
+        _check(lib().shaw_fom_pod_basis(self._h, kv, ks, None if w1 is None else _p(w1),
+                                        None if w2 is None else _p(w2), C.byref(h)))
+        return Basis(h)
+
     def autotune(self, trial_steps: int = 3) -> float:
         """Pick the fastest fused-kernel tile shape (state unchanged); returns
         its ms per step."""
@@ -476,6 +498,62 @@ def rom_project_energy(app: App, phi_v: np.ndarray, phi_s: np.ndarray, mv: np.nd
     ars = np.zeros((ks, kv))
     _check(lib().shaw_rom_project_energy(app.handle, _p(pv), kv, _p(ps), ks, _p(w), _p(arv), _p(ars)))
     return arv, ars
+
+
+class Basis:
+    """POD bases of both dofs resident on the device (shaw_basis_*). dof:
+    "vp" or "sp"; the first k columns of a basis are its k-mode basis."""
+
+    def __init__(self, handle):
+        self._h = handle
+        vals = [C.c_uint64() for _ in range(6)]
+        _check(lib().shaw_basis_info(self._h, *[C.byref(v) for v in vals]))
+        self.n = {"vp": vals[0].value, "sp": vals[1].value}
+        self.k = {"vp": vals[2].value, "sp": vals[3].value}
+        self.nsnap = {"vp": vals[4].value, "sp": vals[5].value}
+
+    @staticmethod
+    def _dof(dof):
+        if dof not in ("vp", "sp"):
+            raise ShawError(f"unknown dof {dof!r}")
+        return 0 if dof == "vp" else 1
+
+    def sigma(self, dof: str) -> np.ndarray:
+        out = np.zeros(self.nsnap[dof])
+        _check(lib().shaw_basis_sigma(self._h, self._dof(dof), _p(out)))
+        return out
+
+    def rows(self, dof: str, gids, k: int = 0) -> np.ndarray:
+        """[m x k]: the first k modes (all by default) at the grid points gids."""
+        g = _u64(gids)
+        k = k or self.k[dof]
+        out = np.zeros((g.size, k))
+        _check(lib().shaw_basis_rows(self._h, self._dof(dof), _p(g), g.size, k, _p(out)))
+        return out
+
+    def download(self, dof: str, k: int = 0) -> np.ndarray:
+        """Phi [n x k] (Fortran order), the first k modes."""
+        k = k or self.k[dof]
+        out = np.zeros((k, self.n[dof]))
+        _check(lib().shaw_basis_download(self._h, self._dof(dof), k, _p(out)))
+        return out.T
+
+    def project(self, app: "App", kv: int, ks: int, mv=None):
+        """(Ar_v [kv x ks], Ar_s [ks x kv]) of the first kv / ks modes: the
+        reference's Galerkin pair, or with mv the energy-consistent pair."""
+        arv, ars = np.zeros((kv, ks)), np.zeros((ks, kv))
+        w = None if mv is None else _f64(mv).ravel()
+        _check(lib().shaw_basis_project(app.handle, self._h, kv, ks, None if w is None else _p(w), _p(arv),
+                                        _p(ars)))
+        return arv, ars
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.shaw_basis_destroy(self._h)
+            self._h = None
+
+    def close(self):
+        self.__del__()
 
 
 class Rom:

@@ -1264,6 +1264,14 @@ __global__ void k_scale_rows(double* __restrict__ x, const double* __restrict__ 
   for (uint64_t c = blockIdx.y; c < cols; c += gridDim.y) x[r + n * c] *= f;
 }
 
+// x[r + n*c] = x[r + n*c] / w[r] for every column c (grid-stride over columns)
+__global__ void k_div_rows(double* __restrict__ x, const double* __restrict__ w, uint64_t n, uint64_t cols) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double f = w[r];
+  for (uint64_t c = blockIdx.y; c < cols; c += gridDim.y) x[r + n * c] = x[r + n * c] / f;
+}
+
 // plain streaming copy (16 B per lane per iteration), for bandwidth and
 // counter calibration
 __global__ void __launch_bounds__(256) k_copy(const double2* __restrict__ src, double2* __restrict__ dst, uint64_t n) {
@@ -1313,6 +1321,9 @@ struct shaw_fom {
   virtual void bytes(double* kernel, double* model) const = 0;
   virtual void config(int* kernel, int* cpt, int* strip, int* seg, int* pf, int* spl, int* colBlock) const = 0;
   virtual void pod(int dof, uint64_t k, const double* w, double* phi, double* sigma) = 0;
+  // POD into a device buffer dPhi [n x k]; podSize: snapshots of the dof (n: rows)
+  virtual void podTo(int dof, uint64_t k, const double* w, double* dPhi, double* sigma) = 0;
+  virtual uint64_t podSize(int dof, uint64_t* n) const = 0;
   virtual void autotune(uint64_t trialSteps, double* bestMs) = 0;
 };
 
@@ -1734,33 +1745,48 @@ class FomEngine final : public shaw_fom {
     }
     if (model) *model = s * (double)M_ * 3.0 * (double)(nVp_ + nSp_) + 8.0 * (4.0 * nVp_ + 2.0 * nSp_);
   }
+  uint64_t podSize(int dof, uint64_t* n) const override {
+    const Snap& s = snap_[dof];
+    if (n) *n = dof == 0 ? nVp_ : nSp_;
+    return s.every ? (step_ / s.every) * s.ncols : 0;
+  }
   void pod(int dof, uint64_t k, const double* w, double* phi, double* sigma) override {
+    uint64_t n = 0;
+    podSize(dof, &n);
+    DevBuf<double> d;
+    d.alloc(n * k);
+    podTo(dof, k, w, d.p, sigma);
+    SHAW_HIP(hipMemcpy(phi, d.p, n * k * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  void podTo(int dof, uint64_t k, const double* w, double* dPhi, double* sigma) override {
     SHAW_REQUIRE(sizeof(T) == 8, SHAW_ERR_ARG, "POD of snapshots needs the FP64 FOM");
     Snap& s = snap_[dof];
     SHAW_REQUIRE(s.ncols > 0, SHAW_ERR_STATE, "no snapshots were configured for this dof");
     const uint64_t T_ = step_ / s.every, n = dof == 0 ? nVp_ : nSp_;
     SHAW_REQUIRE(T_ == s.Tcap, SHAW_ERR_STATE, "POD needs a full snapshot buffer (advance to max_samples)");
     sync();
-    std::vector<double> sw;
+    DevBuf<double> dw;
+    const uint64_t cols = T_ * s.ncols;
     if (w) {  // POD in the inner product diag(w): modes of diag(sqrt w) S, then unscaled
-      sw.resize(n);
+      std::vector<double> sw(n);
       for (uint64_t i = 0; i < n; ++i) {
         SHAW_REQUIRE(w[i] > 0 && std::isfinite(w[i]), SHAW_ERR_ARG, "POD weights must be positive and finite");
         sw[i] = std::sqrt(w[i]);
       }
-      DevBuf<double> dw;
       dw.alloc(n);
       SHAW_HIP(hipMemcpy(dw.p, sw.data(), n * sizeof(double), hipMemcpyHostToDevice));
-      const uint64_t cols = T_ * s.ncols;
       hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((n + 255) / 256), (unsigned)std::min<uint64_t>(cols, 65535)),
                          dim3(256), 0, 0, s.buf.p, dw.p, n, cols);
       SHAW_HIP(hipGetLastError());
       SHAW_HIP(hipDeviceSynchronize());
     }
-    podDevice(s.buf.p, n, T_ * s.ncols, k, phi, sigma, nullptr);
-    if (w)
-      for (uint64_t c = 0; c < k; ++c)
-        for (uint64_t i = 0; i < n; ++i) phi[i + n * c] /= sw[i];
+    podDeviceTo(s.buf.p, n, cols, k, dPhi, sigma, nullptr);
+    if (w) {
+      hipLaunchKernelGGL(k_div_rows, dim3((unsigned)((n + 255) / 256), (unsigned)std::min<uint64_t>(k, 65535)),
+                         dim3(256), 0, 0, dPhi, dw.p, n, k);
+      SHAW_HIP(hipGetLastError());
+      SHAW_HIP(hipDeviceSynchronize());
+    }
     s.ncols = 0;  // the buffer was consumed
   }
   void config(int* kernel, int* cpt, int* strip, int* seg, int* pf, int* spl, int* colBlock) const override {
@@ -2264,6 +2290,28 @@ int shaw_fom_pod_weighted(shaw_fom_t h, int dof, uint64_t k, const double* w, do
   return guarded([&] {
     SHAW_REQUIRE(h && phi && (dof == 0 || dof == 1), SHAW_ERR_ARG, "bad argument");
     h->pod(dof, k, w, phi, sigma);
+  });
+}
+int shaw_fom_pod_basis(shaw_fom_t h, uint64_t kv, uint64_t ks, const double* wv, const double* ws,
+                       shaw_basis_t* out) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && out && kv > 0 && ks > 0 && (!wv) == (!ws), SHAW_ERR_ARG, "bad argument");
+    auto b = std::make_unique<shaw_basis>();
+    const uint64_t kk[2] = {kv, ks};
+    const double* ww[2] = {wv, ws};
+    for (int d = 0; d < 2; ++d) {
+      uint64_t n = 0;
+      const uint64_t ns = h->podSize(d, &n);
+      SHAW_REQUIRE(ns > 0, SHAW_ERR_STATE, "no snapshots were configured for this dof");
+      SHAW_REQUIRE(kk[d] <= ns && kk[d] <= n, SHAW_ERR_ARG, "POD: number of modes exceeds the number of snapshots");
+      b->n[d] = n;
+      b->k[d] = kk[d];
+      b->ns[d] = ns;
+      SHAW_HIP(hipMalloc((void**)&b->phi[d], n * kk[d] * sizeof(double)));
+      b->sigma[d].assign(ns, 0.0);
+      h->podTo(d, kk[d], ww[d], b->phi[d], b->sigma[d].data());
+    }
+    *out = b.release();
   });
 }
 int shaw_debug_copy(uint64_t bytes, int reps, double* ms_per_copy) {

@@ -725,10 +725,20 @@ namespace shaw {
 void podDevice(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* phi, double* sigma, uint64_t* kOut) {
     SHAW_REQUIRE(dSp && phi && n > 0 && ns > 0 && k > 0, SHAW_ERR_ARG, "bad argument");
     SHAW_REQUIRE(k <= ns && k <= n, SHAW_ERR_ARG, "POD: number of modes exceeds the number of snapshots");
+    DevBuf<double> d;
+    d.alloc(n * k);
+    uint64_t have = k;
+    podDeviceTo(dSp, n, ns, k, d.p, sigma, kOut ? &have : nullptr);
+    if (kOut) *kOut = have;
+    SHAW_HIP(hipMemcpy(phi, d.p, n * have * 8, hipMemcpyDeviceToHost));
+}
+
+void podDeviceTo(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* dPhiOut, double* sigma, uint64_t* kOut) {
+    SHAW_REQUIRE(dSp && dPhiOut && n > 0 && ns > 0 && k > 0, SHAW_ERR_ARG, "bad argument");
+    SHAW_REQUIRE(k <= ns && k <= n, SHAW_ERR_ARG, "POD: number of modes exceeds the number of snapshots");
     SHAW_REQUIRE(ns <= 4096, SHAW_ERR_ARG, "POD: at most 4096 snapshots");
-    DevBuf<double> part, dG, dW, dPhi, dNew, dTmp;
-    struct { double* p; } dS{dSp};
-    dPhi.alloc(n * k);
+    DevBuf<double> part, dG, dW, dNew, dTmp;
+    struct { double* p; } dS{dSp}, dPhi{dPhiOut};
     // host <- A^T B (A: n x ka, B: n x kb, column-major on the device), FP64 MFMA
     auto gramTN = [&](const double* A, uint32_t ka, const double* B, uint32_t kb, std::vector<double>& G) {
       const uint64_t need = projPartDoubles(n, ka, kb);
@@ -826,7 +836,6 @@ void podDevice(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* phi, do
     if (sigma) {
       for (uint64_t i = 0; i < ns; ++i) sigma[i] = i < have ? sv[i] : (i - have < lastTail.size() ? lastTail[i - have] : 0.0);
     }
-    SHAW_HIP(hipMemcpy(phi, dPhi.p, n * have * 8, hipMemcpyDeviceToHost));
     if (kOut) *kOut = have;
 }
 }  // namespace shaw
@@ -921,6 +930,55 @@ struct shaw_rom {
 
 using shaw::guarded;
 
+namespace shaw {
+namespace {
+// Ar_v = Phi_v^T (J_vp Phi_s), Ar_s = Phi_s^T (J_sp Phi_v) with the bases on
+// the device (leading dimension nVp / nSp), results to the host (row-major)
+void projectPair(shaw_app_t app, const double* pv, uint64_t kv, const double* ps, uint64_t ks, double* arV,
+                 double* arS) {
+  const Grid& g = app->ops.grid;
+  const uint64_t nVp = g.nVp(), nSp = g.nSp();
+  DevBuf<double> cv, cs, part, out;
+  cv.alloc(app->ops.coefVp.size()); cv.upload(app->ops.coefVp.data(), cv.n);
+  cs.alloc(app->ops.coefSp.size()); cs.upload(app->ops.coefSp.data(), cs.n);
+  part.alloc(std::max(projPartDoubles(nVp, kv, ks), projPartDoubles(nSp, ks, kv)));
+  out.alloc(kv * ks);
+  projectTN(0, (uint32_t)g.nr, (uint32_t)g.nth, nVp, nSp, cv.p, pv, (uint32_t)kv, nullptr, ps, (uint32_t)ks,
+            part.p, out.p, 0);
+  SHAW_HIP(hipMemcpy(arV, out.p, kv * ks * sizeof(double), hipMemcpyDeviceToHost));
+  projectTN(1, (uint32_t)g.nr, (uint32_t)g.nth, nSp, nVp, cs.p, ps, (uint32_t)ks, nullptr, pv, (uint32_t)kv,
+            part.p, out.p, 0);
+  SHAW_HIP(hipMemcpy(arS, out.p, kv * ks * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+// the energy-consistent pair: Ar_v = (diag(Mv) Phi_v)^T (J_vp Phi_s); Ar_s = -Ar_v^T
+void projectPairEnergy(shaw_app_t app, const double* pv, uint64_t kv, const double* ps, uint64_t ks,
+                       const double* mv, double* arV, double* arS) {
+  const Grid& g = app->ops.grid;
+  const uint64_t nVp = g.nVp(), nSp = g.nSp();
+  DevBuf<double> cv, w, part, out;
+  cv.alloc(app->ops.coefVp.size()); cv.upload(app->ops.coefVp.data(), cv.n);
+  w.alloc(nVp); w.upload(mv, nVp);
+  part.alloc(projPartDoubles(nVp, kv, ks));
+  out.alloc(kv * ks);
+  projectTN(0, (uint32_t)g.nr, (uint32_t)g.nth, nVp, nSp, cv.p, pv, (uint32_t)kv, w.p, ps, (uint32_t)ks, part.p,
+            out.p, 0);
+  SHAW_HIP(hipMemcpy(arV, out.p, kv * ks * sizeof(double), hipMemcpyDeviceToHost));
+  for (uint64_t r = 0; r < ks; ++r)
+    for (uint64_t c = 0; c < kv; ++c) arS[r * kv + c] = -arV[c * ks + r];
+}
+
+// out[j][c] = phi[gid[j] + n c], c < k
+__global__ void k_gather_rows(const double* __restrict__ phi, uint64_t n, const uint64_t* __restrict__ gid,
+                              uint64_t m, uint64_t k, double* __restrict__ out) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m * k) return;
+  const uint64_t j = e / k, c = e % k;
+  out[e] = phi[gid[j] + n * c];
+}
+}  // namespace
+}  // namespace shaw
+
 extern "C" {
 
 int shaw_rom_project(shaw_app_t app, const double* phiV, uint64_t kv, const double* phiS, uint64_t ks, double* arV,
@@ -929,21 +987,10 @@ int shaw_rom_project(shaw_app_t app, const double* phiV, uint64_t kv, const doub
   return guarded([&] {
     SHAW_REQUIRE(app && phiV && phiS && arV && arS && kv > 0 && ks > 0, SHAW_ERR_ARG, "bad argument");
     const Grid& g = app->ops.grid;
-    const uint64_t nVp = g.nVp(), nSp = g.nSp();
-    DevBuf<double> cv, cs, pv, ps, part, out;
-    cv.alloc(app->ops.coefVp.size()); cv.upload(app->ops.coefVp.data(), cv.n);
-    cs.alloc(app->ops.coefSp.size()); cs.upload(app->ops.coefSp.data(), cs.n);
-    pv.alloc(nVp * kv); pv.upload(phiV, pv.n);
-    ps.alloc(nSp * ks); ps.upload(phiS, ps.n);
-    part.alloc(std::max(projPartDoubles(nVp, kv, ks), projPartDoubles(nSp, ks, kv)));
-    out.alloc(kv * ks);
-    // Ar_v = Phi_v^T (J_vp Phi_s), Ar_s = Phi_s^T (J_sp Phi_v)
-    projectTN(0, (uint32_t)g.nr, (uint32_t)g.nth, nVp, nSp, cv.p, pv.p, (uint32_t)kv, nullptr, ps.p, (uint32_t)ks,
-              part.p, out.p, 0);
-    SHAW_HIP(hipMemcpy(arV, out.p, kv * ks * sizeof(double), hipMemcpyDeviceToHost));
-    projectTN(1, (uint32_t)g.nr, (uint32_t)g.nth, nSp, nVp, cs.p, ps.p, (uint32_t)ks, nullptr, pv.p, (uint32_t)kv,
-              part.p, out.p, 0);
-    SHAW_HIP(hipMemcpy(arS, out.p, kv * ks * sizeof(double), hipMemcpyDeviceToHost));
+    DevBuf<double> pv, ps;
+    pv.alloc(g.nVp() * kv); pv.upload(phiV, pv.n);
+    ps.alloc(g.nSp() * ks); ps.upload(phiS, ps.n);
+    projectPair(app, pv.p, kv, ps.p, ks, arV, arS);
   });
 }
 
@@ -953,22 +1000,73 @@ int shaw_rom_project_energy(shaw_app_t app, const double* phiV, uint64_t kv, con
   return guarded([&] {
     SHAW_REQUIRE(app && phiV && phiS && mv && arV && arS && kv > 0 && ks > 0, SHAW_ERR_ARG, "bad argument");
     const Grid& g = app->ops.grid;
-    const uint64_t nVp = g.nVp(), nSp = g.nSp();
-    DevBuf<double> cv, pv, ps, w, part, out;
-    cv.alloc(app->ops.coefVp.size()); cv.upload(app->ops.coefVp.data(), cv.n);
-    pv.alloc(nVp * kv); pv.upload(phiV, pv.n);
-    ps.alloc(nSp * ks); ps.upload(phiS, ps.n);
-    w.alloc(nVp); w.upload(mv, nVp);
-    part.alloc(projPartDoubles(nVp, kv, ks));
-    out.alloc(kv * ks);
-    // Ar_v = (diag(Mv) Phi_v)^T (J_vp Phi_s); Ar_s = -Ar_v^T
-    projectTN(0, (uint32_t)g.nr, (uint32_t)g.nth, nVp, nSp, cv.p, pv.p, (uint32_t)kv, w.p, ps.p, (uint32_t)ks, part.p,
-              out.p, 0);
-    SHAW_HIP(hipMemcpy(arV, out.p, kv * ks * sizeof(double), hipMemcpyDeviceToHost));
-    for (uint64_t r = 0; r < ks; ++r)
-      for (uint64_t c = 0; c < kv; ++c) arS[r * kv + c] = -arV[c * ks + r];
+    DevBuf<double> pv, ps;
+    pv.alloc(g.nVp() * kv); pv.upload(phiV, pv.n);
+    ps.alloc(g.nSp() * ks); ps.upload(phiS, ps.n);
+    projectPairEnergy(app, pv.p, kv, ps.p, ks, mv, arV, arS);
   });
 }
+
+int shaw_basis_info(shaw_basis_t b, uint64_t* n_vp, uint64_t* n_sp, uint64_t* kv, uint64_t* ks, uint64_t* nsnap_vp,
+                    uint64_t* nsnap_sp) {
+  return guarded([&] {
+    SHAW_REQUIRE(b, SHAW_ERR_ARG, "null basis");
+    if (n_vp) *n_vp = b->n[0];
+    if (n_sp) *n_sp = b->n[1];
+    if (kv) *kv = b->k[0];
+    if (ks) *ks = b->k[1];
+    if (nsnap_vp) *nsnap_vp = b->ns[0];
+    if (nsnap_sp) *nsnap_sp = b->ns[1];
+  });
+}
+
+int shaw_basis_sigma(shaw_basis_t b, int dof, double* sigma) {
+  return guarded([&] {
+    SHAW_REQUIRE(b && sigma && (dof == 0 || dof == 1), SHAW_ERR_ARG, "bad argument");
+    std::copy(b->sigma[dof].begin(), b->sigma[dof].end(), sigma);
+  });
+}
+
+int shaw_basis_rows(shaw_basis_t b, int dof, const uint64_t* gids, uint64_t m, uint64_t k, double* out) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(b && (dof == 0 || dof == 1) && (m == 0 || (gids && out)), SHAW_ERR_ARG, "bad argument");
+    SHAW_REQUIRE(k > 0 && k <= b->k[dof], SHAW_ERR_ARG, "more modes than the basis holds");
+    for (uint64_t j = 0; j < m; ++j) SHAW_REQUIRE(gids[j] < b->n[dof], SHAW_ERR_ARG, "grid point out of range");
+    if (m == 0) return;
+    DevBuf<uint64_t> dg;
+    DevBuf<double> dout;
+    dg.alloc(m); dg.upload(gids, m);
+    dout.alloc(m * k);
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((m * k + 255) / 256)), dim3(256), 0, 0, b->phi[dof], b->n[dof],
+                       dg.p, m, k, dout.p);
+    SHAW_HIP(hipGetLastError());
+    SHAW_HIP(hipMemcpy(out, dout.p, m * k * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int shaw_basis_download(shaw_basis_t b, int dof, uint64_t k, double* phi) {
+  return guarded([&] {
+    SHAW_REQUIRE(b && phi && (dof == 0 || dof == 1), SHAW_ERR_ARG, "bad argument");
+    SHAW_REQUIRE(k > 0 && k <= b->k[dof], SHAW_ERR_ARG, "more modes than the basis holds");
+    SHAW_HIP(hipMemcpy(phi, b->phi[dof], b->n[dof] * k * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int shaw_basis_project(shaw_app_t app, shaw_basis_t b, uint64_t kv, uint64_t ks, const double* mv, double* ar_v,
+                       double* ar_s) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(app && b && ar_v && ar_s && kv > 0 && ks > 0, SHAW_ERR_ARG, "bad argument");
+    SHAW_REQUIRE(kv <= b->k[0] && ks <= b->k[1], SHAW_ERR_ARG, "more modes than the basis holds");
+    SHAW_REQUIRE(b->n[0] == app->ops.grid.nVp() && b->n[1] == app->ops.grid.nSp(), SHAW_ERR_ARG,
+                 "basis and app have different grids");
+    if (mv) projectPairEnergy(app, b->phi[0], kv, b->phi[1], ks, mv, ar_v, ar_s);
+    else projectPair(app, b->phi[0], kv, b->phi[1], ks, ar_v, ar_s);
+  });
+}
+
+void shaw_basis_destroy(shaw_basis_t b) { delete b; }
 
 int shaw_basis_orthonormalize(const double* phi, uint64_t n, uint64_t k, const double* w, double* phiOut,
                               double* rOut) {

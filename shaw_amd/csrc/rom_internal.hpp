@@ -11,6 +11,9 @@ namespace shaw {
 // kOut == nullptr: exactly k modes or an error; else up to k modes, stopping at
 // the numerical rank, with the count written to *kOut
 void podDevice(double* dS, uint64_t n, uint64_t ns, uint64_t k, double* phi, double* sigma, uint64_t* kOut);
+// As podDevice with phi left on the device: dPhi [n x k] column-major,
+// allocated by the caller.
+void podDeviceTo(double* dS, uint64_t n, uint64_t ns, uint64_t k, double* dPhi, double* sigma, uint64_t* kOut);
 // Symmetric eigen-decomposition on the host (Householder tridiagonalization
 // and the implicit symmetric QR algorithm): A row-major [n x n] (destroyed),
 // w[n] eigenvalues (unordered), V row-major [n x n] with the eigenvector of
@@ -28,3 +31,18 @@ void projectTN(int op, uint32_t nr, uint32_t nth, uint64_t n, uint64_t nX, const
 uint32_t projChunks(uint64_t n, uint64_t Ka, uint64_t Kb);
 uint64_t projPartDoubles(uint64_t n, uint64_t Ka, uint64_t Kb);
 }  // namespace shaw
+
+// POD bases of both dofs kept on the device (shaw_fom_pod_basis): phi[d]
+// [n[d] x k[d]] column-major, whose first j columns are the j-mode basis.
+struct shaw_basis {
+  uint64_t n[2] = {0, 0}, k[2] = {0, 0}, ns[2] = {0, 0};
+  double* phi[2] = {nullptr, nullptr};
+  std::vector<double> sigma[2];  // ns[d] singular values each
+  shaw_basis() = default;
+  shaw_basis(const shaw_basis&) = delete;
+  shaw_basis& operator=(const shaw_basis&) = delete;
+  ~shaw_basis() {
+    for (double* p : phi)
+      if (p) (void)hipFree(p);
+  }
+};

@@ -11,29 +11,45 @@ reduced leapfrog runs on MFMA with receivers fused (y = Phi_v[recv, :] q_v).
 from __future__ import annotations
 
 import time
-from dataclasses import dataclass
 
 import numpy as np
 
 import shaw_amd
 
 
-@dataclass
 class RomBasis:
-    phi_v: np.ndarray   # [nVp x K]
-    phi_s: np.ndarray   # [nSp x K]
-    sigma_v: np.ndarray
-    sigma_s: np.ndarray
-    seconds: dict
-    mv: np.ndarray = None  # energy weights when the bases are energy-orthonormal
-    ms: np.ndarray = None
+    """POD bases of both dofs, resident on the device (`dev`, a
+    shaw_amd.Basis), with their singular values; phi_v / phi_s download the
+    full bases on first use (only for callers that need host arrays)."""
+
+    def __init__(self, dev: shaw_amd.Basis, sigma_v, sigma_s, seconds: dict, mv=None, ms=None):
+        self.dev, self.sigma_v, self.sigma_s, self.seconds = dev, sigma_v, sigma_s, seconds
+        self.mv, self.ms = mv, ms  # energy weights when the bases are energy-orthonormal
+        self._host = {}
+
+    @property
+    def k(self) -> int:
+        return self.dev.k["vp"]
+
+    @property
+    def phi_v(self) -> np.ndarray:
+        if "vp" not in self._host:
+            self._host["vp"] = self.dev.download("vp")
+        return self._host["vp"]
+
+    @property
+    def phi_s(self) -> np.ndarray:
+        if "sp" not in self._host:
+            self._host["sp"] = self.dev.download("sp")
+        return self._host["sp"]
 
 
 def train_basis(app: shaw_amd.App, sources, dt: float, nsteps: int, snap_every: int, k: int,
                 rank1_forcing: bool = False, energy: bool = False) -> RomBasis:
     """Run the FOM for the training forcings with snapshots of both dofs every
     `snap_every` steps (all in HBM), then POD each dof to k modes (FP64),
-    Euclidean (the reference's) or, with `energy`, in the FOM energy weights."""
+    Euclidean (the reference's) or, with `energy`, in the FOM energy weights.
+    The bases stay on the device."""
     t0 = time.time()
     m = len(sources)
     fom = shaw_amd.Fom(app, m, dt, rank1_forcing=rank1_forcing)
@@ -48,43 +64,51 @@ def train_basis(app: shaw_amd.App, sources, dt: float, nsteps: int, snap_every: 
     fom.sync()
     t1 = time.time()
     mv, ms = energy_weights(app) if energy else (None, None)
-    phi_v, sig_v = fom.pod("vp", k, T * m, mv)
-    phi_s, sig_s = fom.pod("sp", k, T * m, ms)
+    dev = fom.pod_basis(k, k, mv, ms)
     fom.close()
     t2 = time.time()
-    return RomBasis(phi_v, phi_s, sig_v, sig_s, {"fom_training": t1 - t0, "pod": t2 - t1}, mv, ms)
+    return RomBasis(dev, dev.sigma("vp"), dev.sigma("sp"), {"fom_training": t1 - t0, "pod": t2 - t1}, mv, ms)
 
 
-def operators(app: shaw_amd.App, basis: RomBasis, k: int):
+def operators(app: shaw_amd.App, basis, k: int):
     """Galerkin operators of the first k modes (POD bases are nested):
     the reference's Euclidean pair, or the energy-consistent pair for
-    energy-orthonormal bases."""
-    if basis.mv is not None:
-        return shaw_amd.rom_project_energy(app, basis.phi_v[:, :k], basis.phi_s[:, :k], basis.mv)
-    return shaw_amd.rom_project(app, basis.phi_v[:, :k], basis.phi_s[:, :k])
+    energy-orthonormal bases. `basis`: a RomBasis (projected on the device)."""
+    return basis.dev.project(app, k, k, basis.mv)
 
 
-def basis_forcing_inputs(app: shaw_amd.App, basis: RomBasis, k: int, sources, dt: float, nsteps: int):
+def basis_forcing_inputs(app: shaw_amd.App, basis, k: int, sources, dt: float, nsteps: int):
     """Forcing inputs of the first k modes for either projection."""
     if basis.mv is not None:
-        return energy_forcing_inputs(app, basis.phi_v[:, :k], basis.mv, sources, dt, nsteps)
-    return forcing_inputs(app, basis.phi_v[:, :k], sources, dt, nsteps)
+        return energy_forcing_inputs(app, basis, basis.mv, sources, dt, nsteps, k)
+    return forcing_inputs(app, basis, sources, dt, nsteps, k)
 
 
-def forcing_inputs(app: shaw_amd.App, phi_v: np.ndarray, sources, dt: float, nsteps: int):
+def _rows_at(phi_v, gids, k=None) -> np.ndarray:
+    """[m x k] rows of Phi_v at gids from a RomBasis (gathered on the device)
+    or a host array."""
+    if isinstance(phi_v, RomBasis):
+        return phi_v.dev.rows("vp", gids, k or phi_v.k)
+    rows = np.stack([phi_v[g, :] for g in gids], axis=0)
+    return rows if k is None else rows[:, :k]
+
+
+def forcing_inputs(app: shaw_amd.App, phi_v, sources, dt: float, nsteps: int, k=None):
     """B [K x m] with column j = rho^-1[g_j] Phi_v[g_j, :] (rom_problem_rank_one.hpp:133-147)
-    and the forcing table [nsteps x m]."""
+    and the forcing table [nsteps x m]. phi_v: [nVp x K] or a RomBasis."""
     _, _, rho_inv, _ = app.operators()
     gids = [app.map_source(d, a) for d, a, p, dl in sources]
-    B = np.stack([rho_inv[g] * phi_v[g, :] for g in gids], axis=1)
+    rows = _rows_at(phi_v, gids, k)
+    B = np.stack([rho_inv[g] * rows[j] for j, g in enumerate(gids)], axis=1)
     ftab = shaw_amd.forcing_table(["ricker"] * len(sources), [s[3] for s in sources], [s[2] for s in sources],
                                   dt, nsteps)
     return B, ftab
 
 
-def receiver_rows(app: shaw_amd.App, phi_v: np.ndarray, receivers):
-    """C [R x K] = Phi_v[recv, :] (main_reconstruct_seismo.cc:13-41)."""
-    return np.stack([phi_v[app.map_receiver(a), :] for a in receivers], axis=0)
+def receiver_rows(app: shaw_amd.App, phi_v, receivers, k=None):
+    """C [R x K] = Phi_v[recv, :K] (main_reconstruct_seismo.cc:13-41); phi_v:
+    [nVp x K] or a RomBasis."""
+    return _rows_at(phi_v, [app.map_receiver(a) for a in receivers], k)
 
 
 def fom_seismograms(app: shaw_amd.App, sources, dt: float, nsteps: int, receivers, every: int,
@@ -133,11 +157,12 @@ def energy_weights(app: shaw_amd.App):
     return app.energy_weights()
 
 
-def energy_forcing_inputs(app: shaw_amd.App, phi_v: np.ndarray, mv: np.ndarray, sources, dt: float, nsteps: int):
+def energy_forcing_inputs(app: shaw_amd.App, phi_v, mv: np.ndarray, sources, dt: float, nsteps: int, k=None):
     """B [K x m], column j = Mv[g_j] rho^-1[g_j] Phi_v[g_j, :] (q_v = Phi_v^T Mv v), and the forcing table."""
     _, _, rho_inv, _ = app.operators()
     gids = [app.map_source(d, a) for d, a, p, dl in sources]
-    B = np.stack([mv[g] * rho_inv[g] * phi_v[g, :] for g in gids], axis=1)
+    rows = _rows_at(phi_v, gids, k)
+    B = np.stack([mv[g] * rho_inv[g] * rows[j] for j, g in enumerate(gids)], axis=1)
     ftab = shaw_amd.forcing_table(["ricker"] * len(sources), [s[3] for s in sources], [s[2] for s in sources],
                                   dt, nsteps)
     return B, ftab

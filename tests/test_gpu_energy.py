@@ -59,6 +59,54 @@ def test_weighted_pod_vs_numpy(setup128):
         check_pod(sw[:, None] * phi, sig, U, s, k)
 
 
+def _training_fom(app, dt, n, every, train):
+    fom = shaw_amd.Fom(app, len(train), dt)
+    fom.set_sources([app.map_source(d, a) for d, a, p, dl in train])
+    fom.set_forcing(shaw_amd.forcing_table(["ricker"] * 4, [105.0] * 4, [t[2] for t in train], dt, n))
+    T = n // every
+    fom.set_snapshots("vp", every, list(range(4)), T)
+    fom.set_snapshots("sp", every, list(range(4)), T)
+    fom.advance(n)
+    return fom
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_device_basis_equals_host_pod(setup128, weighted):
+    """shaw_fom_pod_basis keeps the bases on the device: its modes, singular
+    values, gathered rows and projected operators equal the host-returning
+    POD (shaw_fom_pod / shaw_fom_pod_weighted) and projection calls bit for
+    bit, Euclidean and energy-weighted."""
+    app, ora, dt, n, every, train = setup128
+    say(f"[gpu] device basis == host POD, weighted={weighted}")
+    mv, ms = rw.energy_weights(app) if weighted else (None, None)
+    k = 120
+    host = {}
+    for dof, w in (("vp", mv), ("sp", ms)):
+        fom = _training_fom(app, dt, n, every, train)
+        host[dof] = fom.pod(dof, k, 4 * (n // every), w)
+        fom.close()
+    fom = _training_fom(app, dt, n, every, train)
+    dev = fom.pod_basis(k, k, mv, ms)
+    fom.close()
+    for dof in ("vp", "sp"):
+        np.testing.assert_array_equal(dev.download(dof), host[dof][0], err_msg=dof)
+        np.testing.assert_array_equal(dev.download(dof, 37), host[dof][0][:, :37], err_msg=dof)
+        np.testing.assert_array_equal(dev.sigma(dof), host[dof][1], err_msg=dof)
+    gids = [app.map_source(d, 0.0) for d in (10.0, 640.0, 2000.0)] + [app.map_receiver(a) for a in (0.0, 45.0, 180.0)]
+    np.testing.assert_array_equal(dev.rows("vp", gids, 50), host["vp"][0][gids, :50])
+    for kv, ks in ((k, k), (64, 90)):
+        a = dev.project(app, kv, ks, mv)
+        pv, ps = host["vp"][0][:, :kv], host["sp"][0][:, :ks]
+        b = shaw_amd.rom_project_energy(app, pv, ps, mv) if weighted else shaw_amd.rom_project(app, pv, ps)
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+    with pytest.raises(shaw_amd.ShawError):
+        dev.rows("vp", [app.nVp], 5)
+    with pytest.raises(shaw_amd.ShawError):
+        dev.project(app, k + 1, k)
+    dev.close()
+
+
 def test_energy_projection_vs_numpy(setup128):
     app, ora, dt, n, every, train = setup128
     mv, ms = rw.energy_weights(app)
@@ -88,14 +136,14 @@ def energy_rom(setup128):
 def test_energy_rom_accuracy_fp64_and_bf16x3(setup128, energy_rom, K):
     app, ora, dt, n, every, train = setup128
     basis, euclid, test, fom = energy_rom
-    C = rw.receiver_rows(app, basis.phi_v[:, :K], workloads.RECEIVERS)
+    C = rw.receiver_rows(app, basis, workloads.RECEIVERS, K)
     arv, ars = rw.operators(app, basis, K)
     B, ftab = rw.basis_forcing_inputs(app, basis, K, test, dt, n)
     errs = {}
     for prec, name in ((shaw_amd.ROM_FP64, "fp64"), (shaw_amd.ROM_BF16X3, "bf16x3"), (shaw_amd.ROM_BF16X4, "bf16x4")):
         r = shaw_amd.rom_run(arv, ars, B, ftab, dt, n, C, 10, prec)
         errs[name] = max(rel_l2(r["seismogram"][m], fom[m]) for m in range(len(test)))
-    Ce = rw.receiver_rows(app, euclid.phi_v[:, :K], workloads.RECEIVERS)
+    Ce = rw.receiver_rows(app, euclid, workloads.RECEIVERS, K)
     ae, se = rw.operators(app, euclid, K)
     Be, fe = rw.basis_forcing_inputs(app, euclid, K, test, dt, n)
     eu = {}

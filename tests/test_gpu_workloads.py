@@ -50,7 +50,7 @@ def _config3(w, app, energy):
     d, p = workloads.seeded_depths_periods(32, 32, 3)
     test = [(dd, 0.0, pp, workloads.DELAY_S) for dd in d for pp in p]
     B, ftab = rw.basis_forcing_inputs(app, basis, 256, test, w.dt, N)
-    C = rw.receiver_rows(app, basis.phi_v, workloads.RECEIVERS)
+    C = rw.receiver_rows(app, basis, workloads.RECEIVERS)
     runs = {}
     for prec, name in ((shaw_amd.ROM_FP64, "fp64"), (shaw_amd.ROM_BF16X3, "bf16x3"), (shaw_amd.ROM_BF16X4, "bf16x4")):
         say(f"[gpu] config 3 ({'energy' if energy else 'Euclidean'}) ROM {name}, M = 1024, {N} steps")
@@ -141,7 +141,7 @@ def test_config4_serving_alone_equals_mix(config2_app):
     say("[gpu] config 4: energy basis (5 training periods at 640 km)")
     train = [(640.0, 0.0, p, workloads.DELAY_S) for p in (20.0, 25.0, 35.0, 50.0, 70.0)]
     basis = rw.train_basis(app, train, w.dt, N, N // 200, 512, energy=True)
-    C512 = rw.receiver_rows(app, basis.phi_v, workloads.RECEIVERS)
+    C512 = rw.receiver_rows(app, basis, workloads.RECEIVERS)
     ops = {k: rw.operators(app, basis, k) for k in (128, 256, 512)}
     engines = {k: serving.GpuEngine(shaw_amd.Rom(ops[k][0], ops[k][1], w.dt, shaw_amd.ROM_BF16X4, C512[:, :k],
                                                  max_cols=512, max_samples=800), 512) for k in (128, 256, 512)}
