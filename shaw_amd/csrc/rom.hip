@@ -69,8 +69,10 @@ __host__ __device__ inline float bf16ToF32(unsigned short h) {
   return x;
 }
 
-// ------------------------------------------------------------ POD products
-// out[r + n*c] = sum_j S[r + n*j] * W[j*k + c]   (S: n x ns col-major, W: ns x k row-major)
+// ------------------------------------------------------------ reconstruction
+// out[r + n*c] = sum_j S[r + n*j] * W[j*k + c]   (S: n x ns col-major, W: ns x k row-major),
+// each entry summed over j in order (the reconstruction tools' loop; the POD's
+// products run on MFMA, gemmNN in proj.hip)
 constexpr int NNC = 16;  // output columns per thread
 // out = base + alpha * (S W); base may be null (treated as 0) or alias out
 __global__ void __launch_bounds__(256) k_gemm_nn_rows(const double* __restrict__ S, const double* __restrict__ W,
@@ -741,7 +743,7 @@ void podDeviceTo(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* dPhiO
     struct { double* p; } dS{dSp}, dPhi{dPhiOut};
     // host <- A^T B (A: n x ka, B: n x kb, column-major on the device), FP64 MFMA
     auto gramTN = [&](const double* A, uint32_t ka, const double* B, uint32_t kb, std::vector<double>& G) {
-      const uint64_t need = projPartDoubles(n, ka, kb);
+      const uint64_t need = gramPartDoubles(n, ka, kb);
       if (part.n < need) part.alloc(need);
       if (dG.n < (size_t)ka * kb) dG.alloc((size_t)ka * kb);
       projectTN(2, 0, 0, n, n, nullptr, A, ka, nullptr, B, kb, part.p, dG.p, 0);
@@ -753,9 +755,7 @@ void podDeviceTo(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* dPhiO
                        const double* base, double alpha) {
       dW.alloc(Wh.size());
       dW.upload(Wh.data(), Wh.size());
-      hipLaunchKernelGGL(k_gemm_nn_rows, dim3((unsigned)((n + 255) / 256), (kout + NNC - 1) / NNC), dim3(256), 0, 0, X,
-                         dW.p, out, base, alpha, n, kin, kout);
-      SHAW_HIP(hipGetLastError());
+      gemmNN(X, n, kin, dW.p, kout, out, base, alpha, 0);
     };
     // Phi_new <- Phi_new L^-T, Phi_new^T Phi_new = L L^T
     auto cholOrth = [&](double* P, uint32_t r) {
@@ -824,6 +824,7 @@ void podDeviceTo(double* dSp, uint64_t n, uint64_t ns, uint64_t k, double* dPhiO
       SHAW_HIP(hipMemcpy(dPhi.p + n * have, dNew.p, n * r * 8, hipMemcpyDeviceToDevice));
       for (uint64_t c = 0; c < r; ++c) sv.push_back(sc[c]);
       have += r;
+      if (have >= k) break;  // no further pass reads the residual
       // S_c <- S_c - Phi_new (Phi_new^T S_c)
       std::vector<double> Y;
       gramTN(dNew.p, (uint32_t)r, dS.p, (uint32_t)ns, Y);
@@ -1076,7 +1077,7 @@ int shaw_basis_orthonormalize(const double* phi, uint64_t n, uint64_t k, const d
     DevBuf<double> dPhi, dW, part, dG, dR, dOut;
     dPhi.alloc(n * k); dPhi.upload(phi, n * k);
     dW.alloc(n); dW.upload(w, n);
-    part.alloc(projPartDoubles(n, k, k));
+    part.alloc(gramPartDoubles(n, k, k));
     dG.alloc(k * k);
     // G = Phi^T diag(w) Phi (FP64 MFMA)
     projectTN(2, 0, 0, n, n, nullptr, dPhi.p, (uint32_t)k, dW.p, dPhi.p, (uint32_t)k, part.p, dG.p, 0);
@@ -1106,8 +1107,7 @@ int shaw_basis_orthonormalize(const double* phi, uint64_t n, uint64_t k, const d
     // Phi_out = Phi Rinv
     dR.alloc(k * k); dR.upload(Ri.data(), k * k);
     dOut.alloc(n * k);
-    hipLaunchKernelGGL(k_gemm_nn_rows, dim3((unsigned)((n + 255) / 256), (unsigned)((k + NNC - 1) / NNC)), dim3(256), 0,
-                       0, dPhi.p, dR.p, dOut.p, nullptr, 1.0, n, (uint32_t)k, (uint32_t)k);
+    gemmNN(dPhi.p, n, (uint32_t)k, dR.p, (uint32_t)k, dOut.p, nullptr, 1.0, 0);
     SHAW_HIP(hipGetLastError());
     SHAW_HIP(hipMemcpy(phiOut, dOut.p, n * k * sizeof(double), hipMemcpyDeviceToHost));
     if (rOut) std::copy(R.begin(), R.end(), rOut);

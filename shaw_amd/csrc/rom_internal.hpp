@@ -24,12 +24,20 @@ void symmetricEigen(std::vector<double>& A, int n, std::vector<double>& w, std::
 // coefVp [nVp][4]), op 1 J = J_sp (n = nSp, nX = nVp, coef = coefSp
 // [nSp][2]), op 2 J = I (nX = n, coef unused). L, X column-major on the
 // device; w [n] or null. part: device scratch of projPartDoubles(n, Ka, Kb)
-// doubles (split-K partial tiles, every entry written before it is read).
+// doubles for ops 0 and 1, gramPartDoubles(n, Ka, Kb) for op 2 (split-K
+// partial tiles, every entry written before it is read).
 void projectTN(int op, uint32_t nr, uint32_t nth, uint64_t n, uint64_t nX, const double* coef, const double* L,
                uint32_t Ka, const double* w, const double* X, uint32_t Kb, double* part, double* out,
                hipStream_t stream);
 uint32_t projChunks(uint64_t n, uint64_t Ka, uint64_t Kb);
+// out (device, [n x kout] column-major) = base + alpha * X W in FP64 on MFMA
+// (proj.hip): X [n x kin] column-major, W [kin x kout] row-major, both on
+// the device; base may be null (0) or alias out.
+void gemmNN(const double* X, uint64_t n, uint32_t kin, const double* W, uint32_t kout, double* out,
+            const double* base, double alpha, hipStream_t stream);
 uint64_t projPartDoubles(uint64_t n, uint64_t Ka, uint64_t Kb);
+uint32_t gramChunks(uint64_t n, uint64_t Ka, uint64_t Kb);
+uint64_t gramPartDoubles(uint64_t n, uint64_t Ka, uint64_t Kb);
 }  // namespace shaw
 
 // POD bases of both dofs kept on the device (shaw_fom_pod_basis): phi[d]
