@@ -113,6 +113,32 @@ void shaw_app_destroy(shaw_app_t app);
 int shaw_forcing_table(int m, const int* kinds, const double* delays, const double* periods,
                        double dt, uint64_t nsteps, double* table);
 
+/* ---------------------------------------------------- device-side setup --
+ * The setup the reference does on the host (material evaluation and
+ * operator assembly, shwavepp.hpp:246-466; forcing values, signal.hpp:86-128;
+ * source/receiver mapping, map_point_source_to_velocity_grid_point.hpp:61-133
+ * and map_nominal_location_to_velocity_grid_point.hpp:57-117) run on the GPU,
+ * one thread per point, step or location, from the mesh's per-row and
+ * per-column tables (checked against every point of the mesh once; a mesh
+ * that is not separable gives SHAW_ERR_ARG). For inversion loops that change
+ * the material and for serving many source locations.
+ *
+ * The operators of `mat` on app's mesh into device buffers (current device):
+ * coef_vp [nvp][4], coef_sp [nsp][2] (the layout of shaw_app_operators),
+ * rho_inv [nvp]; any may be NULL. Equal to the host assembly bit for bit.  */
+int shaw_app_assemble_device(shaw_app_t app, const shaw_material* mat, double* coef_vp, double* coef_sp,
+                             double* rho_inv, void* stream);
+/* shaw_forcing_table into a device buffer table [nsteps][m]; equal to the
+ * host table up to the rounding of the device exp/sin (a few ulp).       */
+int shaw_forcing_table_device(uint64_t m, const int* kinds, const double* delays, const double* periods,
+                              double dt, uint64_t nsteps, double* table, void* stream);
+/* gids[n] of n sources (depths_km, angles_deg) or, with depths_km NULL, n
+ * receivers at the surface (angles_deg): shaw_map_source / shaw_map_receiver
+ * for many locations in one launch, with the same distance formula and
+ * tie rule (ties to the last gid).                                         */
+int shaw_map_points_device(shaw_app_t app, uint64_t n, const double* depths_km, const double* angles_deg,
+                           uint64_t* gids);
+
 /* ------------------------------------------------------------------ FOM --
  * Full-order leapfrog on one GPU: x_v += dt J_vp x_s + dt rho^-1 f, then
  * x_s += dt J_sp x_v, for m forcing columns (m = 1 is the rank-1 state).   */
@@ -162,6 +188,15 @@ int shaw_fom_set_sources(shaw_fom_t h, const uint64_t* vp_gid);
 /* Forcing values [nsteps][m] for steps 1..nsteps (see shaw_forcing_table);
  * uploaded once, read by the velocity kernels.                             */
 int shaw_fom_set_forcing(shaw_fom_t h, uint64_t nsteps, const double* table);
+/* The forcing table of m signals (kinds/delays/periods as shaw_forcing_table)
+ * generated on the device instead of uploaded (no counterpart in the
+ * reference, which evaluates each forcing on the host every step).       */
+int shaw_fom_set_signals(shaw_fom_t h, const int* kinds, const double* delays, const double* periods,
+                         uint64_t nsteps);
+/* Replace the FOM's operators by those of another material on the same mesh,
+ * assembled on the device (shaw_app_assemble_device); the state is kept.
+ * Equal to a FOM created on an app with that material.                    */
+int shaw_fom_set_material(shaw_fom_t h, shaw_app_t app, const shaw_material* mat);
 /* Record x_v at r receivers every `every` steps (step % every == 0), up to
  * `max_samples` samples; a device buffer LayoutLeft [r x T x m].          */
 int shaw_fom_set_receivers(shaw_fom_t h, uint64_t r, const uint64_t* vp_gid, uint64_t every,
@@ -336,6 +371,12 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out);
  * launches in flight (they write back every slot of their workgroups).    */
 int shaw_rom_admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const double* ftab,
                    const uint64_t* horizons, uint64_t every, uint64_t* slots_out);
+/* shaw_rom_admit with each column's forcing series evaluated on the device
+ * from its signal (kinds/delays/periods [n], shaw_forcing_table's
+ * conventions, time step the engine's dt) instead of a host table.        */
+int shaw_rom_admit_signals(shaw_rom_t h, uint64_t n, const double* b, const int* kinds, const double* delays,
+                           const double* periods, uint64_t nsteps, const uint64_t* horizons, uint64_t every,
+                           uint64_t* slots_out);
 /* Advance every active column by up to nsteps (columns stop at their horizon). */
 int shaw_rom_advance(shaw_rom_t h, uint64_t nsteps);
 int shaw_rom_sync(shaw_rom_t h);

@@ -114,6 +114,12 @@ def lib():
         "shaw_fom_pod": (i32, [vp, i32, u64, vp, vp]),
         "shaw_fom_pod_weighted": (i32, [vp, i32, u64, vp, vp, vp]),
         "shaw_fom_pod_basis": (i32, [vp, u64, u64, vp, vp, P(vp)]),
+        "shaw_app_assemble_device": (i32, [vp, P(_Material), vp, vp, vp, vp]),
+        "shaw_forcing_table_device": (i32, [u64, vp, vp, vp, dbl, u64, vp, vp]),
+        "shaw_map_points_device": (i32, [vp, u64, vp, vp, vp]),
+        "shaw_fom_set_signals": (i32, [vp, vp, vp, vp, u64]),
+        "shaw_fom_set_material": (i32, [vp, vp, P(_Material)]),
+        "shaw_rom_admit_signals": (i32, [vp, u64, vp, vp, vp, vp, u64, vp, u64, vp]),
         "shaw_basis_info": (i32, [vp, P(u64), P(u64), P(u64), P(u64), P(u64), P(u64)]),
         "shaw_basis_sigma": (i32, [vp, i32, vp]),
         "shaw_basis_rows": (i32, [vp, i32, vp, u64, u64, vp]),
@@ -206,6 +212,50 @@ def mesh_write(directory: str, nr: int, nth: int, th_left=0.0, th_right=180.0, r
     _check(lib().shaw_mesh_write(directory.encode(), nr, nth, th_left, th_right, r_cmb, r_surf))
 
 
+class _DeviceBuffer:
+    """Device memory of the current GPU through the HIP runtime (for the
+    entry points that write device buffers); `host()` copies it back."""
+    _hip = None
+
+    def __init__(self, count: int):
+        if _DeviceBuffer._hip is None:
+            _DeviceBuffer._hip = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+            for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize"):
+                getattr(_DeviceBuffer._hip, f).restype = C.c_int
+            _DeviceBuffer._hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            _DeviceBuffer._hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            _DeviceBuffer._hip.hipFree.argtypes = [C.c_void_p]
+        self.count = count
+        self.ptr = C.c_void_p()
+        if _DeviceBuffer._hip.hipMalloc(C.byref(self.ptr), max(8, count * 8)) != 0:
+            raise ShawError("hipMalloc failed")
+
+    def host(self, dtype=np.float64) -> np.ndarray:
+        out = np.empty(self.count, dtype=dtype)
+        if _DeviceBuffer._hip.hipDeviceSynchronize() != 0 or _DeviceBuffer._hip.hipMemcpy(
+                out.ctypes.data_as(C.c_void_p), self.ptr, self.count * out.itemsize, 2) != 0:
+            raise ShawError("device to host copy failed")
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _DeviceBuffer._hip is not None:
+            _DeviceBuffer._hip.hipFree(self.ptr)
+            self.ptr = None
+
+
+def _signal_arrays(kinds, delays, periods):
+    kinds = np.ascontiguousarray([SIGNAL_KINDS[k] if isinstance(k, str) else int(k) for k in kinds], dtype=np.int32)
+    return kinds, _f64(delays).ravel(), _f64(periods).ravel()
+
+
+def forcing_table_device(kinds, delays, periods, dt: float, nsteps: int) -> np.ndarray:
+    """forcing_table evaluated on the GPU (shaw_forcing_table_device), copied back."""
+    k, d, p = _signal_arrays(kinds, delays, periods)
+    buf = _DeviceBuffer(len(k) * nsteps)
+    _check(lib().shaw_forcing_table_device(len(k), _p(k), _p(d), _p(p), dt, nsteps, buf.ptr, None))
+    return buf.host().reshape(nsteps, len(k))
+
+
 def forcing_table(kinds, delays, periods, dt: float, nsteps: int) -> np.ndarray:
     """[nsteps][m] forcing values f_m((s-1) dt) for steps s = 1..nsteps."""
     kinds = np.ascontiguousarray([SIGNAL_KINDS[k] if isinstance(k, str) else int(k) for k in kinds], dtype=np.int32)
@@ -283,6 +333,26 @@ class App:
         _check(lib().shaw_map_receiver(self._h, angle_deg, C.byref(g)))
         return int(g.value)
 
+    def map_points_device(self, angles_deg, depths_km=None) -> np.ndarray:
+        """gids of many sources (depths_km given) or surface receivers, mapped
+        on the GPU in one launch (shaw_map_points_device)."""
+        a = _f64(angles_deg).ravel()
+        d = None if depths_km is None else _f64(depths_km).ravel()
+        if d is not None and d.size != a.size:
+            raise ShawError("map_points_device: depths and angles differ in length")
+        g = np.zeros(a.size, dtype=np.uint64)
+        _check(lib().shaw_map_points_device(self._h, a.size, None if d is None else _p(d), _p(a),
+                                            g.ctypes.data_as(C.c_void_p)))
+        return g
+
+    def operators_device(self, material: dict):
+        """(coef_vp [nVp x 4], coef_sp [nSp x 2], rho_inv [nVp]) of `material`
+        on this mesh, assembled on the GPU (shaw_app_assemble_device)."""
+        m, keep = _material_struct(material)
+        cv, cs, ri = _DeviceBuffer(4 * self.nVp), _DeviceBuffer(2 * self.nSp), _DeviceBuffer(self.nVp)
+        _check(lib().shaw_app_assemble_device(self._h, C.byref(m), cv.ptr, cs.ptr, ri.ptr, None))
+        return cv.host().reshape(self.nVp, 4), cs.host().reshape(self.nSp, 2), ri.host()
+
 
 class Fom:
     """Full-order leapfrog for m forcing columns on the GPU."""
@@ -317,6 +387,18 @@ class Fom:
         t = _f64(table)
         assert t.ndim == 2 and t.shape[1] == self.m
         _check(lib().shaw_fom_set_forcing(self._h, t.shape[0], _p(t)))
+
+    def set_signals(self, kinds, delays, periods, nsteps: int):
+        """The forcing table generated on the GPU (shaw_fom_set_signals)."""
+        k, d, p = _signal_arrays(kinds, delays, periods)
+        assert len(k) == self.m
+        _check(lib().shaw_fom_set_signals(self._h, _p(k), _p(d), _p(p), nsteps))
+
+    def set_material(self, app: "App", material: dict):
+        """Operators of another material on the same mesh, assembled on the
+        GPU (shaw_fom_set_material); the state is kept."""
+        m, keep = _material_struct(material)
+        _check(lib().shaw_fom_set_material(self._h, app.handle, C.byref(m)))
 
     def set_receivers(self, gids, every: int, max_samples: int):
         g = _u64(gids)
@@ -594,6 +676,20 @@ class Rom:
         for s in slots:
             self.every[int(s)] = every
         return [int(s) for s in slots]
+
+    def admit_signals(self, b, kinds, delays, periods, nsteps: int, horizons=None, every=0):
+        """admit with each column's forcing series evaluated on the GPU from
+        its signal (shaw_rom_admit_signals)."""
+        b = _f64(b)
+        n = b.shape[1]
+        k, d, p = _signal_arrays(kinds, delays, periods)
+        hz = _u64(horizons if horizons is not None else [nsteps] * n)
+        slots = np.zeros(n, np.uint64)
+        _check(lib().shaw_rom_admit_signals(self._h, n, _p(b), _p(k), _p(d), _p(p), nsteps, _p(hz), every,
+                                            _p(slots)))
+        for sl in slots:
+            self.every[int(sl)] = every
+        return [int(sl) for sl in slots]
 
     def advance(self, nsteps: int):
         _check(lib().shaw_rom_advance(self._h, nsteps))

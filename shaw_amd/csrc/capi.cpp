@@ -15,7 +15,7 @@ thread_local std::string g_lastError;
 void setLastError(const std::string& msg) { g_lastError = msg; }
 void clearLastError() { g_lastError.clear(); }
 
-static Material toMaterial(const shaw_material* m) {
+Material materialFromC(const shaw_material* m) {
   SHAW_REQUIRE(m != nullptr, SHAW_ERR_ARG, "null material");
   SHAW_REQUIRE(m->nparams >= 0 && (m->nparams == 0 || m->params), SHAW_ERR_ARG, "bad material params");
   Material mat;
@@ -44,7 +44,7 @@ int shaw_app_create_from_mesh(const char* dir, const shaw_material* mat, shaw_ap
   return guarded([&] {
     SHAW_REQUIRE(dir && out, SHAW_ERR_ARG, "null argument");
     auto a = std::make_unique<shaw_app>();
-    a->mat = shaw::toMaterial(mat);
+    a->mat = shaw::materialFromC(mat);
     a->mesh = shaw::Mesh::load(dir);
     *out = shaw::finishApp(std::move(a));
   });
@@ -55,7 +55,7 @@ int shaw_app_create_structured(uint64_t nr, uint64_t nth, double thL, double thR
   return guarded([&] {
     SHAW_REQUIRE(out, SHAW_ERR_ARG, "null argument");
     auto a = std::make_unique<shaw_app>();
-    a->mat = shaw::toMaterial(mat);
+    a->mat = shaw::materialFromC(mat);
     a->mesh = shaw::Mesh::structured(nr, nth, thL, thR, rCmb, rSurf);
     *out = shaw::finishApp(std::move(a));
   });

@@ -1264,6 +1264,11 @@ __global__ void k_scale_rows(double* __restrict__ x, const double* __restrict__ 
   for (uint64_t c = blockIdx.y; c < cols; c += gridDim.y) x[r + n * c] *= f;
 }
 
+__global__ void k_to_float(const double* __restrict__ x, float* __restrict__ y, uint64_t n) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) y[e] = (float)x[e];
+}
+
 // x[r + n*c] = x[r + n*c] / w[r] for every column c (grid-stride over columns)
 __global__ void k_div_rows(double* __restrict__ x, const double* __restrict__ w, uint64_t n, uint64_t cols) {
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1306,6 +1311,10 @@ struct shaw_fom {
   virtual ~shaw_fom() = default;
   virtual void setSources(const uint64_t* gid) = 0;
   virtual void setForcing(uint64_t nsteps, const double* tab) = 0;
+  // forcing table generated on the device from signal parameters
+  virtual void setSignals(const int* kinds, const double* delays, const double* periods, uint64_t nsteps) = 0;
+  // operators of another material on the same mesh, assembled on the device
+  virtual void setMaterial(shaw_app& app, const shaw::Material& mat) = 0;
   virtual void setReceivers(uint64_t R, const uint64_t* gid, uint64_t every, uint64_t maxSamples) = 0;
   virtual void setSnapshots(int dof, uint64_t every, uint64_t ncols, const uint64_t* cols, uint64_t maxSamples) = 0;
   virtual void reset() = 0;
@@ -1441,6 +1450,7 @@ class FomEngine final : public shaw_fom {
   }
 
   void setSources(const uint64_t* gid) override {
+    srcGid_.assign(gid, gid + M_);
     std::vector<int64_t> s((size_t)nb_ * MC_, -1);
     std::vector<T> r((size_t)nb_ * MC_, T(0));
     for (uint64_t m = 0; m < M_; ++m) {
@@ -1459,6 +1469,37 @@ class FomEngine final : public shaw_fom {
     ftab_.alloc(nsteps * M_);
     SHAW_HIP(hipMemcpy(ftab_.p, tab, nsteps * M_ * sizeof(double), hipMemcpyHostToDevice));
     ftabSteps_ = nsteps;
+  }
+  void setSignals(const int* kinds, const double* delays, const double* periods, uint64_t nsteps) override {
+    sync();
+    dropGraph();  // the graph holds the table's address
+    ftab_.alloc(nsteps * M_);
+    forcingTableDevice(M_, kinds, delays, periods, dt_, nsteps, ftab_.p, 0, stream_);
+    ftabSteps_ = nsteps;
+  }
+  void setMaterial(shaw_app& app, const Material& mat) override {
+    SHAW_REQUIRE(app.mesh.grid.nVp() == nVp_ && app.mesh.grid.nSp() == nSp_, SHAW_ERR_ARG,
+                 "the app's mesh is not this FOM's mesh");
+    sync();
+    dropGraph();
+    DevBuf<double> cv, cs, ri;
+    cv.alloc(nVp_ * 4);
+    cs.alloc(nSp_ * 2);
+    ri.alloc(nVp_);
+    assembleOperatorsDevice(app, mat, cv.p, cs.p, ri.p, stream_);
+    if constexpr (sizeof(T) == 8) {
+      SHAW_HIP(hipMemcpyAsync(cv_.p, cv.p, nVp_ * 4 * 8, hipMemcpyDeviceToDevice, stream_));
+      SHAW_HIP(hipMemcpyAsync(cs_.p, cs.p, nSp_ * 2 * 8, hipMemcpyDeviceToDevice, stream_));
+    } else {  // the state precision, as the host cast at creation
+      hipLaunchKernelGGL(k_to_float, dim3((unsigned)((nVp_ * 4 + 255) / 256)), dim3(256), 0, stream_, cv.p,
+                         reinterpret_cast<float*>(cv_.p), nVp_ * 4);
+      hipLaunchKernelGGL(k_to_float, dim3((unsigned)((nSp_ * 2 + 255) / 256)), dim3(256), 0, stream_, cs.p,
+                         reinterpret_cast<float*>(cs_.p), nSp_ * 2);
+      SHAW_HIP(hipGetLastError());
+    }
+    SHAW_HIP(hipMemcpyAsync(rhoInv_.data(), ri.p, nVp_ * 8, hipMemcpyDeviceToHost, stream_));
+    SHAW_HIP(hipStreamSynchronize(stream_));
+    if (haveSources_) setSources(srcGid_.data());  // 1/rho at the source points of the new material
   }
   void setReceivers(uint64_t R, const uint64_t* gid, uint64_t every, uint64_t maxSamples) override {
     SHAW_REQUIRE(R == 0 || every > 0, SHAW_ERR_ARG, "receiver sampling period must be > 0");
@@ -2177,6 +2218,7 @@ class FomEngine final : public shaw_fom {
   static constexpr size_t kMaxLds = 160 * 1024;
   static constexpr uint64_t kMinWaves = 1536;  // 6 per CU of the 256
   std::vector<double> rhoInv_;
+  std::vector<uint64_t> srcGid_;
   std::vector<std::pair<uint32_t, uint32_t>> dupRecv_;
   uint64_t ftabSteps_ = 0, R_ = 0, every_ = 1, Tcap_ = 0, step_ = 0, stepForSnap_ = 0;
   int cur_ = 0;
@@ -2290,6 +2332,22 @@ int shaw_fom_pod_weighted(shaw_fom_t h, int dof, uint64_t k, const double* w, do
   return guarded([&] {
     SHAW_REQUIRE(h && phi && (dof == 0 || dof == 1), SHAW_ERR_ARG, "bad argument");
     h->pod(dof, k, w, phi, sigma);
+  });
+}
+int shaw_fom_set_signals(shaw_fom_t h, const int* kinds, const double* delays, const double* periods,
+                         uint64_t nsteps) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && kinds && delays && periods, SHAW_ERR_ARG, "null argument");
+    h->setSignals(kinds, delays, periods, nsteps);
+  });
+}
+int shaw_fom_set_material(shaw_fom_t h, shaw_app_t app, const shaw_material* mat) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && app && mat, SHAW_ERR_ARG, "null argument");
+    shaw::Material m = shaw::materialFromC(mat);
+    m.rSurfM = app->mesh.info.bounds[3];
+    m.validate(app->mesh.info);
+    h->setMaterial(*app, m);
   });
 }
 int shaw_fom_pod_basis(shaw_fom_t h, uint64_t kv, uint64_t ks, const double* wv, const double* ws,

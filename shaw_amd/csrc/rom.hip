@@ -1170,11 +1170,16 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
   });
 }
 
-int shaw_rom_admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const double* ftab, const uint64_t* horizons,
-                   uint64_t every, uint64_t* slots) {
-  using namespace shaw;
-  return guarded([&] {
-    SHAW_REQUIRE(h && b && ftab && horizons && slots, SHAW_ERR_ARG, "null argument");
+}  // extern "C"
+
+namespace shaw {
+namespace {
+// admission with the forcing series from a host table ftab [nsteps][n], or
+// (ftab null) evaluated on the device from signal parameters
+void admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const double* ftab, const int* kinds,
+           const double* delays, const double* periods, const uint64_t* horizons, uint64_t every, uint64_t* slots) {
+    SHAW_REQUIRE(h && b && (ftab || (kinds && delays && periods)) && horizons && slots, SHAW_ERR_ARG,
+                 "null argument");
     SHAW_REQUIRE(h->R == 0 || every > 0, SHAW_ERR_ARG, "sampling period must be > 0 when receivers are set");
     std::vector<uint64_t> free;
     for (uint64_t c = 0; c < h->maxCols && free.size() < n; ++c)
@@ -1201,14 +1206,17 @@ int shaw_rom_admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, c
       h->maxSteps = need;
     }
     // stage [n][Kvp] forcing columns and [n][nsteps] forcing series, one copy, one scatter
-    std::vector<double> st(n * (h->kvp + nsteps), 0.0);
+    std::vector<double> st(n * (h->kvp + (ftab ? nsteps : 0)), 0.0);
     for (uint64_t j = 0; j < n; ++j) {
       for (uint64_t k = 0; k < h->kv; ++k) st[j * h->kvp + k] = b[k * n + j];
+      if (!ftab) continue;
       double* f = st.data() + n * h->kvp + j * nsteps;
       for (uint64_t t = 0; t < nsteps; ++t) f[t] = ftab[t * n + j];
     }
-    if (h->stage.n < st.size()) h->stage.alloc(st.size());
+    if (h->stage.n < n * (h->kvp + nsteps)) h->stage.alloc(n * (h->kvp + nsteps));
     h->stage.upload(st.data(), st.size());
+    if (!ftab)  // the series [n][nsteps] of the staging buffer, on the device
+      forcingTableDevice(n, kinds, delays, periods, h->dt, nsteps, h->stage.p + n * h->kvp, nsteps, h->stream);
     if (h->stageSlots.n < n) h->stageSlots.alloc(n);
     h->stageSlots.upload(free.data(), n);
     hipLaunchKernelGGL(k_admit_scatter, dim3(64, (unsigned)n), dim3(256), 0, h->stream, h->stageSlots.p, n, h->stage.p,
@@ -1224,6 +1232,26 @@ int shaw_rom_admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, c
     }
     SHAW_HIP(hipStreamSynchronize(h->stream));
     h->uploadSlotTables();
+}
+}  // namespace
+}  // namespace shaw
+
+extern "C" {
+
+int shaw_rom_admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const double* ftab, const uint64_t* horizons,
+                   uint64_t every, uint64_t* slots) {
+  return guarded([&] {
+    SHAW_REQUIRE(ftab, SHAW_ERR_ARG, "null argument");
+    shaw::admit(h, n, b, nsteps, ftab, nullptr, nullptr, nullptr, horizons, every, slots);
+  });
+}
+
+int shaw_rom_admit_signals(shaw_rom_t h, uint64_t n, const double* b, const int* kinds, const double* delays,
+                           const double* periods, uint64_t nsteps, const uint64_t* horizons, uint64_t every,
+                           uint64_t* slots) {
+  return guarded([&] {
+    SHAW_REQUIRE(kinds && delays && periods, SHAW_ERR_ARG, "null argument");
+    shaw::admit(h, n, b, nsteps, nullptr, kinds, delays, periods, horizons, every, slots);
   });
 }
 

@@ -5,7 +5,18 @@
 #include <cstdint>
 #include <vector>
 
+#include "shaw_app.hpp"
+
 namespace shaw {
+// device_setup.hip: the operators of `mat` on app's mesh assembled on the
+// device into coef_vp [nVp][4], coef_sp [nSp][2], rho_inv [nVp] (device
+// doubles, any may be null), equal to Operators::build bit for bit; and the
+// forcing table of m signals, out[s * m + k] (ldOut = 0) or out[k * ldOut + s]
+// (device), equal to forcingTable up to the device exp/sin rounding
+void assembleOperatorsDevice(shaw_app& app, const Material& mat, double* coefVp, double* coefSp, double* rhoInv,
+                             hipStream_t stream);
+void forcingTableDevice(uint64_t m, const int* kinds, const double* delays, const double* periods, double dt,
+                        uint64_t nsteps, double* out, uint64_t ldOut, hipStream_t stream);
 // POD of device-resident snapshots dS [n x ns] column-major (overwritten);
 // phi: host [n x k] column-major, sigma: host [ns] (may be null).
 // kOut == nullptr: exactly k modes or an error; else up to k modes, stopping at
