@@ -444,7 +444,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--rom-steps", type=int, default=20000)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
-    ap.add_argument("--config5", action="store_true", help="also measure the large FOM (2048 x 8192, M = 64)")
+    ap.add_argument("--no-config5", action="store_true", help="skip the large FOM (2048 x 8192, M = 64)")
     ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--c5-err-steps", type=int, default=8000)
     ap.add_argument("--rom-kernel", action="store_true", help="time only the batched ROM kernel (profiling)")
@@ -551,7 +551,7 @@ def main() -> int:
         out["serving"] = serving_section(app, w, basis, args)
     if not args.no_config1:
         out["fom_config1"] = config1_section(args)
-    if args.config5:
+    if not args.no_config5:
         out["fom_config5"] = config5_section(args)
     if not args.no_cpu:
         log(f"cpu baseline: oracle, {args.cpu_steps} steps")
