@@ -422,8 +422,14 @@ typedef struct {
                       uint64_t* samples);
   int (*free_slots)(void* ctx, uint64_t* n);
   int (*query)(void* ctx, int* idle);  /* non-blocking: *idle = 1 when all enqueued work is done */
+  /* admission with the forcing series evaluated by the engine from signal
+   * parameters (shaw_rom_admit_signals); NULL: requests need a table      */
+  int (*admit_signals)(void* ctx, uint64_t n, const double* b, const int* kinds, const double* delays,
+                       const double* periods, uint64_t nsteps, const uint64_t* horizons, uint64_t every,
+                       uint64_t* slots);
 } shaw_engine;
-/* The table of a GPU engine: shaw_rom_admit/advance/sync/retire_batch/free_slots. */
+/* The table of a GPU engine: shaw_rom_admit/advance/sync/retire_batch/free_slots/
+ * query/admit_signals. */
 int shaw_engine_from_rom(shaw_rom_t rom, shaw_engine* out);
 
 typedef struct {
@@ -438,6 +444,11 @@ typedef struct {
   double* seis;          /* out: [m][r x horizon/every] (NULL: discarded) */
   double admitted_s;     /* out: admission time */
   double done_s;         /* out: completion time (latency = done_s - arrival_s) */
+  /* instead of ftab (NULL): each column's signal, kinds/delays/periods [m]
+   * as shaw_forcing_table, evaluated by the engine (admit_signals)        */
+  const int* kinds;
+  const double* delays;
+  const double* periods;
 } shaw_request;
 
 #define SHAW_SERVE_FIFO 0        /* arrival order; the head is never passed */

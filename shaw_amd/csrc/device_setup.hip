@@ -125,17 +125,17 @@ __global__ void k_assemble_sp(DevGrid g, int kind, const double* __restrict__ mp
   }
 }
 
-// out[s * m + k] = signal_k(s * dt) (forcingTable's time: the product s dt)
-__global__ void k_forcing_table(uint64_t m, const int* __restrict__ kinds, const double* __restrict__ delays,
-                                const double* __restrict__ periods, double dt, uint64_t nsteps, double* __restrict__ out,
-                                uint64_t ldOut) {
+// out[s * m + k] = signal_k(s * dt) (forcingTable's time: the product s dt);
+// signal k: kind (as a double), delay, period at par[3 k .. 3 k + 2]
+__global__ void k_forcing_table(uint64_t m, const double* __restrict__ par, double dt, uint64_t nsteps,
+                                double* __restrict__ out, uint64_t ldOut) {
   const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m * nsteps) return;
   const uint64_t s = e / m, k = e % m;
-  const double period = periods[k];
+  const double period = par[3 * k + 2];
   const double freq = 1.0 / period;
   const double t = (double)s * dt;
-  out[ldOut ? k * ldOut + s : e] = signalAt(kinds[k], delays[k], period, freq * freq, t);
+  out[ldOut ? k * ldOut + s : e] = signalAt((int)par[3 * k], par[3 * k + 1], period, freq * freq, t);
 }
 
 // nearest vp point of each target (r [m], theta [rad]) by polar distance,
@@ -265,23 +265,35 @@ void assembleOperatorsDevice(shaw_app& app, const Material& mat, double* coefVp,
   SHAW_HIP(hipStreamSynchronize(stream));  // the uploaded tables are freed on return
 }
 
-void forcingTableDevice(uint64_t m, const int* kinds, const double* delays, const double* periods, double dt,
-                        uint64_t nsteps, double* out, uint64_t ldOut, hipStream_t stream) {
+std::vector<double> signalParams(uint64_t m, const int* kinds, const double* delays, const double* periods) {
+  std::vector<double> par(3 * m);
   for (uint64_t k = 0; k < m; ++k) {
     SHAW_REQUIRE(kinds[k] >= 0 && kinds[k] <= 2, SHAW_ERR_ARG, "unknown signal kind");
     SHAW_REQUIRE(periods[k] > 0, SHAW_ERR_ARG, "Period for signal must be positive.");
+    par[3 * k] = (double)kinds[k];
+    par[3 * k + 1] = delays[k];
+    par[3 * k + 2] = periods[k];
   }
-  if (m == 0 || nsteps == 0) return;
-  Buf<int> dk;
-  Buf<double> dd, dp;
-  dk.upload(kinds, m);
-  dd.upload(delays, m);
-  dp.upload(periods, m);
+  return par;
+}
+
+void forcingTableFromParams(uint64_t m, const double* parDevice, double dt, uint64_t nsteps, double* out,
+                            uint64_t ldOut, hipStream_t stream) {
   const uint64_t n = m * nsteps;
-  hipLaunchKernelGGL(k_forcing_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, m, dk.p, dd.p, dp.p,
-                     dt, nsteps, out, ldOut);
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_forcing_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, m, parDevice, dt,
+                     nsteps, out, ldOut);
   SHAW_HIP(hipGetLastError());
-  SHAW_HIP(hipStreamSynchronize(stream));  // the parameter arrays are freed on return
+}
+
+void forcingTableDevice(uint64_t m, const int* kinds, const double* delays, const double* periods, double dt,
+                        uint64_t nsteps, double* out, uint64_t ldOut, hipStream_t stream) {
+  const std::vector<double> par = signalParams(m, kinds, delays, periods);
+  if (m == 0 || nsteps == 0) return;
+  Buf<double> dp;
+  dp.upload(par.data(), par.size());
+  forcingTableFromParams(m, dp.p, dt, nsteps, out, ldOut, stream);
+  SHAW_HIP(hipStreamSynchronize(stream));  // the parameter array is freed on return
 }
 
 }  // namespace shaw

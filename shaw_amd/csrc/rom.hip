@@ -1206,6 +1206,8 @@ void admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const dou
       h->maxSteps = need;
     }
     // stage [n][Kvp] forcing columns and [n][nsteps] forcing series, one copy, one scatter
+    // (with signals the parameters [n][3] follow the series region, and only
+    // b and the parameters cross PCIe)
     std::vector<double> st(n * (h->kvp + (ftab ? nsteps : 0)), 0.0);
     for (uint64_t j = 0; j < n; ++j) {
       for (uint64_t k = 0; k < h->kv; ++k) st[j * h->kvp + k] = b[k * n + j];
@@ -1213,10 +1215,15 @@ void admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const dou
       double* f = st.data() + n * h->kvp + j * nsteps;
       for (uint64_t t = 0; t < nsteps; ++t) f[t] = ftab[t * n + j];
     }
-    if (h->stage.n < n * (h->kvp + nsteps)) h->stage.alloc(n * (h->kvp + nsteps));
+    const uint64_t stageNeed = n * (h->kvp + nsteps) + (ftab ? 0 : 3 * n);
+    if (h->stage.n < stageNeed) h->stage.alloc(stageNeed);
     h->stage.upload(st.data(), st.size());
-    if (!ftab)  // the series [n][nsteps] of the staging buffer, on the device
-      forcingTableDevice(n, kinds, delays, periods, h->dt, nsteps, h->stage.p + n * h->kvp, nsteps, h->stream);
+    if (!ftab) {  // the series [n][nsteps] of the staging buffer, evaluated on the device
+      const std::vector<double> par = signalParams(n, kinds, delays, periods);
+      double* dPar = h->stage.p + n * (h->kvp + nsteps);
+      SHAW_HIP(hipMemcpy(dPar, par.data(), par.size() * sizeof(double), hipMemcpyHostToDevice));
+      forcingTableFromParams(n, dPar, h->dt, nsteps, h->stage.p + n * h->kvp, nsteps, h->stream);
+    }
     if (h->stageSlots.n < n) h->stageSlots.alloc(n);
     h->stageSlots.upload(free.data(), n);
     hipLaunchKernelGGL(k_admit_scatter, dim3(64, (unsigned)n), dim3(256), 0, h->stream, h->stageSlots.p, n, h->stage.p,

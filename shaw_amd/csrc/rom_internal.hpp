@@ -17,6 +17,11 @@ void assembleOperatorsDevice(shaw_app& app, const Material& mat, double* coefVp,
                              hipStream_t stream);
 void forcingTableDevice(uint64_t m, const int* kinds, const double* delays, const double* periods, double dt,
                         uint64_t nsteps, double* out, uint64_t ldOut, hipStream_t stream);
+// the same with the parameters already on the device (par [m][3]: kind,
+// delay, period, as signalParams packs them); only enqueues
+std::vector<double> signalParams(uint64_t m, const int* kinds, const double* delays, const double* periods);
+void forcingTableFromParams(uint64_t m, const double* parDevice, double dt, uint64_t nsteps, double* out,
+                            uint64_t ldOut, hipStream_t stream);
 // POD of device-resident snapshots dS [n x ns] column-major (overwritten);
 // phi: host [n x k] column-major, sigma: host [ns] (may be null).
 // kOut == nullptr: exactly k modes or an error; else up to k modes, stopping at

@@ -87,7 +87,10 @@ void serve(const shaw_engine* eng, int ne, const shaw_serve_options& o, shaw_req
     const shaw_engine& e = eng[r.engine];
     SHAW_REQUIRE(r.m > 0 && r.m <= e.capacity, SHAW_ERR_ARG,
                  "request " + std::to_string(r.id) + ": columns exceed the engine capacity");
-    SHAW_REQUIRE(r.horizon > 0 && r.b && r.ftab, SHAW_ERR_ARG, "request " + std::to_string(r.id) + ": no inputs");
+    SHAW_REQUIRE(r.horizon > 0 && r.b && (r.ftab || (r.kinds && r.delays && r.periods)), SHAW_ERR_ARG,
+                 "request " + std::to_string(r.id) + ": no inputs");
+    SHAW_REQUIRE(r.ftab || e.admit_signals, SHAW_ERR_ARG,
+                 "request " + std::to_string(r.id) + ": its engine takes forcing tables only");
     footprint[i] = r.m * columnBytes(e, r.horizon, r.every);
     SHAW_REQUIRE(footprint[i] <= o.hbm_budget, SHAW_ERR_ARG,
                  "request " + std::to_string(r.id) + ": footprint exceeds the HBM budget");
@@ -122,7 +125,12 @@ void serve(const shaw_engine* eng, int ne, const shaw_serve_options& o, shaw_req
     const shaw_engine& e = eng[r.engine];
     Live L{i, r.horizon, footprint[i], std::vector<uint64_t>(r.m)};
     std::vector<uint64_t> hz(r.m, r.horizon);
-    check(e.admit(e.ctx, r.m, r.b, r.horizon, r.ftab, hz.data(), r.every, L.slots.data()), "admit");
+    if (r.ftab)
+      check(e.admit(e.ctx, r.m, r.b, r.horizon, r.ftab, hz.data(), r.every, L.slots.data()), "admit");
+    else
+      check(e.admit_signals(e.ctx, r.m, r.b, r.kinds, r.delays, r.periods, r.horizon, hz.data(), r.every,
+                            L.slots.data()),
+            "admit");
     refreshFree(r.engine);
     r.admitted_s = rel();
     maxWait = std::max(maxWait, r.admitted_s - r.arrival_s);
@@ -244,6 +252,10 @@ int romAdmit(void* c, uint64_t n, const double* b, uint64_t nsteps, const double
              uint64_t every, uint64_t* slots) {
   return shaw_rom_admit(static_cast<shaw_rom_t>(c), n, b, nsteps, ftab, h, every, slots);
 }
+int romAdmitSignals(void* c, uint64_t n, const double* b, const int* kinds, const double* delays,
+                    const double* periods, uint64_t nsteps, const uint64_t* h, uint64_t every, uint64_t* slots) {
+  return shaw_rom_admit_signals(static_cast<shaw_rom_t>(c), n, b, kinds, delays, periods, nsteps, h, every, slots);
+}
 int romAdvance(void* c, uint64_t n) { return shaw_rom_advance(static_cast<shaw_rom_t>(c), n); }
 int romSync(void* c) { return shaw_rom_sync(static_cast<shaw_rom_t>(c)); }
 int romRetire(void* c, uint64_t n, const uint64_t* slots, uint64_t maxSamples, double* seis, uint64_t* samples) {
@@ -270,6 +282,7 @@ int shaw_engine_from_rom(shaw_rom_t rom, shaw_engine* out) {
     e.retire_batch = romRetire;
     e.free_slots = romFree;
     e.query = romQuery;
+    e.admit_signals = romAdmitSignals;
     *out = e;
   });
 }

@@ -27,6 +27,9 @@ _SYNC = C.CFUNCTYPE(C.c_int, C.c_void_p)
 _RETIRE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64), C.c_uint64, C.POINTER(C.c_double),
                       C.POINTER(C.c_uint64))
 _FREE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint64))
+_ADMIT_SIG = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                         C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_uint64), C.c_uint64,
+                         C.POINTER(C.c_uint64))
 _QUERY = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_int))
 _CLOCK = C.CFUNCTYPE(C.c_double, C.c_void_p)
 _WAIT = C.CFUNCTYPE(None, C.c_void_p, C.c_double)
@@ -35,14 +38,15 @@ _WAIT = C.CFUNCTYPE(None, C.c_void_p, C.c_double)
 class ShawEngine(C.Structure):
     _fields_ = [("ctx", C.c_void_p), ("kv", C.c_uint64), ("ks", C.c_uint64), ("r", C.c_uint64),
                 ("capacity", C.c_uint64), ("admit", _ADMIT), ("advance", _ADVANCE), ("sync", _SYNC),
-                ("retire_batch", _RETIRE), ("free_slots", _FREE), ("query", _QUERY)]
+                ("retire_batch", _RETIRE), ("free_slots", _FREE), ("query", _QUERY), ("admit_signals", _ADMIT_SIG)]
 
 
 class ShawRequest(C.Structure):
     _fields_ = [("id", C.c_uint64), ("engine", C.c_int), ("m", C.c_uint64), ("horizon", C.c_uint64),
                 ("every", C.c_uint64), ("arrival_s", C.c_double), ("b", C.POINTER(C.c_double)),
                 ("ftab", C.POINTER(C.c_double)), ("seis", C.POINTER(C.c_double)), ("admitted_s", C.c_double),
-                ("done_s", C.c_double)]
+                ("done_s", C.c_double), ("kinds", C.POINTER(C.c_int)), ("delays", C.POINTER(C.c_double)),
+                ("periods", C.POINTER(C.c_double))]
 
 
 class ShawServeOptions(C.Structure):
@@ -75,13 +79,14 @@ class Request:
     rid: int
     k: int
     b: np.ndarray            # [K x m]
-    ftab: np.ndarray         # [horizon x m]
+    ftab: np.ndarray         # [horizon x m], or None with `signals`
     horizon: int
     every: int
     arrival: float           # seconds after the start of the run
     admitted: float = -1.0
     done: float = -1.0
     seismograms: list = field(default_factory=list)  # per column [T x R]
+    signals: tuple = None    # (kinds, delays, periods) per column: the engine evaluates the forcing
 
     @property
     def m(self) -> int:
@@ -132,15 +137,26 @@ class Server:
         for i, r in enumerate(requests):
             e = self.engines[r.k]
             b = np.ascontiguousarray(r.b, dtype=np.float64)
-            f = np.ascontiguousarray(np.asarray(r.ftab)[:r.horizon], dtype=np.float64)
             T = r.horizon // r.every if r.every else 0
             seis = np.zeros((r.m, T, e.R)) if (T and e.R and (keep is None or keep(r))) else None
-            alive += [b, f]
             outs.append(seis)
             q = arr[i]
             q.id, q.engine, q.m, q.horizon, q.every, q.arrival_s = r.rid, index[r.k], r.m, r.horizon, r.every, r.arrival
             q.b = b.ctypes.data_as(C.POINTER(C.c_double))
-            q.ftab = f.ctypes.data_as(C.POINTER(C.c_double))
+            if r.ftab is not None:
+                f = np.ascontiguousarray(np.asarray(r.ftab)[:r.horizon], dtype=np.float64)
+                q.ftab = f.ctypes.data_as(C.POINTER(C.c_double))
+                alive += [b, f]
+            else:
+                kinds, delays, periods = r.signals
+                ka = np.ascontiguousarray([shaw_amd.SIGNAL_KINDS[x] if isinstance(x, str) else int(x) for x in kinds],
+                                          dtype=np.int32)
+                da = np.ascontiguousarray(delays, dtype=np.float64)
+                pa = np.ascontiguousarray(periods, dtype=np.float64)
+                q.kinds = ka.ctypes.data_as(C.POINTER(C.c_int))
+                q.delays = da.ctypes.data_as(C.POINTER(C.c_double))
+                q.periods = pa.ctypes.data_as(C.POINTER(C.c_double))
+                alive += [b, ka, da, pa]
             q.seis = seis.ctypes.data_as(C.POINTER(C.c_double)) if seis is not None else None
         opts = ShawServeOptions()
         opts.hbm_budget, opts.chunk_steps, opts.policy = int(self.budget), self.chunk, self.policy
@@ -164,7 +180,8 @@ class Server:
 
 def poisson_requests(rng, rate: float, duration: float, k_choices, m_range, horizon_range, make_inputs,
                      every: int = 10) -> list:
-    """Requests with Poisson arrivals; make_inputs(k, m, horizon, rng) -> (b, ftab)."""
+    """Requests with Poisson arrivals; make_inputs(k, m, horizon, rng) -> (b,
+    ftab) or (b, None, (kinds, delays, periods)) for engine-evaluated forcing."""
     t, out, rid = 0.0, [], 0
     while True:
         t += rng.exponential(1.0 / rate)
@@ -174,8 +191,8 @@ def poisson_requests(rng, rate: float, duration: float, k_choices, m_range, hori
         m = int(rng.integers(m_range[0], m_range[1] + 1))
         h = int(rng.integers(horizon_range[0], horizon_range[1] + 1))
         h -= h % every
-        b, ftab = make_inputs(k, m, h, rng)
-        out.append(Request(rid, k, b, ftab, h, every, t))
+        inp = make_inputs(k, m, h, rng)
+        out.append(Request(rid, k, inp[0], inp[1], h, every, t, signals=inp[2] if len(inp) > 2 else None))
         rid += 1
 
 
@@ -261,6 +278,17 @@ class NumpyEngine:
             self.calls.append(("admit", int(n)))
             return 0
 
+        def admit_signals(ctx, n, b, kinds, delays, periods, nsteps, horizons, every, slots):
+            ks = [int(kinds[j]) for j in range(n)]
+            ff = shaw_amd.forcing_table(ks, [delays[j] for j in range(n)], [periods[j] for j in range(n)], self.dt,
+                                        int(nsteps))
+            bb = np.ctypeslib.as_array(b, (self.kv, n)).copy()
+            hz = [int(horizons[j]) for j in range(n)]
+            for j, s in enumerate(self.admit(bb, ff, hz, int(every))):
+                slots[j] = s
+            self.calls.append(("admit_signals", int(n)))
+            return 0
+
         def advance(ctx, n):
             self.advance(int(n))
             self.calls.append(("advance", int(n)))
@@ -289,7 +317,8 @@ class NumpyEngine:
         t.kv, t.ks, t.r, t.capacity = self.kv, self.ks, self.R, self.capacity
         t.admit, t.advance, t.sync = _ADMIT(guard(admit)), _ADVANCE(guard(advance)), _SYNC(lambda ctx: 0)
         t.retire_batch, t.free_slots, t.query = _RETIRE(guard(retire)), _FREE(guard(free)), _QUERY(guard(query))
-        self._keep = (t.admit, t.advance, t.sync, t.retire_batch, t.free_slots, t.query)
+        t.admit_signals = _ADMIT_SIG(guard(admit_signals))
+        self._keep = (t.admit, t.advance, t.sync, t.retire_batch, t.free_slots, t.query, t.admit_signals)
         return t
 
 

@@ -205,3 +205,36 @@ def test_unkept_seismograms_are_discarded():
     serving.Server({8: make_engine(8, 16, clock=c2)}, clock=c2, hbm_budget=1e12, chunk=5).run([alone])
     for x, y in zip(reqs[1].seismograms, alone.seismograms):
         np.testing.assert_array_equal(x, y)
+
+
+def test_signal_requests_equal_table_requests():
+    """A request that carries its forcing as signal parameters (the engine
+    evaluates the series at admission) gives the seismograms of the same
+    request with the host table; an engine without that entry refuses it."""
+    clk = serving.SimClock()
+    engines = {8: make_engine(8, 16, clock=clk)}
+    rng = np.random.default_rng(40)
+    kinds, delays, periods = ["ricker", "gaussDer", "sinusoid"], [4.0, 6.0, 2.0], [3.0, 5.0, 9.0]
+    b = rng.standard_normal((8, 3))
+    h = 45
+    tab = shaw_amd.forcing_table(kinds, delays, periods, 0.5, h)
+    sig = serving.Request(0, 8, b, None, h, 5, 0.0, signals=(kinds, delays, periods))
+    ref = serving.Request(1, 8, b, tab, h, 5, 0.0)
+    serve(engines, [sig, ref], clk, hbm_budget=1e12, chunk=10)
+    assert len(sig.seismograms) == 3
+    for x, y in zip(sig.seismograms, ref.seismograms):
+        np.testing.assert_array_equal(x, y)
+    assert any(c[0] == "admit_signals" for c in engines[8].calls)
+    eng = make_engine(8, 16)
+    t = eng.table()
+    t.admit_signals = serving._ADMIT_SIG()  # NULL: tables only
+
+    class TablesOnly:
+        kv, ks, R = eng.kv, eng.ks, eng.R
+
+        def table(self):
+            return t
+
+    with pytest.raises(shaw_amd.ShawError, match="forcing tables only"):
+        serve({8: TablesOnly()}, [serving.Request(2, 8, b, None, h, 5, 0.0, signals=(kinds, delays, periods))],
+              serving.SimClock(), hbm_budget=1e12)
