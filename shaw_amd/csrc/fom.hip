@@ -1975,10 +1975,10 @@ class FomEngine final : public shaw_fom {
   size_t ldsBytesN(uint32_t P, int ns) const {
     return ((size_t)ns * P * 8 + 2 * (size_t)(P + 1) * MC_ + (size_t)ns * P * MC_) * sizeof(T);
   }
-  // k_fusedW: a ring of 4 (3 steps) or 8 (4 to 6 steps) coefficient rows of a 64-point strip
+  // k_fusedW: a ring of 4 (3 steps) or 8 (4 to 7 steps) coefficient rows of a 64-point strip
   // and the forcing increments ([4 waves][steps][2 columns] at most)
   size_t ldsBytesW(int ns) const { return ((size_t)(ns + 1 <= 4 ? 4 : 8) * 64 * 8 + 4 * ns * 2) * sizeof(T); }
-  // 3 to 6 steps per launch run k_fusedW when a strip is one wavefront: one
+  // 3 to 7 steps per launch run k_fusedW when a strip is one wavefront: one
   // thread per point (all columns of the block) and 64-point strips
   bool waveStrip(int cpt, uint32_t P) const { return (uint32_t)cpt == MC_ && P == 64 && cpt <= 2; }
   // column blocks (waves) per k_fusedW workgroup
