@@ -896,16 +896,6 @@ __global__ void __launch_bounds__(512) k_fusedN(StepArgsN<T, NS> aa) {
 // Every value is the one-step kernel's expression in its order, as in
 // k_fusedN.
 
-__device__ __forceinline__ double laneRead(double x, int addr) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
-  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)u);
-  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)(u >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo));
-}
-__device__ __forceinline__ float laneRead(float x, int addr) {
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
-}
-
 // Neighbour exchange along a one-wavefront strip without LDS: DPP whole-wave
 // rotates by one lane (gfx9 family), a VALU move instead of a round trip
 // through the LDS crossbar. fromWest: lane p receives lane p - 1 (lane 0
