@@ -471,8 +471,9 @@ def main() -> int:
     w = workloads.config2()
     material = workloads.layered_material()
     # after the timed region: ROOF_STEPS more steps, timed apart, in whole
-    # launches of the default kernel, give that kernel's launch duration
-    n_total = args.warmup + args.steps + ROOF_STEPS
+    # launches of the default kernel, give that kernel's launch duration; then
+    # the timed steps once more (steady_repeat)
+    n_total = args.warmup + 2 * args.steps + ROOF_STEPS
     log(f"workload {w.describe()}; warmup {args.warmup}, timed {args.steps}")
     t0 = time.time()
     app = shaw_amd.App.structured(w.nr, w.nth, material)
@@ -506,6 +507,10 @@ def main() -> int:
     log(f"{roof_launches} launches of {spl} steps: {launch_ms:.4f} ms per launch, {achieved:.0f} GB/s algorithmic "
         f"({bytes_per_launch / 1e9:.3f} GB per launch), {roof_launches * spl / (roof_ms / 1e3) * w.m:.0f} "
         "forcing-steps/s over that region")
+    # the same run's steady state: the timed region again, right after the
+    # roofline launches (the first timed region starts from an idle GPU)
+    steady_ms, _ = fom.time(args.steps)
+    log(f"steady: {steady_ms:.2f} ms for {args.steps} steps ({args.steps / steady_ms * w.m * 1e3:.0f} forcing-steps/s)")
     fom.sync()
     seis = fom.seismogram()
     assert np.isfinite(seis).all(), "seismogram is not finite"
@@ -537,6 +542,11 @@ def main() -> int:
                      "traffic_GB_s": traffic / (launch_ms / 1e3) / 1e9 if traffic else None,
                      "traffic_frac": traffic / (launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS if traffic else None,
                      "reference_model_bytes_per_step": model_bytes},
+        # the timed steps repeated once the GPU is busy: value's figure
+        # without the ramp from idle of its first launch
+        "steady_repeat": {"ms_per_step": steady_ms / args.steps,
+                          "forcing_steps_per_s": w.m * args.steps / (steady_ms / 1e3),
+                          "value_over_steady": value / (w.m * args.steps / (steady_ms / 1e3))},
     }
     fom.close()
     if args.copy_bytes:
