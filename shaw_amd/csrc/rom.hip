@@ -480,7 +480,9 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_steps_bf(RomArgs a) {
   // (K = 384 and 512, M = 4096: BF16x4 1.77e8 -> 2.06e8 and 7.6e7 -> 1.19e8,
   // BF16x3 3.02e8 -> 3.11e8 and 1.55e8 -> 1.72e8 forcing-steps/s)
   constexpr int TPP = TPW == 4 ? 2 : (TPW == 3 ? 1 : TPW), NP = TPW / TPP;
-  constexpr int D = ringDepth<PARTS, TPP, WAVES, true>();
+  // (BF16x4 at three tiles a wave: a 2-deep ring keeps it within 256 VGPRs
+  // without spills; K = 384, M = 4096: 2.10e8 -> 2.30e8 forcing-steps/s)
+  constexpr int D = (PARTS == 3 && TPW == 3) ? 2 : ringDepth<PARTS, TPP, WAVES, true>();
   using Item = typename Prec<PARTS>::Item;
   using Acc = typename Prec<PARTS>::Acc;
   extern __shared__ double sm[];
