@@ -368,7 +368,9 @@ class Fom:
         _check(lib().shaw_fom_create(app.handle, m, dt, C.byref(o), C.byref(h)))
         self._h, self.app, self.m, self.dt = h, app, m, dt
         self.R = 0
+        self._seis_cap = 0
         self._snap_cols = {0: 0, 1: 0}
+        self._snap_cap = {0: 0, 1: 0}
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -402,14 +404,14 @@ class Fom:
 
     def set_receivers(self, gids, every: int, max_samples: int):
         g = _u64(gids)
-        self.R = len(g)
         _check(lib().shaw_fom_set_receivers(self._h, len(g), _p(g), every, max_samples))
+        self.R, self._seis_cap = len(g), max_samples
 
     def set_snapshots(self, dof: str, every: int, cols, max_samples: int):
         c = _u64(cols)
         d = 0 if dof == "vp" else 1
-        self._snap_cols[d] = len(c)
         _check(lib().shaw_fom_set_snapshots(self._h, d, every, len(c), _p(c), max_samples))
+        self._snap_cols[d], self._snap_cap[d] = len(c), max_samples
 
     def reset(self):
         _check(lib().shaw_fom_reset(self._h))
@@ -428,8 +430,7 @@ class Fom:
 
     def seismogram(self) -> np.ndarray:
         """[m][T][R] (= LayoutLeft [R x T x m])."""
-        every_max = max(1, self.step)
-        out = np.zeros(max(1, self.R * every_max * self.m))
+        out = np.zeros(max(1, self.R * self._seis_cap * self.m))  # at most max_samples samples
         t = C.c_uint64()
         _check(lib().shaw_fom_get_seismogram(self._h, _p(out), C.byref(t)))
         T = int(t.value)
@@ -440,7 +441,7 @@ class Fom:
         d = 0 if dof == "vp" else 1
         n = self.app.nVp if d == 0 else self.app.nSp
         nc = self._snap_cols[d]
-        out = np.zeros(max(1, n * max(1, self.step) * nc))
+        out = np.zeros(max(1, n * self._snap_cap[d] * nc))  # at most max_samples samples
         t = C.c_uint64()
         _check(lib().shaw_fom_get_snapshots(self._h, d, _p(out), C.byref(t)))
         T = int(t.value)

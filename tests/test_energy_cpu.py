@@ -131,3 +131,29 @@ def test_amplification_radius_closed_form(a, s, dt):
     want = 1.0 if t * t < 4.0 else (abs(t) + np.sqrt(t * t - 4.0)) / 2.0
     got = rw.amplification_radius(np.array([[a]]), np.array([[s]]), dt)
     assert got == pytest.approx(want, rel=1e-14)
+
+
+def test_fom_discrete_energy_conserved():
+    """The oracle's FOM (the reference's leapfrog) conserves
+    Q = v^T Mv v + s^T Ms s + dt v^T Mv J_vp s once its forcing has ended, to
+    a few ulp; the energy without the cross term, or with its sign flipped,
+    is not conserved (so the property pins the cross term, which the GPU
+    test of the full config-2 grid relies on)."""
+    nr, nth, dt, n, every = 64, 256, 0.5, 1200, 100
+    app = shaw_amd.App.structured(nr, nth, LAYERED)
+    mv, ms = app.energy_weights()
+    ora = oracle.App("", LAYERED, structured=(nr, nth, 0.0, 180.0, 3480.0, 6371.0))
+    Jv = csr(ora, "vp", (ora.nVp, ora.nSp))
+    # pulses ended (below 1e-30 of their peak) by step 500
+    src = [(300.0, 40.0, 60.0, 60.0), (1200.0, 120.0, 50.0, 80.0)]
+    out = ora.fom_run(dt, n, src, rank2=True, snap_freq_vp=every, snap_freq_sp=every)
+    V, S = out["snaps_vp"], out["snaps_sp"]
+    first = 500 // every  # sample of step 600
+    for m in range(len(src)):
+        e = np.array([V[m, k] @ (mv * V[m, k]) + S[m, k] @ (ms * S[m, k]) for k in range(V.shape[1])])
+        x = np.array([dt * (mv * V[m, k]) @ (Jv @ S[m, k]) for k in range(V.shape[1])])
+        q = e + x
+        assert q[-1] > 0
+        assert np.abs(q[first:] / q[-1] - 1).max() <= 1e-13
+        assert np.ptp(e[first:]) / q[-1] > 1e-4
+        assert np.ptp((e - x)[first:]) / q[-1] > 1e-4
