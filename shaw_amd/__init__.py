@@ -137,6 +137,7 @@ def lib():
         "shaw_rom_retire": (i32, [vp, u64, vp, vp, vp, P(u64)]),
         "shaw_rom_retire_batch": (i32, [vp, u64, vp, u64, vp, vp]),
         "shaw_rom_free_slots": (i32, [vp, P(u64)]),
+        "shaw_rom_set_macro_steps": (i32, [vp, C.c_uint32]),
         "shaw_rom_state": (i32, [vp, u64, vp, vp, P(u64)]),
         "shaw_debug_mfma": (i32, [i32, vp, vp, vp]),
         "shaw_pod_basis": (i32, [vp, u64, u64, u64, vp, vp]),
@@ -691,6 +692,11 @@ class Rom:
         for sl in slots:
             self.every[int(sl)] = every
         return [int(sl) for sl in slots]
+
+    def set_macro_steps(self, m: int):
+        """BF16 engines: quiet, aligned tiles advance m steps per product
+        (shaw_rom_set_macro_steps); 0 turns it off."""
+        _check(lib().shaw_rom_set_macro_steps(self._h, m))
 
     def advance(self, nsteps: int):
         _check(lib().shaw_rom_advance(self._h, nsteps))

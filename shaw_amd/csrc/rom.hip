@@ -189,7 +189,58 @@ struct RomArgs {
   uint32_t Kvp, Ksp, Rp, R;
   uint64_t maxSteps, Tcap, nsteps;
   double dt;
+  // macro steps (BF16 kernels): the blocks of G = (F^m - I) / dt, F the
+  // one-step matrix of [q_v; q_s]: G_vv [Kvp x Kvp], G_vs [Kvp x Ksp], G_sv,
+  // G_ss, packed as the operators; m = 0 turns them off
+  const void *gvv, *gvs, *gsv, *gss;
+  const uint32_t* quiet;  // [slot] first step from which the column's forcing is negligible
+  uint32_t macro;
+  // with macro steps a launch is three kernels per tile: one-step kernel
+  // (phase 1, up to where the tile may take macro steps), macro kernel,
+  // one-step kernel (phase 3, the rest); tileDone [tile] counts the steps done
+  uint32_t* tileDone;
+  uint32_t phase;
 };
+
+// A tile's macro plan for a budget of B steps: k one-steps first (until every
+// active column is at a multiple of m and quiet), then k2 macro steps (every
+// column still active then at least m k2 steps from its horizon, within B).
+// k2 = 0 when the tile cannot take any: active columns at different
+// residues mod m, or sampled at a period that is not a multiple of m. Lanes
+// hold column lane % 16 (each column 4 times); every wave computes the same.
+__device__ __forceinline__ void macroPlan(uint32_t m, uint32_t s, uint32_t hz, uint32_t ev, uint32_t quiet, uint64_t B,
+                                          uint64_t& k, uint64_t& k2) {
+  const bool act = s < hz;
+  int bad = act && ((ev != 0 && ev % m != 0) || quiet == 0xffffffffu);
+  const uint32_t r = s % m;
+  uint32_t rmin = act ? r : 0xffffffffu, rmax = act ? r : 0u;
+  uint32_t kq = act && quiet > s ? quiet - s : 0u;
+#pragma unroll
+  for (int off = 32; off >= 1; off /= 2) {
+    const uint32_t a = (uint32_t)__shfl_xor((int)rmin, off), b = (uint32_t)__shfl_xor((int)rmax, off);
+    const uint32_t c = (uint32_t)__shfl_xor((int)kq, off);
+    rmin = a < rmin ? a : rmin;
+    rmax = b > rmax ? b : rmax;
+    kq = c > kq ? c : kq;
+    bad |= __shfl_xor(bad, off);
+  }
+  k = B;
+  k2 = 0;
+  if (rmin > rmax || bad || rmin != rmax) return;  // no active column, or no common macro point
+  const uint64_t k0 = (uint64_t)kq + (m - (rmin + kq % m) % m) % m;
+  uint64_t lim = ~0ull;  // macro steps the columns still active at k0 allow
+  if (act && (uint64_t)hz > (uint64_t)s + k0) lim = ((uint64_t)hz - s - k0) / m;
+#pragma unroll
+  for (int off = 32; off >= 1; off /= 2) {
+    const uint64_t o = (uint64_t)__shfl_xor((long long)lim, off);
+    lim = o < lim ? o : lim;
+  }
+  if (lim == ~0ull || k0 > B) return;
+  const uint64_t n = (B - k0) / m < lim ? (B - k0) / m : lim;
+  if (n == 0) return;
+  k = k0;
+  k2 = n;
+}
 
 constexpr int kRing = 4;  // operator groups kept in flight per row tile
 
@@ -264,7 +315,8 @@ __device__ __forceinline__ void preloadRing(const void* __restrict__ Araw, int k
 
 // acc[t] = (row tile rt[t] of A) X over all kGroups groups; the ring holds
 // groups 0 .. D-1 on entry (preloadRing) and is refilled D groups ahead
-template <int PARTS, int TPW, int D>
+// (ZERO = false: add to acc instead, for a product of two operator blocks)
+template <int PARTS, int TPW, int D, bool ZERO = true>
 __device__ __forceinline__ void consumeRing(const void* __restrict__ Araw, int kGroups, const int (&rt)[TPW],
                                             const void* X, int loOff, int lane,
                                             typename Prec<PARTS>::Item (&ring)[D][TPW],
@@ -274,7 +326,7 @@ __device__ __forceinline__ void consumeRing(const void* __restrict__ Araw, int k
   const Item* row[TPW];
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
-    acc[t] = typename Prec<PARTS>::Acc{0, 0, 0, 0};
+    if constexpr (ZERO) acc[t] = typename Prec<PARTS>::Acc{0, 0, 0, 0};
     row[t] = A + (size_t)rt[t] * kGroups * 64 + lane;
   }
   int g0 = 0;
@@ -538,7 +590,19 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_steps_bf(RomArgs a) {
     const uint32_t o = (uint32_t)__shfl_xor((int)steps, off);
     steps = o > steps ? o : steps;
   }
-  if ((uint64_t)steps > a.nsteps) steps = (uint32_t)a.nsteps;
+  // this launch's steps for the tile; with macro steps, phase 1 runs up to
+  // the tile's macro point and phase 3 what phases 1 and 2 left
+  uint64_t budget = a.nsteps;
+  if (a.macro) {
+    if (a.phase == 1) {
+      uint64_t k, k2;
+      macroPlan(a.macro, s0, hz, ev, a.quiet[slot0 + col], a.nsteps, k, k2);
+      budget = k;
+    } else {
+      budget = a.nsteps - a.tileDone[blockIdx.x];
+    }
+  }
+  if ((uint64_t)steps > budget) steps = (uint32_t)budget;
   Item ring[D][TPP];
   Acc acc[TPP];
   if (steps) preloadRing<PARTS, TPP, D>(a.av, kGroupsS, rtVp[0], lane, ring);
@@ -618,13 +682,169 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_steps_bf(RomArgs a) {
     }
   if (tid < kTileCols) {
     const uint64_t h = a.horizon[slot0 + tid];
-    const uint64_t st = a.stepCol[slot0 + tid], s = st + a.nsteps;
+    const uint64_t st = a.stepCol[slot0 + tid], s = st + budget;
     a.stepCol[slot0 + tid] = s < h ? s : (st < h ? h : st);
   }
+  if (a.macro && a.phase == 1 && tid == 0) a.tileDone[blockIdx.x] = (uint32_t)budget;
+}
+
+// Macro steps (phase 2 of a launch with macro steps on): a tile whose active
+// columns are all at a multiple of m and quiet advances k2 macro steps of m
+// steps each, x += dt (G x) with x = [q_v; q_s] and G = (F^m - I) / dt, F the
+// leapfrog's one-step matrix (no forcing: every column is quiet). The
+// layout, splits, operator ring and receivers are k_rom_steps_bf's; G's
+// blocks G_vv, G_vs (rows of q_v) and G_sv, G_ss (rows of q_s) are four
+// products per pass, read from the splits of the old state, and the new
+// states stay in the registers until every wave is done with the old splits.
+template <int PARTS, int TPW, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) k_rom_macro_bf(RomArgs a) {
+  constexpr int TPP = TPW == 4 ? 2 : (TPW == 3 ? 1 : TPW), NP = TPW / TPP;
+  // (a 2-deep ring at three tiles a wave: its four blocks a pass otherwise spill)
+  constexpr int D = TPW == 3 ? 2 : ringDepth<PARTS, TPP, WAVES, true>();
+  using Item = typename Prec<PARTS>::Item;
+  using Acc = typename Prec<PARTS>::Acc;
+  extern __shared__ double sm[];
+  int4v* Sv = reinterpret_cast<int4v*>(sm);
+  int4v* Ss = Sv + 4 * (size_t)a.Kvp;
+  const int loV = 2 * (int)a.Kvp, loS = 2 * (int)a.Ksp;
+  const int slot0 = blockIdx.x * kTileCols;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64, col = lane % 16;
+  const uint32_t m = a.macro;
+  const uint32_t s0 = (uint32_t)a.stepCol[slot0 + col];
+  const uint32_t hz = (uint32_t)a.horizon[slot0 + col];
+  const uint32_t ev = (uint32_t)a.every[slot0 + col];
+  uint64_t k, n;
+  macroPlan(m, s0, hz, ev, a.quiet[slot0 + col], a.nsteps - a.tileDone[blockIdx.x], k, n);
+  if (k != 0 || n == 0) return;  // block-uniform: phase 1 stopped at the macro point, or no macro steps
+  const int kvTiles = a.Kvp / 16, ksTiles = a.Ksp / 16;
+  const int kGroupsS = a.Ksp / 32, kGroupsV = a.Kvp / 32;
+  int rtV[TPW], rtS[TPW];
+  bool okV[TPW], okS[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    okV[t] = wave + t * WAVES < kvTiles;
+    okS[t] = wave + t * WAVES < ksTiles;
+    rtV[t] = okV[t] ? wave + t * WAVES : (wave < kvTiles ? wave : 0);
+    rtS[t] = okS[t] ? wave + t * WAVES : (wave < ksTiles ? wave : 0);
+  }
+  int rtVp[NP][TPP], rtSp[NP][TPP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int j = 0; j < TPP; ++j) {
+      rtVp[p][j] = rtV[p * TPP + j];
+      rtSp[p][j] = rtS[p * TPP + j];
+    }
+  const uint64_t colV = (uint64_t)(slot0 + col) * a.Kvp, colS = (uint64_t)(slot0 + col) * a.Ksp;
+  double qv[TPW][4], qs[TPW][4];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      qv[t][r] = okV[t] ? a.qv[colV + rtV[t] * 16 + rowBf16(lane, r)] : 0.0;
+      qs[t][r] = okS[t] ? a.qs[colS + rtS[t] * 16 + rowBf16(lane, r)] : 0.0;
+    }
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    if (okV[t]) writeSplitRows(Sv, loV, rtV[t], lane, qv[t]);
+    if (okS[t]) writeSplitRows(Ss, loS, rtS[t], lane, qs[t]);
+  }
+  const bool act = s0 < hz;
+  Item ring[D][TPP];
+  Acc acc[TPP];
+  preloadRing<PARTS, TPP, D>(a.gvv, kGroupsV, rtVp[0], lane, ring);
+  __syncthreads();  // both splits are in LDS
+  for (uint64_t it = 0; it < n; ++it) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      consumeRing<PARTS, TPP, D>(a.gvv, kGroupsV, rtVp[p], Sv, loV, lane, ring, acc);
+      preloadRing<PARTS, TPP, D>(a.gvs, kGroupsS, rtVp[p], lane, ring);
+      consumeRing<PARTS, TPP, D, false>(a.gvs, kGroupsS, rtVp[p], Ss, loS, lane, ring, acc);
+      if (p + 1 < NP) preloadRing<PARTS, TPP, D>(a.gvv, kGroupsV, rtVp[p + 1 < NP ? p + 1 : 0], lane, ring);
+      else preloadRing<PARTS, TPP, D>(a.gsv, kGroupsV, rtSp[0], lane, ring);
+#pragma unroll
+      for (int j = 0; j < TPP; ++j)
+        if (act && okV[p * TPP + j])
+#pragma unroll
+          for (int r = 0; r < 4; ++r) qv[p * TPP + j][r] = qv[p * TPP + j][r] + a.dt * (double)acc[j][r];
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      consumeRing<PARTS, TPP, D>(a.gsv, kGroupsV, rtSp[p], Sv, loV, lane, ring, acc);
+      preloadRing<PARTS, TPP, D>(a.gss, kGroupsS, rtSp[p], lane, ring);
+      consumeRing<PARTS, TPP, D, false>(a.gss, kGroupsS, rtSp[p], Ss, loS, lane, ring, acc);
+      if (p + 1 < NP) preloadRing<PARTS, TPP, D>(a.gsv, kGroupsV, rtSp[p + 1 < NP ? p + 1 : 0], lane, ring);
+      else if (it + 1 < n) preloadRing<PARTS, TPP, D>(a.gvv, kGroupsV, rtVp[0], lane, ring);
+#pragma unroll
+      for (int j = 0; j < TPP; ++j)
+        if (act && okS[p * TPP + j])
+#pragma unroll
+          for (int r = 0; r < 4; ++r) qs[p * TPP + j][r] = qs[p * TPP + j][r] + a.dt * (double)acc[j][r];
+    }
+    __syncthreads();  // every read of the old splits done
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      if (okV[t]) writeSplitRows(Sv, loV, rtV[t], lane, qv[t]);
+      if (okS[t]) writeSplitRows(Ss, loS, rtS[t], lane, qs[t]);
+    }
+    __syncthreads();  // the new splits complete
+    // the macro step's last step is sampled when it ends at a multiple of the period
+    const uint64_t sn = (uint64_t)s0 + (it + 1) * m;
+    const bool smp = act && ev > 0 && sn % ev == 0;
+    if ((__ballot(smp) & 0xffffull) != 0)
+      receivers<PARTS, 1, WAVES, 2>(a, Sv, loV, wave, lane, smp, smp ? (uint32_t)(sn / ev - 1) : 0u, slot0);
+  }
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (okV[t]) a.qv[colV + rtV[t] * 16 + rowBf16(lane, r)] = qv[t][r];
+      if (okS[t]) a.qs[colS + rtS[t] * 16 + rowBf16(lane, r)] = qs[t][r];
+    }
+  __syncthreads();  // every wave has read the step counters
+  if (tid < kTileCols) {
+    const uint64_t h = a.horizon[slot0 + tid];
+    const uint64_t st = a.stepCol[slot0 + tid], sn = st + n * m;
+    a.stepCol[slot0 + tid] = st < h ? (sn < h ? sn : h) : st;
+  }
+  if (tid == 0) a.tileDone[blockIdx.x] += (uint32_t)(n * m);
 }
 
 // admission: column j of the staged inputs goes to slot slots[j]; its states
 // start at zero
+// admission: the first step from which the staged forcing series of
+// column j ([n][nsteps]) stays at or below 2^-64 of its largest magnitude (0
+// when it is all zero) goes to quiet[slots[j]]. A forcing value that small
+// adds under 2^-64 of the column's response, far below the BF16 kernels'
+// rounding, so a quiet column may take macro steps without its forcing.
+__global__ void __launch_bounds__(256) k_quiet(const double* __restrict__ f, uint64_t nsteps,
+                                               const uint64_t* __restrict__ slots, uint32_t* __restrict__ quiet) {
+  __shared__ double peak[256];
+  __shared__ uint64_t last[256];
+  const uint64_t j = blockIdx.x;
+  const unsigned t = threadIdx.x;
+  const double* fj = f + j * nsteps;
+  double mx = 0.0;
+  for (uint64_t e = t; e < nsteps; e += 256) mx = fmax(mx, fabs(fj[e]));
+  peak[t] = mx;
+  __syncthreads();
+  for (unsigned w = 128; w > 0; w /= 2) {
+    if (t < w) peak[t] = fmax(peak[t], peak[t + w]);
+    __syncthreads();
+  }
+  const double thr = peak[0] * 0x1p-64;
+  uint64_t lastAbove = 0;  // 1 + the last index above the threshold
+  for (uint64_t e = t; e < nsteps; e += 256)
+    if (fabs(fj[e]) > thr) lastAbove = e + 1;
+  last[t] = lastAbove;
+  __syncthreads();
+  for (unsigned w = 128; w > 0; w /= 2) {
+    if (t < w) last[t] = last[t] > last[t + w] ? last[t] : last[t + w];
+    __syncthreads();
+  }
+  if (t == 0) quiet[slots[j]] = last[0] < 0xffffffffull ? (uint32_t)last[0] : 0xffffffffu;
+}
+
 __global__ void k_admit_scatter(const uint64_t* __restrict__ slots, uint64_t n, const double* __restrict__ bStage,
                                 const double* __restrict__ fStage, uint32_t kvp, uint32_t ksp, uint64_t nsteps,
                                 uint64_t maxSteps, double* __restrict__ b, double* __restrict__ ftab,
@@ -700,23 +920,24 @@ std::vector<double> packBf16(const double* A, uint64_t rows, uint64_t cols, uint
 }
 
 template <int PARTS, int TPW, int WAVES>
-void launchRomKernel(dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a) {
+void launchRomKernel(dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a, bool macroKernel) {
   auto go = [&](auto kernel) {
     (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(kernel, grid, dim3(WAVES * 64), lds, stream, a);
   };
   if constexpr (PARTS == 0) go(k_rom_steps<0, TPW, WAVES>);
+  else if (macroKernel) go(k_rom_macro_bf<PARTS, TPW, WAVES>);
   else go(k_rom_steps_bf<PARTS, TPW, WAVES>);
 }
 
 template <int PARTS>
-void launchRomSteps(int tpw, dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a) {
+void launchRomSteps(int tpw, dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a, bool macroKernel = false) {
   switch (tpw) {
-    case 0: launchRomKernel<PARTS, 1, 16>(grid, lds, stream, a); break;  // 9..16 row tiles: one per wave of 16
-    case 1: launchRomKernel<PARTS, 1, kWaves>(grid, lds, stream, a); break;
-    case 2: launchRomKernel<PARTS, 2, kWaves>(grid, lds, stream, a); break;
-    case 3: launchRomKernel<PARTS, 3, kWaves>(grid, lds, stream, a); break;
-    default: launchRomKernel<PARTS, 4, kWaves>(grid, lds, stream, a); break;
+    case 0: launchRomKernel<PARTS, 1, 16>(grid, lds, stream, a, macroKernel); break;  // 9..16 row tiles: one per wave of 16
+    case 1: launchRomKernel<PARTS, 1, kWaves>(grid, lds, stream, a, macroKernel); break;
+    case 2: launchRomKernel<PARTS, 2, kWaves>(grid, lds, stream, a, macroKernel); break;
+    case 3: launchRomKernel<PARTS, 3, kWaves>(grid, lds, stream, a, macroKernel); break;
+    default: launchRomKernel<PARTS, 4, kWaves>(grid, lds, stream, a, macroKernel); break;
   }
 }
 
@@ -873,6 +1094,13 @@ struct shaw_rom {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int tpw = 1;  // row tiles per wave
   shaw::DevBuf<double> av, as, c, qv, qs, b, ftab, seis;  // av/as/c: packed operator items
+  // macro steps (BF16 engines): the FP64 operators they are built from, the
+  // packed blocks of G = (F^m - I) / dt, m (0: off) and each slot's first
+  // quiet step
+  std::vector<double> hArV, hArS;
+  shaw::DevBuf<double> gvv, gvs, gsv, gss;
+  shaw::DevBuf<uint32_t> quiet, tileDone;
+  uint32_t macro = 0, macroBuilt = 0;
   shaw::DevBuf<uint64_t> stepCol, horizon, every;
   std::vector<uint64_t> hStep, hHorizon, hEvery;
   std::vector<uint8_t> used;
@@ -918,12 +1146,25 @@ struct shaw_rom {
     a.stepCol = stepCol.p; a.horizon = horizon.p; a.every = every.p; a.seis = seis.p;
     a.Kvp = kvp; a.Ksp = ksp; a.Rp = rp; a.R = (uint32_t)R;
     a.maxSteps = maxSteps; a.Tcap = maxSamples; a.nsteps = nsteps; a.dt = dt;
+    a.gvv = gvv.p; a.gvs = gvs.p; a.gsv = gsv.p; a.gss = gss.p;
+    a.quiet = quiet.p;
+    a.macro = precision == SHAW_ROM_FP64 ? 0 : macro;
+    a.tileDone = tileDone.p;
+    a.phase = 1;
     const dim3 grid((unsigned)(maxCols / kTileCols));
     const size_t lds = ldsBytes();
-    if (precision == SHAW_ROM_FP64) launchRomSteps<0>(tpw, grid, lds, stream, a);
-    else if (precision == SHAW_ROM_BF16X3) launchRomSteps<2>(tpw, grid, lds, stream, a);
-    else launchRomSteps<3>(tpw, grid, lds, stream, a);
-    SHAW_HIP(hipGetLastError());
+    auto run = [&](bool macroKernel) {
+      if (precision == SHAW_ROM_FP64) launchRomSteps<0>(tpw, grid, lds, stream, a);
+      else if (precision == SHAW_ROM_BF16X3) launchRomSteps<2>(tpw, grid, lds, stream, a, macroKernel);
+      else launchRomSteps<3>(tpw, grid, lds, stream, a, macroKernel);
+      SHAW_HIP(hipGetLastError());
+    };
+    run(false);
+    if (a.macro) {  // the tiles' macro steps, then the steps they leave
+      run(true);
+      a.phase = 3;
+      run(false);
+    }
     for (uint64_t c = 0; c < maxCols; ++c) {
       const uint64_t s = hStep[c] + nsteps;
       hStep[c] = s < hHorizon[c] ? s : std::max(hStep[c], hHorizon[c]);
@@ -1159,12 +1400,19 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
     };
     pack(d->ar_v, d->kv, d->ks, h->kvp, h->ksp, h->av);
     pack(d->ar_s, d->ks, d->kv, h->ksp, h->kvp, h->as);
+    if (d->precision != SHAW_ROM_FP64) {
+      h->hArV.assign(d->ar_v, d->ar_v + d->kv * d->ks);
+      h->hArS.assign(d->ar_s, d->ar_s + d->ks * d->kv);
+    }
     pack(cPad.data(), h->rp, d->kv, h->rp, h->kvp, h->c);
     const uint64_t C = h->maxCols;
     h->qv.alloc(C * h->kvp); h->qs.alloc(C * h->ksp); h->b.alloc(C * h->kvp);
     SHAW_HIP(hipMemset(h->qv.p, 0, h->qv.n * 8)); SHAW_HIP(hipMemset(h->qs.p, 0, h->qs.n * 8));
     SHAW_HIP(hipMemset(h->b.p, 0, h->b.n * 8));
     h->stepCol.alloc(C); h->horizon.alloc(C); h->every.alloc(C);
+    h->quiet.alloc(C);
+    SHAW_HIP(hipMemset(h->quiet.p, 0xff, C * sizeof(uint32_t)));  // never quiet until admitted
+    h->tileDone.alloc(C / kTileCols);
     h->hStep.assign(C, 0); h->hHorizon.assign(C, 0); h->hEvery.assign(C, 0); h->used.assign(C, 0);
     h->stepCol.upload(h->hStep.data(), C); h->horizon.upload(h->hHorizon.data(), C); h->every.upload(h->hEvery.data(), C);
     h->seis.alloc(C * std::max<uint64_t>(h->R, 1) * std::max<uint64_t>(h->maxSamples, 1));
@@ -1231,6 +1479,9 @@ void admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const dou
     hipLaunchKernelGGL(k_admit_scatter, dim3(64, (unsigned)n), dim3(256), 0, h->stream, h->stageSlots.p, n, h->stage.p,
                        h->stage.p + n * h->kvp, h->kvp, h->ksp, nsteps, h->maxSteps, h->b.p, h->ftab.p, h->qv.p, h->qs.p);
     SHAW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_quiet, dim3((unsigned)n), dim3(256), 0, h->stream, h->stage.p + n * h->kvp, nsteps,
+                       h->stageSlots.p, h->quiet.p);
+    SHAW_HIP(hipGetLastError());
     for (uint64_t j = 0; j < n; ++j) {
       const uint64_t s = free[j];
       h->hStep[s] = 0;
@@ -1261,6 +1512,31 @@ int shaw_rom_admit_signals(shaw_rom_t h, uint64_t n, const double* b, const int*
   return guarded([&] {
     SHAW_REQUIRE(kinds && delays && periods, SHAW_ERR_ARG, "null argument");
     shaw::admit(h, n, b, nsteps, nullptr, kinds, delays, periods, horizons, every, slots);
+  });
+}
+
+int shaw_rom_set_macro_steps(shaw_rom_t h, uint32_t m) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle");
+    SHAW_REQUIRE(m == 0 || h->precision != SHAW_ROM_FP64, SHAW_ERR_ARG, "macro steps run in the BF16 kernels");
+    SHAW_REQUIRE(m == 0 || (m >= 2 && m <= 64), SHAW_ERR_ARG, "macro steps: m must be 0 or 2..64");
+    SHAW_HIP(hipStreamSynchronize(h->stream));
+    if (m && m != h->macroBuilt) {
+      std::vector<double> g[4];
+      macroOperator(h->hArV.data(), h->hArS.data(), h->kv, h->ks, h->dt, m, g[0], g[1], g[2], g[3]);
+      const int parts = h->precision == SHAW_ROM_BF16X4 ? 3 : 2;
+      const uint64_t rows[4] = {h->kv, h->kv, h->ks, h->ks}, cols[4] = {h->kv, h->ks, h->kv, h->ks};
+      const uint32_t rowsP[4] = {h->kvp, h->kvp, h->ksp, h->ksp}, colsP[4] = {h->kvp, h->ksp, h->kvp, h->ksp};
+      DevBuf<double>* dst[4] = {&h->gvv, &h->gvs, &h->gsv, &h->gss};
+      for (int q = 0; q < 4; ++q) {
+        const std::vector<double> p = packBf16(g[q].data(), rows[q], cols[q], rowsP[q], colsP[q], parts);
+        dst[q]->alloc(p.size());
+        dst[q]->upload(p.data(), p.size());
+      }
+      h->macroBuilt = m;
+    }
+    h->macro = m;
   });
 }
 

@@ -49,6 +49,11 @@ uint32_t projChunks(uint64_t n, uint64_t Ka, uint64_t Kb);
 // out (device, [n x kout] column-major) = base + alpha * X W in FP64 on MFMA
 // (proj.hip): X [n x kin] column-major, W [kin x kout] row-major, both on
 // the device; base may be null (0) or alias out.
+// G = (F^m - I) / dt of the reduced leapfrog's one-step matrix F, in blocks
+// (row-major G_vv [kv x kv], G_vs [kv x ks], G_sv [ks x kv], G_ss [ks x ks])
+void macroOperator(const double* arV, const double* arS, uint64_t kv, uint64_t ks, double dt, uint32_t m,
+                   std::vector<double>& gvv, std::vector<double>& gvs, std::vector<double>& gsv,
+                   std::vector<double>& gss);
 void gemmNN(const double* X, uint64_t n, uint32_t kin, const double* W, uint32_t kout, double* out,
             const double* base, double alpha, hipStream_t stream);
 uint64_t projPartDoubles(uint64_t n, uint64_t Ka, uint64_t Kb);

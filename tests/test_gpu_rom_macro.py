@@ -1,0 +1,122 @@
+"""Macro steps of the BF16 ROM kernels (shaw_rom_set_macro_steps): tiles
+whose columns are quiet and aligned advance m steps per product by
+G = (F^m - I) / dt. Checked against the FP64 ROM (the one-step reference)
+at the bounds the one-step BF16 runs meet, on every row-tile layout (K = 128,
+256, 384, 512), with horizons that are not multiples of m, sampling, a tile
+whose columns sit at different residues mod m (no macro steps), and chunked
+launches; and the step counters and seismogram lengths equal the one-step
+run's exactly."""
+import sys
+
+import numpy as np
+import pytest
+
+import shaw_amd
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def problem(K, M, n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((K, K)) / np.sqrt(K)
+    B = rng.standard_normal((K, M))
+    C = rng.standard_normal((12, K))
+    nf = 40 + 7 * (np.arange(M) % 5)  # forcing ends at different steps per column
+    ftab = np.zeros((n, M))
+    for j in range(M):
+        ftab[: nf[j], j] = rng.standard_normal(nf[j])
+    hz = [n - (j % 3) * 7 for j in range(M)]  # some horizons off the multiples of m
+    return A, B, C, ftab, hz
+
+
+def run(A, B, C, ftab, hz, precision, macro, every, chunks):
+    K, M = B.shape
+    n = ftab.shape[0]
+    rom = shaw_amd.Rom(A, -A.T.copy(), 0.5, precision, C, max_cols=M, max_samples=n // every)
+    if macro:
+        rom.set_macro_steps(macro)
+    slots = rom.admit(B, ftab, hz, every)
+    for c in chunks:
+        rom.advance(c)
+    rom.sync()
+    steps = [rom.state(s)[2] for s in slots]
+    out = [rom.retire(s, n // every) for s in slots]
+    rom.close()
+    return [o[0] for o in out], np.array([np.concatenate([o[1], o[2]]) for o in out]), steps
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("precision", [shaw_amd.ROM_BF16X3, shaw_amd.ROM_BF16X4])
+@pytest.mark.parametrize("K", [128, 256, 384, 512])
+def test_macro_steps_vs_fp64(K, precision):
+    say(f"[gpu] ROM macro K={K} precision={precision}")
+    M, n, every = 48, 3000, 10
+    A, B, C, ftab, hz = problem(K, M, n, K + precision)
+    chunks = [1000, 1000, 1000]
+    ref_seis, ref_q, ref_steps = run(A, B, C, ftab, hz, shaw_amd.ROM_FP64, 0, every, chunks)
+    one_seis, one_q, one_steps = run(A, B, C, ftab, hz, precision, 0, every, chunks)
+    mac_seis, mac_q, mac_steps = run(A, B, C, ftab, hz, precision, 10, every, chunks)
+    assert mac_steps == one_steps == ref_steps == hz
+    err_one = max(rel(a, b) for a, b in zip(one_seis, ref_seis))
+    err_mac = max(rel(a, b) for a, b in zip(mac_seis, ref_seis))
+    q_one, q_mac = rel(one_q, ref_q), rel(mac_q, ref_q)
+    say(f"[gpu] K={K} precision={precision}: seismogram vs FP64 one-step {err_one:.2e} macro {err_mac:.2e}; "
+        f"states {q_one:.2e} / {q_mac:.2e}")
+    for a, b in zip(mac_seis, ref_seis):
+        assert a.shape == b.shape
+    bound = 2e-3 if precision == shaw_amd.ROM_BF16X3 else 5e-4
+    assert err_mac <= bound and q_mac <= bound
+    # the macro run changes the result (its tiles did take macro steps)
+    assert max(np.abs(a - b).max() for a, b in zip(mac_seis, one_seis)) > 0
+
+
+def test_macro_steps_skip_unaligned_tiles():
+    """Two admissions five steps apart in one 16-column tile: the columns sit
+    at different residues mod m, the tile takes no macro steps, and its
+    results equal the one-step run bit for bit; a second tile admitted
+    aligned takes them."""
+    say("[gpu] ROM macro unaligned tile")
+    K, n, every = 128, 2000, 10
+    A, B, C, ftab, hz = problem(K, 32, n, 3)
+    hz = [n] * 32
+
+    def go(macro):
+        rom = shaw_amd.Rom(A, -A.T.copy(), 0.5, shaw_amd.ROM_BF16X4, C, max_cols=32, max_samples=n // every)
+        if macro:
+            rom.set_macro_steps(macro)
+        s1 = rom.admit(B[:, :8], ftab[:, :8], hz[:8], every)    # slots 0-7 (tile 0)
+        rom.advance(5)
+        s2 = rom.admit(B[:, 8:], ftab[:, 8:], [n - 5] * 24, every)  # slots 8-31: tile 0 (8-15) and tile 1
+        rom.advance(n)
+        rom.sync()
+        res = [rom.retire(s, n // every) for s in s1 + s2]
+        rom.close()
+        return res
+
+    one, mac = go(0), go(10)
+    for j in range(16):  # tile 0: no macro steps, identical
+        for x, y in zip(one[j], mac[j]):
+            np.testing.assert_array_equal(x, y)
+    assert any(np.abs(one[j][0] - mac[j][0]).max() > 0 for j in range(16, 32))  # tile 1 took macro steps
+
+
+def test_macro_steps_only_bf16():
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((32, 32)) / 6.0
+    rom = shaw_amd.Rom(A, -A.T.copy(), 0.5, shaw_amd.ROM_FP64, None, max_cols=16)
+    with pytest.raises(shaw_amd.ShawError):
+        rom.set_macro_steps(10)
+    rom.set_macro_steps(0)
+    rom.close()
+    rom = shaw_amd.Rom(A, -A.T.copy(), 0.5, shaw_amd.ROM_BF16X3, None, max_cols=16)
+    with pytest.raises(shaw_amd.ShawError):
+        rom.set_macro_steps(1)
+    rom.close()
