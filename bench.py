@@ -118,13 +118,12 @@ def rom_section(app, w, fom_value, args) -> dict:
         C = rw.receiver_rows(app, basis, workloads.RECEIVERS)
         t3 = time.time()
         res, rates = {}, {}
-        for prec, name in ((shaw_amd.ROM_FP64, "fp64"), (shaw_amd.ROM_BF16X3, "bf16x3"),
-                           (shaw_amd.ROM_BF16X4, "bf16x4")):
-            res[name], rates[name] = run_rom(arv, ars, B, ftab, C, w.dt, n, prec)
+        for prec, name, macro in rom_variants(args):
+            res[name], rates[name] = run_rom(arv, ars, B, ftab, C, w.dt, n, prec, macro)
             log(f"config3 ROM ({tag}) K=256 {name}: {rates[name]:.3e} forcing-steps/s")
         err_fom = {k: [rw.rel_l2(res[k][j], fom[i]) for i, j in enumerate(spot)] for k in res}
         err_split = {k: summary([rw.rel_l2(res[k][j], res["fp64"][j]) for j in range(len(test))])
-                     for k in ("bf16x3", "bf16x4")}
+                     for k in res if k != "fp64"}
         rho = rw.amplification_radius(arv, ars, w.dt)
         out[tag] = {"projection": "reference (Euclidean Galerkin)" if not energy else
                     "energy-consistent (POD and projection in the FOM energy weights)",
@@ -141,13 +140,16 @@ def rom_section(app, w, fom_value, args) -> dict:
     return out
 
 
-def run_rom(arv, ars, B, ftab, C, dt, n, prec):
+def run_rom(arv, ars, B, ftab, C, dt, n, prec, macro=0):
     """Seismograms [M][T][R] and forcing-steps/s of one batched ROM run (HIP
-    events around the persistent kernel on the handle's stream)."""
+    events around the persistent kernels on the handle's stream); macro > 0:
+    BF16 tiles whose forcing has ended advance `macro` steps per product."""
     import shaw_amd
 
     m = B.shape[1]
     rom = shaw_amd.Rom(arv, ars, dt, prec, C, max_cols=m, max_samples=n // 10)
+    if macro:
+        rom.set_macro_steps(macro)
     slots = rom.admit(B, ftab, [n] * m, 10)
     ms = rom.time(n)
     seis = np.stack([rom.retire(s_, n // 10)[0] for s_ in slots])
@@ -155,15 +157,27 @@ def run_rom(arv, ars, B, ftab, C, dt, n, prec):
     return seis, m * n / (ms / 1e3)
 
 
+def rom_variants(args):
+    """(precision, name, macro steps) of the ROM runs: FP64, BF16x3 and
+    BF16x4 one step at a time, and the BF16 ones with macro steps."""
+    import shaw_amd
+
+    out = [(shaw_amd.ROM_FP64, "fp64", 0), (shaw_amd.ROM_BF16X3, "bf16x3", 0), (shaw_amd.ROM_BF16X4, "bf16x4", 0)]
+    if args.rom_macro:
+        out += [(shaw_amd.ROM_BF16X3, "bf16x3_macro", args.rom_macro),
+                (shaw_amd.ROM_BF16X4, "bf16x4_macro", args.rom_macro)]
+    return out
+
+
 def summary(v) -> dict:
     return {"max": float(np.max(v)), "median": float(np.median(v))}
 
 
 def mfma_tflops(k: int, rates: dict) -> dict:
-    """MFMA work rate: 4 K^2 flops per forcing-step, times the products per
-    multiply-add (3 for BF16x3, 4 for BF16x4)."""
+    """MFMA work rate of the one-step runs: 4 K^2 flops per forcing-step,
+    times the products per multiply-add (3 for BF16x3, 4 for BF16x4)."""
     mult = {"fp64": 1, "bf16x3": 3, "bf16x4": 4}
-    return {name: 4.0 * k * k * mult[name] * r / 1e12 for name, r in rates.items()}
+    return {name: 4.0 * k * k * mult[name] * r / 1e12 for name, r in rates.items() if name in mult}
 
 
 def rom_err_section(app, w, fom_value, args):
@@ -188,14 +202,14 @@ def rom_err_section(app, w, fom_value, args):
     C = rw.receiver_rows(app, basis, workloads.RECEIVERS, K)
     setup = time.time() - t0
     res, rates = {}, {}
-    for prec, name in ((shaw_amd.ROM_FP64, "fp64"), (shaw_amd.ROM_BF16X3, "bf16x3"), (shaw_amd.ROM_BF16X4, "bf16x4")):
-        res[name], rates[name] = run_rom(arv, ars, B, ftab, C, w.dt, n, prec)
+    for prec, name, macro in rom_variants(args):
+        res[name], rates[name] = run_rom(arv, ars, B, ftab, C, w.dt, n, prec, macro)
         log(f"energy ROM K={K} {name}: {rates[name]:.3e} forcing-steps/s")
     spot = list(range(0, len(test), len(test) // 16))[:16]
     fom = rw.fom_seismograms(app, [test[j] for j in spot], w.dt, n, workloads.RECEIVERS, 10)
     err_fom = {k: [rw.rel_l2(res[k][j], fom[i]) for i, j in enumerate(spot)] for k in res}
     err_split = {k: summary([rw.rel_l2(res[k][j], res["fp64"][j]) for j in range(len(test))])
-                 for k in ("bf16x3", "bf16x4")}
+                 for k in res if k != "fp64"}
     ok = {k: bool(max(v) <= 1e-3) for k, v in err_fom.items()}
     log(f"energy ROM errors vs FP64 FOM: { {k: summary(v) for k, v in err_fom.items()} }")
     out = {"workload": f"energy-consistent Galerkin ROM K={K}, one source depth ({depth:g} km), POD in the FOM "
@@ -240,6 +254,8 @@ def serving_section(app, w, basis, args) -> dict:
         ops[k] = rw.operators(app, basis, k)
         rom = shaw_amd.Rom(ops[k][0], ops[k][1], w.dt, shaw_amd.ROM_BF16X4, C512[:, :k], max_cols=4096,
                            max_samples=2000)
+        if args.rom_macro:
+            rom.set_macro_steps(args.rom_macro)
         engines[k] = serving.GpuEngine(rom, 4096)
     bcol = basis.mv[g] * rho_inv[g] * basis.dev.rows("vp", [g])[0]
     setup = time.time() - t0
@@ -298,7 +314,8 @@ def serving_section(app, w, basis, args) -> dict:
         e.rom.close()
     rep.update({"workload": f"config4 on the energy-consistent basis: Poisson 200 req/s for {args.serve_seconds:g} s, "
                             "M~U{1..256}, K~U{128,256,512}, horizon~U[5000,20000] steps, source depth 640 km, "
-                            "periods~U[20,70] s (forcing evaluated on the GPU at admission), BF16x4 engines, "
+                            "periods~U[20,70] s (forcing evaluated on the GPU at admission), BF16x4 engines"
+                            + (f" with macro steps of {args.rom_macro}" if args.rom_macro else "") + ", "
                             "180 receivers every 10 steps, C++ loop "
                             "(least work first, 2 s wait bound); then 10 s at 80 % of the measured drain rate",
                 "setup_s": setup})
@@ -454,6 +471,8 @@ def main() -> int:
     ap.add_argument("--rom-kernel-k", type=int, default=256)
     ap.add_argument("--rom-kernel-m", type=int, default=1024)
     ap.add_argument("--rom-kernel-prec", default="fp64,bf16x3,bf16x4")
+    ap.add_argument("--rom-macro", type=int, default=10,
+                    help="macro steps of the BF16 ROM runs and serving engines (0: one step at a time only)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

@@ -195,11 +195,6 @@ struct RomArgs {
   const void *gvv, *gvs, *gsv, *gss;
   const uint32_t* quiet;  // [slot] first step from which the column's forcing is negligible
   uint32_t macro;
-  // with macro steps a launch is three kernels per tile: one-step kernel
-  // (phase 1, up to where the tile may take macro steps), macro kernel,
-  // one-step kernel (phase 3, the rest); tileDone [tile] counts the steps done
-  uint32_t* tileDone;
-  uint32_t phase;
 };
 
 // A tile's macro plan for a budget of B steps: k one-steps first (until every
@@ -590,19 +585,7 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_steps_bf(RomArgs a) {
     const uint32_t o = (uint32_t)__shfl_xor((int)steps, off);
     steps = o > steps ? o : steps;
   }
-  // this launch's steps for the tile; with macro steps, phase 1 runs up to
-  // the tile's macro point and phase 3 what phases 1 and 2 left
-  uint64_t budget = a.nsteps;
-  if (a.macro) {
-    if (a.phase == 1) {
-      uint64_t k, k2;
-      macroPlan(a.macro, s0, hz, ev, a.quiet[slot0 + col], a.nsteps, k, k2);
-      budget = k;
-    } else {
-      budget = a.nsteps - a.tileDone[blockIdx.x];
-    }
-  }
-  if ((uint64_t)steps > budget) steps = (uint32_t)budget;
+  if ((uint64_t)steps > a.nsteps) steps = (uint32_t)a.nsteps;
   Item ring[D][TPP];
   Acc acc[TPP];
   if (steps) preloadRing<PARTS, TPP, D>(a.av, kGroupsS, rtVp[0], lane, ring);
@@ -682,42 +665,43 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_steps_bf(RomArgs a) {
     }
   if (tid < kTileCols) {
     const uint64_t h = a.horizon[slot0 + tid];
-    const uint64_t st = a.stepCol[slot0 + tid], s = st + budget;
+    const uint64_t st = a.stepCol[slot0 + tid], s = st + a.nsteps;
     a.stepCol[slot0 + tid] = s < h ? s : (st < h ? h : st);
   }
-  if (a.macro && a.phase == 1 && tid == 0) a.tileDone[blockIdx.x] = (uint32_t)budget;
 }
 
-// Macro steps (phase 2 of a launch with macro steps on): a tile whose active
-// columns are all at a multiple of m and quiet advances k2 macro steps of m
-// steps each, x += dt (G x) with x = [q_v; q_s] and G = (F^m - I) / dt, F the
-// leapfrog's one-step matrix (no forcing: every column is quiet). The
-// layout, splits, operator ring and receivers are k_rom_steps_bf's; G's
-// blocks G_vv, G_vs (rows of q_v) and G_sv, G_ss (rows of q_s) are four
-// products per pass, read from the splits of the old state, and the new
-// states stay in the registers until every wave is done with the old splits.
+// k_rom_steps_bf with macro steps (the engine's m = a.macro > 0): each tile
+// plans its launch (macroPlan) and runs k one-steps, k2 macro steps of m
+// steps each, then one-steps to the end, all in this launch, so tiles at
+// different stages do not wait for each other. A macro step is x += dt (G x)
+// with x = [q_v; q_s] and G = (F^m - I) / dt, F the leapfrog's one-step
+// matrix (no forcing: every active column is quiet); G's blocks G_vv, G_vs
+// (rows of q_v) and G_sv, G_ss (rows of q_s) are two products a pass, read
+// from the splits of the old state, and the new states stay in the registers
+// until every wave is done with the old splits.
 template <int PARTS, int TPW, int WAVES>
-__global__ void __launch_bounds__(WAVES * 64) k_rom_macro_bf(RomArgs a) {
+__global__ void __launch_bounds__(WAVES * 64) k_rom_steps_bfm(RomArgs a) {
+  // tiles per pass: at three or four tiles a wave the products run in passes
+  // of one or two tiles, whose rings fit the registers beside the states
+  // (K = 384 and 512, M = 4096: BF16x4 1.77e8 -> 2.06e8 and 7.6e7 -> 1.19e8,
+  // BF16x3 3.02e8 -> 3.11e8 and 1.55e8 -> 1.72e8 forcing-steps/s)
   constexpr int TPP = TPW == 4 ? 2 : (TPW == 3 ? 1 : TPW), NP = TPW / TPP;
-  // (a 2-deep ring at three tiles a wave: its four blocks a pass otherwise spill)
-  constexpr int D = TPW == 3 ? 2 : ringDepth<PARTS, TPP, WAVES, true>();
+  // (BF16x4 at three tiles a wave: a 2-deep ring keeps it within 256 VGPRs
+  // without spills; K = 384, M = 4096: 2.10e8 -> 2.30e8 forcing-steps/s)
+  constexpr int D = (PARTS == 3 && TPW == 3) ? 2 : ringDepth<PARTS, TPP, WAVES, true>();
   using Item = typename Prec<PARTS>::Item;
   using Acc = typename Prec<PARTS>::Acc;
   extern __shared__ double sm[];
+  // splits: hi items, then lo items, 2 K items of 16 bytes each per state
   int4v* Sv = reinterpret_cast<int4v*>(sm);
   int4v* Ss = Sv + 4 * (size_t)a.Kvp;
   const int loV = 2 * (int)a.Kvp, loS = 2 * (int)a.Ksp;
   const int slot0 = blockIdx.x * kTileCols;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64, col = lane % 16;
-  const uint32_t m = a.macro;
-  const uint32_t s0 = (uint32_t)a.stepCol[slot0 + col];
-  const uint32_t hz = (uint32_t)a.horizon[slot0 + col];
-  const uint32_t ev = (uint32_t)a.every[slot0 + col];
-  uint64_t k, n;
-  macroPlan(m, s0, hz, ev, a.quiet[slot0 + col], a.nsteps - a.tileDone[blockIdx.x], k, n);
-  if (k != 0 || n == 0) return;  // block-uniform: phase 1 stopped at the macro point, or no macro steps
   const int kvTiles = a.Kvp / 16, ksTiles = a.Ksp / 16;
   const int kGroupsS = a.Ksp / 32, kGroupsV = a.Kvp / 32;
+  // this wave's row tiles (the host launches WAVES * TPW >= both tile counts);
+  // a spare slot repeats tile `wave` and neither stores nor writes a split
   int rtV[TPW], rtS[TPW];
   bool okV[TPW], okS[TPW];
 #pragma unroll
@@ -727,7 +711,7 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_macro_bf(RomArgs a) {
     rtV[t] = okV[t] ? wave + t * WAVES : (wave < kvTiles ? wave : 0);
     rtS[t] = okS[t] ? wave + t * WAVES : (wave < ksTiles ? wave : 0);
   }
-  int rtVp[NP][TPP], rtSp[NP][TPP];
+  int rtVp[NP][TPP], rtSp[NP][TPP];  // the tiles of each pass
 #pragma unroll
   for (int p = 0; p < NP; ++p)
 #pragma unroll
@@ -741,58 +725,153 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_macro_bf(RomArgs a) {
   for (int t = 0; t < TPW; ++t)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      qv[t][r] = okV[t] ? a.qv[colV + rtV[t] * 16 + rowBf16(lane, r)] : 0.0;
-      qs[t][r] = okS[t] ? a.qs[colS + rtS[t] * 16 + rowBf16(lane, r)] : 0.0;
+      const int rowV = rtV[t] * 16 + rowBf16(lane, r), rowS = rtS[t] * 16 + rowBf16(lane, r);
+      qv[t][r] = okV[t] ? a.qv[colV + rowV] : 0.0;
+      qs[t][r] = okS[t] ? a.qs[colS + rowS] : 0.0;
     }
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
     if (okV[t]) writeSplitRows(Sv, loV, rtV[t], lane, qv[t]);
     if (okS[t]) writeSplitRows(Ss, loS, rtS[t], lane, qs[t]);
   }
-  const bool act = s0 < hz;
+  const uint32_t s0 = (uint32_t)a.stepCol[slot0 + col];
+  const uint32_t hz = (uint32_t)a.horizon[slot0 + col];
+  const uint32_t ev = (uint32_t)a.every[slot0 + col];
+  const double* fcol = a.ftab + (uint64_t)(slot0 + col) * a.maxSteps;
+  uint32_t steps = hz > s0 ? hz - s0 : 0;
+#pragma unroll
+  for (int off = 32; off >= 1; off /= 2) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)steps, off);
+    steps = o > steps ? o : steps;
+  }
+  if ((uint64_t)steps > a.nsteps) steps = (uint32_t)a.nsteps;
+  uint64_t mk = 0, mk2 = 0;  // macro steps: mk one-steps, mk2 macro steps, then one-steps to the end
+  if (a.macro) macroPlan(a.macro, s0, hz, ev, a.quiet[slot0 + col], a.nsteps, mk, mk2);
+  const uint32_t mSteps = a.macro;
   Item ring[D][TPP];
   Acc acc[TPP];
-  preloadRing<PARTS, TPP, D>(a.gvv, kGroupsV, rtVp[0], lane, ring);
-  __syncthreads();  // both splits are in LDS
-  for (uint64_t it = 0; it < n; ++it) {
+  for (int seg = 0; seg < 2; ++seg) {
+    const uint32_t from = seg == 0 ? 0u : (uint32_t)(mk + (uint64_t)mSteps * mk2);
+    const uint32_t to = seg == 0 ? (mk2 ? (uint32_t)mk : steps) : (mk2 ? steps : 0u);
+    if (from < to) {
+      preloadRing<PARTS, TPP, D>(a.av, kGroupsS, rtVp[0], lane, ring);
+      __syncthreads();  // both splits are in LDS
+      const uint32_t sf = s0 + from;
+      double fcur = sf < hz ? fcol[sf] : 0.0;
+      uint32_t phase = ev ? (sf + 1) % ev : 0;
+      uint32_t nextIdx = ev ? (sf + 1) / ev : 0;
+      for (uint32_t it = from; it < to; ++it) {
+        const uint32_t s = s0 + it;
+        const bool act = s < hz;
+        const double fnext = act && s + 1 < hz ? fcol[s + 1] : 0.0;
+        const double fdt = act ? fcur * a.dt : 0.0;
+        const bool smp = act && ev > 0 && phase == 0;
+        const bool anySample = (__ballot(smp) & 0xffffull) != 0;
+        // velocity: q_v += dt Ar_v q_s ; q_v += (f dt) b
 #pragma unroll
-    for (int p = 0; p < NP; ++p) {
-      consumeRing<PARTS, TPP, D>(a.gvv, kGroupsV, rtVp[p], Sv, loV, lane, ring, acc);
-      preloadRing<PARTS, TPP, D>(a.gvs, kGroupsS, rtVp[p], lane, ring);
-      consumeRing<PARTS, TPP, D, false>(a.gvs, kGroupsS, rtVp[p], Ss, loS, lane, ring, acc);
-      if (p + 1 < NP) preloadRing<PARTS, TPP, D>(a.gvv, kGroupsV, rtVp[p + 1 < NP ? p + 1 : 0], lane, ring);
-      else preloadRing<PARTS, TPP, D>(a.gsv, kGroupsV, rtSp[0], lane, ring);
+        for (int p = 0; p < NP; ++p) {
+          consumeRing<PARTS, TPP, D>(a.av, kGroupsS, rtVp[p], Ss, loS, lane, ring, acc);
+          double bv[TPP][4];  // b stays in L1/L2; loaded here to keep it out of the products' registers
 #pragma unroll
-      for (int j = 0; j < TPP; ++j)
-        if (act && okV[p * TPP + j])
+          for (int j = 0; j < TPP; ++j)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) qv[p * TPP + j][r] = qv[p * TPP + j][r] + a.dt * (double)acc[j][r];
+            for (int r = 0; r < 4; ++r) bv[j][r] = a.b[colV + rtVp[p][j] * 16 + rowBf16(lane, r)];
+          if (p + 1 < NP) preloadRing<PARTS, TPP, D>(a.av, kGroupsS, rtVp[p + 1 < NP ? p + 1 : 0], lane, ring);
+          else preloadRing<PARTS, TPP, D>(a.as, kGroupsV, rtSp[0], lane, ring);
+#pragma unroll
+          for (int j = 0; j < TPP; ++j) {
+            const int t = p * TPP + j;
+            if (okV[t]) {
+              if (act)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                  double y = qv[t][r] + a.dt * (double)acc[j][r];
+                  qv[t][r] = y + fdt * bv[j][r];
+                }
+              writeSplitRows(Sv, loV, rtV[t], lane, qv[t]);
+            }
+          }
+        }
+        __syncthreads();
+        // stress: q_s += dt Ar_s q_v
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          consumeRing<PARTS, TPP, D>(a.as, kGroupsV, rtSp[p], Sv, loV, lane, ring, acc);
+          if (p + 1 < NP) {
+            preloadRing<PARTS, TPP, D>(a.as, kGroupsV, rtSp[p + 1 < NP ? p + 1 : 0], lane, ring);
+          } else {
+            // one receiver tile at a time with a short ring: sample steps are
+            // rare, and the states stay in registers across them
+            if (anySample) receivers<PARTS, 1, WAVES, 2>(a, Sv, loV, wave, lane, smp, nextIdx - 1, slot0);
+            if (it + 1 < to) preloadRing<PARTS, TPP, D>(a.av, kGroupsS, rtVp[0], lane, ring);
+          }
+#pragma unroll
+          for (int j = 0; j < TPP; ++j) {
+            const int t = p * TPP + j;
+            if (okS[t]) {
+              if (act)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) qs[t][r] = qs[t][r] + a.dt * (double)acc[j][r];
+              writeSplitRows(Ss, loS, rtS[t], lane, qs[t]);
+            }
+          }
+        }
+        __syncthreads();
+        fcur = fnext;
+        if (ev && act) {
+          phase = phase + 1 == ev ? 0 : phase + 1;
+          if (phase == 0) ++nextIdx;
+        }
+        }
     }
+    if (seg == 0 && mk2) {
+      constexpr int DM = TPW == 3 ? 2 : D;
+      const bool act = s0 + (uint32_t)mk < hz;
+      Item mring[DM][TPP];
+      preloadRing<PARTS, TPP, DM>(a.gvv, kGroupsV, rtVp[0], lane, mring);
+      __syncthreads();  // both splits are in LDS
+        for (uint64_t it = 0; it < mk2; ++it) {
 #pragma unroll
-    for (int p = 0; p < NP; ++p) {
-      consumeRing<PARTS, TPP, D>(a.gsv, kGroupsV, rtSp[p], Sv, loV, lane, ring, acc);
-      preloadRing<PARTS, TPP, D>(a.gss, kGroupsS, rtSp[p], lane, ring);
-      consumeRing<PARTS, TPP, D, false>(a.gss, kGroupsS, rtSp[p], Ss, loS, lane, ring, acc);
-      if (p + 1 < NP) preloadRing<PARTS, TPP, D>(a.gsv, kGroupsV, rtSp[p + 1 < NP ? p + 1 : 0], lane, ring);
-      else if (it + 1 < n) preloadRing<PARTS, TPP, D>(a.gvv, kGroupsV, rtVp[0], lane, ring);
+          for (int p = 0; p < NP; ++p) {
+            consumeRing<PARTS, TPP, DM>(a.gvv, kGroupsV, rtVp[p], Sv, loV, lane, mring, acc);
+            preloadRing<PARTS, TPP, DM>(a.gvs, kGroupsS, rtVp[p], lane, mring);
+            consumeRing<PARTS, TPP, DM, false>(a.gvs, kGroupsS, rtVp[p], Ss, loS, lane, mring, acc);
+            if (p + 1 < NP) preloadRing<PARTS, TPP, DM>(a.gvv, kGroupsV, rtVp[p + 1 < NP ? p + 1 : 0], lane, mring);
+            else preloadRing<PARTS, TPP, DM>(a.gsv, kGroupsV, rtSp[0], lane, mring);
 #pragma unroll
-      for (int j = 0; j < TPP; ++j)
-        if (act && okS[p * TPP + j])
+            for (int j = 0; j < TPP; ++j)
+              if (act && okV[p * TPP + j])
 #pragma unroll
-          for (int r = 0; r < 4; ++r) qs[p * TPP + j][r] = qs[p * TPP + j][r] + a.dt * (double)acc[j][r];
+                for (int r = 0; r < 4; ++r) qv[p * TPP + j][r] = qv[p * TPP + j][r] + a.dt * (double)acc[j][r];
+          }
+#pragma unroll
+          for (int p = 0; p < NP; ++p) {
+            consumeRing<PARTS, TPP, DM>(a.gsv, kGroupsV, rtSp[p], Sv, loV, lane, mring, acc);
+            preloadRing<PARTS, TPP, DM>(a.gss, kGroupsS, rtSp[p], lane, mring);
+            consumeRing<PARTS, TPP, DM, false>(a.gss, kGroupsS, rtSp[p], Ss, loS, lane, mring, acc);
+            if (p + 1 < NP) preloadRing<PARTS, TPP, DM>(a.gsv, kGroupsV, rtSp[p + 1 < NP ? p + 1 : 0], lane, mring);
+            else if (it + 1 < mk2) preloadRing<PARTS, TPP, DM>(a.gvv, kGroupsV, rtVp[0], lane, mring);
+#pragma unroll
+            for (int j = 0; j < TPP; ++j)
+              if (act && okS[p * TPP + j])
+#pragma unroll
+                for (int r = 0; r < 4; ++r) qs[p * TPP + j][r] = qs[p * TPP + j][r] + a.dt * (double)acc[j][r];
+          }
+          __syncthreads();  // every read of the old splits done
+#pragma unroll
+          for (int t = 0; t < TPW; ++t) {
+            if (okV[t]) writeSplitRows(Sv, loV, rtV[t], lane, qv[t]);
+            if (okS[t]) writeSplitRows(Ss, loS, rtS[t], lane, qs[t]);
+          }
+          __syncthreads();  // the new splits complete
+          // the macro step's last step is sampled when it ends at a multiple of the period
+          const uint64_t sn = (uint64_t)s0 + mk + (it + 1) * mSteps;
+          const bool smp = act && ev > 0 && sn % ev == 0;
+          if ((__ballot(smp) & 0xffffull) != 0)
+            receivers<PARTS, 1, WAVES, 2>(a, Sv, loV, wave, lane, smp, smp ? (uint32_t)(sn / ev - 1) : 0u, slot0);
+        }
+      __syncthreads();  // the next one-steps write the split the receivers read
     }
-    __syncthreads();  // every read of the old splits done
-#pragma unroll
-    for (int t = 0; t < TPW; ++t) {
-      if (okV[t]) writeSplitRows(Sv, loV, rtV[t], lane, qv[t]);
-      if (okS[t]) writeSplitRows(Ss, loS, rtS[t], lane, qs[t]);
-    }
-    __syncthreads();  // the new splits complete
-    // the macro step's last step is sampled when it ends at a multiple of the period
-    const uint64_t sn = (uint64_t)s0 + (it + 1) * m;
-    const bool smp = act && ev > 0 && sn % ev == 0;
-    if ((__ballot(smp) & 0xffffull) != 0)
-      receivers<PARTS, 1, WAVES, 2>(a, Sv, loV, wave, lane, smp, smp ? (uint32_t)(sn / ev - 1) : 0u, slot0);
   }
 #pragma unroll
   for (int t = 0; t < TPW; ++t)
@@ -801,13 +880,11 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_macro_bf(RomArgs a) {
       if (okV[t]) a.qv[colV + rtV[t] * 16 + rowBf16(lane, r)] = qv[t][r];
       if (okS[t]) a.qs[colS + rtS[t] * 16 + rowBf16(lane, r)] = qs[t][r];
     }
-  __syncthreads();  // every wave has read the step counters
   if (tid < kTileCols) {
     const uint64_t h = a.horizon[slot0 + tid];
-    const uint64_t st = a.stepCol[slot0 + tid], sn = st + n * m;
-    a.stepCol[slot0 + tid] = st < h ? (sn < h ? sn : h) : st;
+    const uint64_t st = a.stepCol[slot0 + tid], s = st + a.nsteps;
+    a.stepCol[slot0 + tid] = s < h ? s : (st < h ? h : st);
   }
-  if (tid == 0) a.tileDone[blockIdx.x] += (uint32_t)(n * m);
 }
 
 // admission: column j of the staged inputs goes to slot slots[j]; its states
@@ -920,24 +997,24 @@ std::vector<double> packBf16(const double* A, uint64_t rows, uint64_t cols, uint
 }
 
 template <int PARTS, int TPW, int WAVES>
-void launchRomKernel(dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a, bool macroKernel) {
+void launchRomKernel(dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a) {
   auto go = [&](auto kernel) {
     (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(kernel, grid, dim3(WAVES * 64), lds, stream, a);
   };
   if constexpr (PARTS == 0) go(k_rom_steps<0, TPW, WAVES>);
-  else if (macroKernel) go(k_rom_macro_bf<PARTS, TPW, WAVES>);
+  else if (a.macro) go(k_rom_steps_bfm<PARTS, TPW, WAVES>);
   else go(k_rom_steps_bf<PARTS, TPW, WAVES>);
 }
 
 template <int PARTS>
-void launchRomSteps(int tpw, dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a, bool macroKernel = false) {
+void launchRomSteps(int tpw, dim3 grid, size_t lds, hipStream_t stream, const RomArgs& a) {
   switch (tpw) {
-    case 0: launchRomKernel<PARTS, 1, 16>(grid, lds, stream, a, macroKernel); break;  // 9..16 row tiles: one per wave of 16
-    case 1: launchRomKernel<PARTS, 1, kWaves>(grid, lds, stream, a, macroKernel); break;
-    case 2: launchRomKernel<PARTS, 2, kWaves>(grid, lds, stream, a, macroKernel); break;
-    case 3: launchRomKernel<PARTS, 3, kWaves>(grid, lds, stream, a, macroKernel); break;
-    default: launchRomKernel<PARTS, 4, kWaves>(grid, lds, stream, a, macroKernel); break;
+    case 0: launchRomKernel<PARTS, 1, 16>(grid, lds, stream, a); break;  // 9..16 row tiles: one per wave of 16
+    case 1: launchRomKernel<PARTS, 1, kWaves>(grid, lds, stream, a); break;
+    case 2: launchRomKernel<PARTS, 2, kWaves>(grid, lds, stream, a); break;
+    case 3: launchRomKernel<PARTS, 3, kWaves>(grid, lds, stream, a); break;
+    default: launchRomKernel<PARTS, 4, kWaves>(grid, lds, stream, a); break;
   }
 }
 
@@ -1099,7 +1176,7 @@ struct shaw_rom {
   // quiet step
   std::vector<double> hArV, hArS;
   shaw::DevBuf<double> gvv, gvs, gsv, gss;
-  shaw::DevBuf<uint32_t> quiet, tileDone;
+  shaw::DevBuf<uint32_t> quiet;
   uint32_t macro = 0, macroBuilt = 0;
   shaw::DevBuf<uint64_t> stepCol, horizon, every;
   std::vector<uint64_t> hStep, hHorizon, hEvery;
@@ -1149,22 +1226,12 @@ struct shaw_rom {
     a.gvv = gvv.p; a.gvs = gvs.p; a.gsv = gsv.p; a.gss = gss.p;
     a.quiet = quiet.p;
     a.macro = precision == SHAW_ROM_FP64 ? 0 : macro;
-    a.tileDone = tileDone.p;
-    a.phase = 1;
     const dim3 grid((unsigned)(maxCols / kTileCols));
     const size_t lds = ldsBytes();
-    auto run = [&](bool macroKernel) {
-      if (precision == SHAW_ROM_FP64) launchRomSteps<0>(tpw, grid, lds, stream, a);
-      else if (precision == SHAW_ROM_BF16X3) launchRomSteps<2>(tpw, grid, lds, stream, a, macroKernel);
-      else launchRomSteps<3>(tpw, grid, lds, stream, a, macroKernel);
-      SHAW_HIP(hipGetLastError());
-    };
-    run(false);
-    if (a.macro) {  // the tiles' macro steps, then the steps they leave
-      run(true);
-      a.phase = 3;
-      run(false);
-    }
+    if (precision == SHAW_ROM_FP64) launchRomSteps<0>(tpw, grid, lds, stream, a);
+    else if (precision == SHAW_ROM_BF16X3) launchRomSteps<2>(tpw, grid, lds, stream, a);
+    else launchRomSteps<3>(tpw, grid, lds, stream, a);
+    SHAW_HIP(hipGetLastError());
     for (uint64_t c = 0; c < maxCols; ++c) {
       const uint64_t s = hStep[c] + nsteps;
       hStep[c] = s < hHorizon[c] ? s : std::max(hStep[c], hHorizon[c]);
@@ -1412,7 +1479,6 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
     h->stepCol.alloc(C); h->horizon.alloc(C); h->every.alloc(C);
     h->quiet.alloc(C);
     SHAW_HIP(hipMemset(h->quiet.p, 0xff, C * sizeof(uint32_t)));  // never quiet until admitted
-    h->tileDone.alloc(C / kTileCols);
     h->hStep.assign(C, 0); h->hHorizon.assign(C, 0); h->hEvery.assign(C, 0); h->used.assign(C, 0);
     h->stepCol.upload(h->hStep.data(), C); h->horizon.upload(h->hHorizon.data(), C); h->every.upload(h->hEvery.data(), C);
     h->seis.alloc(C * std::max<uint64_t>(h->R, 1) * std::max<uint64_t>(h->maxSamples, 1));

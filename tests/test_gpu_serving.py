@@ -74,3 +74,49 @@ def test_gpu_server_alone_equals_mix(prec):
             eng.rom.close()
     for eng in e.values():
         eng.rom.close()
+
+
+def test_gpu_server_macro_steps_vs_fp64():
+    """The server on BF16x4 engines with macro steps (forcing that ends after
+    30 steps, sampling every 10 steps, chunks of 20 so that admissions stay
+    aligned): every request's seismograms stay within the BF16 bound of the
+    FP64 engines' (which are bit-identical alone and in a mix), and the macro
+    steps did run (the results differ from one-step BF16x4)."""
+    say("[gpu] server with macro steps")
+
+    def engines(prec, macro):
+        e = {32: serving.GpuEngine(skew_rom(32, 10, prec), 64), 80: serving.GpuEngine(skew_rom(80, 11, prec), 64)}
+        for eng in e.values():
+            if macro:
+                eng.rom.set_macro_steps(macro)
+        return e
+
+    specs = [(0, 32, 5, 120), (1, 80, 17, 60), (2, 32, 40, 200), (3, 80, 3, 95), (4, 32, 20, 40), (5, 80, 33, 180),
+             (6, 32, 30, 75), (7, 80, 30, 400)]
+
+    def requests():
+        out = []
+        for rid, k, m, h in specs:
+            r = request(rid, k, m, h, 200 + rid, every=10)
+            r.ftab[30:] = 0.0
+            out.append(r)
+        return out
+
+    res = {}
+    for name, prec, macro in (("fp64", shaw_amd.ROM_FP64, 0), ("one", shaw_amd.ROM_BF16X4, 0),
+                              ("macro", shaw_amd.ROM_BF16X4, 10)):
+        e = engines(prec, macro)
+        reqs = requests()
+        rep = serving.Server(e, hbm_budget=1e12, chunk=20).run(reqs)
+        assert rep["requests"] == len(specs)
+        res[name] = [r.seismograms for r in reqs]
+        for eng in e.values():
+            eng.rom.close()
+    worst = 0.0
+    for ref, mac in zip(res["fp64"], res["macro"]):
+        for x, y in zip(mac, ref):
+            assert x.shape == y.shape
+            worst = max(worst, float(np.linalg.norm(x - y) / np.linalg.norm(y)))
+    say(f"[gpu] macro-step serving vs FP64: worst column {worst:.2e}")
+    assert worst <= 2e-3
+    assert any(np.abs(x - y).max() > 0 for a, b in zip(res["one"], res["macro"]) for x, y in zip(a, b))
