@@ -1497,9 +1497,23 @@ void admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const dou
     SHAW_REQUIRE(h && b && (ftab || (kinds && delays && periods)) && horizons && slots, SHAW_ERR_ARG,
                  "null argument");
     SHAW_REQUIRE(h->R == 0 || every > 0, SHAW_ERR_ARG, "sampling period must be > 0 when receivers are set");
+    // the columns go to wholly free tiles first, then to the free slots of
+    // the other tiles: a tile shared with columns at other stages takes
+    // macro steps only when all of them allow it
     std::vector<uint64_t> free;
-    for (uint64_t c = 0; c < h->maxCols && free.size() < n; ++c)
-      if (!h->used[c]) free.push_back(c);
+    std::vector<uint8_t> taken(h->used);
+    const uint64_t tiles = h->maxCols / kTileCols;
+    for (int pass = 0; pass < 2 && free.size() < n; ++pass)
+      for (uint64_t t = 0; t < tiles && free.size() < n; ++t) {
+        uint64_t nfree = 0;
+        for (uint64_t c = t * kTileCols; c < (t + 1) * kTileCols; ++c) nfree += taken[c] ? 0 : 1;
+        if (nfree == 0 || (pass == 0 && nfree < (uint64_t)kTileCols)) continue;
+        for (uint64_t c = t * kTileCols; c < (t + 1) * kTileCols && free.size() < n; ++c)
+          if (!taken[c]) {
+            taken[c] = 1;
+            free.push_back(c);
+          }
+      }
     SHAW_REQUIRE(free.size() == n, SHAW_ERR_MEMORY, "not enough free ROM column slots");
     for (uint64_t j = 0; j < n; ++j) {
       SHAW_REQUIRE(horizons[j] <= nsteps, SHAW_ERR_ARG, "horizon longer than the forcing table");

@@ -92,20 +92,26 @@ def test_macro_steps_skip_unaligned_tiles():
         rom = shaw_amd.Rom(A, -A.T.copy(), 0.5, shaw_amd.ROM_BF16X4, C, max_cols=32, max_samples=n // every)
         if macro:
             rom.set_macro_steps(macro)
-        s1 = rom.admit(B[:, :8], ftab[:, :8], hz[:8], every)    # slots 0-7 (tile 0)
+        s1 = rom.admit(B[:, :8], ftab[:, :8], hz[:8], every)     # 8 slots of a free tile
         rom.advance(5)
-        s2 = rom.admit(B[:, 8:], ftab[:, 8:], [n - 5] * 24, every)  # slots 8-31: tile 0 (8-15) and tile 1
+        s2 = rom.admit(B[:, 8:], ftab[:, 8:], [n - 5] * 24, every)  # the other tile, then the 8 left beside s1
         rom.advance(n)
         rom.sync()
-        res = [rom.retire(s, n // every) for s in s1 + s2]
+        slots = s1 + s2
+        res = [rom.retire(s, n // every) for s in slots]
         rom.close()
-        return res
+        return res, slots
 
-    one, mac = go(0), go(10)
-    for j in range(16):  # tile 0: no macro steps, identical
-        for x, y in zip(one[j], mac[j]):
-            np.testing.assert_array_equal(x, y)
-    assert any(np.abs(one[j][0] - mac[j][0]).max() > 0 for j in range(16, 32))  # tile 1 took macro steps
+    (one, slots), (mac, slots2) = go(0), go(10)
+    assert slots == slots2
+    shared = slots[0] // 16  # the tile of s1, shared with 8 columns of s2
+    assert sum(1 for s in slots if s // 16 == shared) == 16
+    for j, s in enumerate(slots):
+        if s // 16 == shared:  # no macro steps there: identical
+            for x, y in zip(one[j], mac[j]):
+                np.testing.assert_array_equal(x, y)
+    # the tile of s2 alone took macro steps
+    assert any(np.abs(one[j][0] - mac[j][0]).max() > 0 for j, s in enumerate(slots) if s // 16 != shared)
 
 
 def test_macro_steps_only_bf16():
