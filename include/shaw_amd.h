@@ -378,16 +378,17 @@ int shaw_rom_admit_signals(shaw_rom_t h, uint64_t n, const double* b, const int*
                            const double* periods, uint64_t nsteps, const uint64_t* horizons, uint64_t every,
                            uint64_t* slots_out);
 /* Macro steps (BF16 engines; no counterpart in the reference): with m in
- * 2..64 a tile of 16 column slots whose columns are all at a multiple of m,
- * sampled at multiples of m (or not at all), at least m steps from their
- * horizons and quiet (their forcing at or below 2^-64 of its peak from there
- * on, found at admission) advances m steps with one product by
- * G = (F^m - I) / dt, F the leapfrog's one-step matrix of [q_v; q_s] (built
- * on the host from the engine's FP64 operators by stepping the identity m
- * times); other tiles and steps run the one-step kernel. The results differ
- * from one-step BF16 runs at the BF16 rounding level, and a column's steps
- * depend on the other columns of its tile. m = 0 (the default) turns them
- * off.                                                                    */
+ * 2..64 a tile of 16 column slots whose active columns are at steps
+ * congruent mod m and sampled at multiples of m (or not at all) advances,
+ * from the first multiple of m at which all of them are quiet (their forcing
+ * at or below 2^-64 of its peak from there on, found at admission), m steps
+ * per product by G = (F^m - I) / dt, F the leapfrog's one-step matrix of
+ * [q_v; q_s] (built on the host from the engine's FP64 operators by
+ * stepping the identity m times), while every column stays at least m steps
+ * from its horizon; all other steps are one-steps, in the same launch. The
+ * results differ from one-step BF16 runs at the BF16 rounding level, and a
+ * column's steps depend on the other columns of its tile. m = 0 (the
+ * default) turns them off.                                                */
 int shaw_rom_set_macro_steps(shaw_rom_t h, uint32_t m);
 /* Advance every active column by up to nsteps (columns stop at their horizon). */
 int shaw_rom_advance(shaw_rom_t h, uint64_t nsteps);
