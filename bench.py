@@ -420,7 +420,10 @@ def rom_kernel_only(args) -> int:
     """The batched reduced leapfrog alone (k_rom_steps, k_rom_steps_bf), for profiling: K x K
     skew operator pairs (random, seeded), M columns, 180 receivers every 10
     steps, --rom-steps steps timed per precision with HIP events on the
-    engine's stream. Prints one JSON line of forcing-steps/s and MFMA TFLOP/s."""
+    engine's stream. Prints one JSON line of forcing-steps/s and MFMA TFLOP/s.
+    With --rom-kernel-quiet-after Q > 0 each column's forcing ends at a seeded
+    step in [Q/2, Q] and the BF16 precisions run again with macro steps
+    (--rom-macro, k_rom_steps_bfm)."""
     import shaw_amd
 
     K, M, n = args.rom_kernel_k, args.rom_kernel_m, args.rom_steps
@@ -429,18 +432,32 @@ def rom_kernel_only(args) -> int:
     C = rng.standard_normal((180, K))
     B = rng.standard_normal((K, M))
     f = rng.standard_normal((n + 50, M))
-    out = {"workload": f"batched ROM kernel only: K={K}, M={M}, {n} steps, synthetic skew operators"}
+    q = args.rom_kernel_quiet_after
+    if q > 0:
+        ends = rng.integers(q // 2, q + 1, M)
+        for j in range(M):
+            f[50 + ends[j]:, j] = 0.0
+    out = {"workload": f"batched ROM kernel only: K={K}, M={M}, {n} steps, synthetic skew operators"
+                       + (f", forcing ending at steps in [{q // 2}, {q}]" if q > 0 else "")}
     names = {"fp64": shaw_amd.ROM_FP64, "bf16x3": shaw_amd.ROM_BF16X3, "bf16x4": shaw_amd.ROM_BF16X4}
-    for name in args.rom_kernel_prec.split(","):
+    runs = [(name, 0) for name in args.rom_kernel_prec.split(",")]
+    if q > 0 and args.rom_macro:
+        runs += [(name, args.rom_macro) for name in args.rom_kernel_prec.split(",") if name != "fp64"]
+    for name, macro in runs:
         rom = shaw_amd.Rom(A, -A.T.copy(), 0.05, names[name], C, max_cols=M, max_samples=(n + 50) // 10)
+        if macro:
+            rom.set_macro_steps(macro)
         rom.admit(B, f, [n + 50] * M, 10)
         rom.advance(50)
         rom.sync()
         ms = rom.time(n)
         rom.close()
         rate = M * n / (ms / 1e3)
-        out[name] = {"ms": ms, "forcing_steps_per_s": rate, "mfma_tflops": mfma_tflops(K, {name: rate})[name]}
-        log(f"ROM kernel {name}: {rate:.3e} forcing-steps/s")
+        key = name + (f"_macro{macro}" if macro else "")
+        out[key] = {"ms": ms, "forcing_steps_per_s": rate}
+        if not macro:
+            out[key]["mfma_tflops"] = mfma_tflops(K, {name: rate})[name]
+        log(f"ROM kernel {key}: {rate:.3e} forcing-steps/s")
     print(json.dumps(out), flush=True)
     return 0
 
@@ -471,6 +488,8 @@ def main() -> int:
     ap.add_argument("--rom-kernel-k", type=int, default=256)
     ap.add_argument("--rom-kernel-m", type=int, default=1024)
     ap.add_argument("--rom-kernel-prec", default="fp64,bf16x3,bf16x4")
+    ap.add_argument("--rom-kernel-quiet-after", type=int, default=0,
+                    help="--rom-kernel: forcing ends at seeded steps up to this (0: forcing throughout)")
     ap.add_argument("--rom-macro", type=int, default=10,
                     help="macro steps of the BF16 ROM runs and serving engines (0: one step at a time only)")
     args = ap.parse_args()
