@@ -252,11 +252,14 @@ def serving_section(app, w, basis, args) -> dict:
     ops, engines = {}, {}
     for k in (128, 256, 512):
         ops[k] = rw.operators(app, basis, k)
-        rom = shaw_amd.Rom(ops[k][0], ops[k][1], w.dt, shaw_amd.ROM_BF16X4, C512[:, :k], max_cols=4096,
-                           max_samples=2000)
-        if args.rom_macro:
-            rom.set_macro_steps(args.rom_macro)
-        engines[k] = serving.GpuEngine(rom, 4096)
+        group = []
+        for _ in range(args.serve_engines_512 if k == 512 else 1):
+            rom = shaw_amd.Rom(ops[k][0], ops[k][1], w.dt, shaw_amd.ROM_BF16X4, C512[:, :k], max_cols=4096,
+                               max_samples=2000)
+            if args.rom_macro:
+                rom.set_macro_steps(args.rom_macro)
+            group.append(serving.GpuEngine(rom, 4096))
+        engines[k] = group if len(group) > 1 else group[0]
     bcol = basis.mv[g] * rho_inv[g] * basis.dev.rows("vp", [g])[0]
     setup = time.time() - t0
     mean_work = 128.5 * 12500.0  # E[M] E[horizon] forcing-steps per request
@@ -311,7 +314,8 @@ def serving_section(app, w, basis, args) -> dict:
                                                 "p50_latency_s", "p99_latency_s", "max_latency_s", "max_wait_s",
                                                 "throughput_forcing_steps_per_s", "wall_s", "launches", "max_queue")}
     for e in engines.values():
-        e.rom.close()
+        for eng in (e if isinstance(e, list) else [e]):
+            eng.rom.close()
     rep.update({"workload": f"config4 on the energy-consistent basis: Poisson 200 req/s for {args.serve_seconds:g} s, "
                             "M~U{1..256}, K~U{128,256,512}, horizon~U[5000,20000] steps, source depth 640 km, "
                             "periods~U[20,70] s (forcing evaluated on the GPU at admission), BF16x4 engines"
@@ -488,6 +492,8 @@ def main() -> int:
     ap.add_argument("--rom-kernel-k", type=int, default=256)
     ap.add_argument("--rom-kernel-m", type=int, default=1024)
     ap.add_argument("--rom-kernel-prec", default="fp64,bf16x3,bf16x4")
+    ap.add_argument("--serve-engines-512", type=int, default=1,
+                    help="serving: K = 512 engines (each on its own stream) taking the K = 512 requests in turn")
     ap.add_argument("--rom-kernel-quiet-after", type=int, default=0,
                     help="--rom-kernel: forcing ends at seeded steps up to this (0: forcing throughout)")
     ap.add_argument("--rom-macro", type=int, default=10,

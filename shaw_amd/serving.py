@@ -106,7 +106,8 @@ def column_bytes(kv: int, ks: int, horizon: int, every: int, r: int) -> int:
 
 
 class Server:
-    """One serving run over resident engines (K -> engine)."""
+    """One serving run over resident engines (K -> engine, or K -> list of
+    engines of that size, which take the requests of size K in turn)."""
 
     def __init__(self, engines: dict, hbm_budget: float, chunk: int = 500, policy: str = "fifo",
                  max_wait: float = 0.0, clock=None):
@@ -125,23 +126,27 @@ class Server:
         """Serve the requests; keep(r) -> bool selects the requests whose
         seismograms are returned (default: all)."""
         L = _bind()
-        keys = sorted(self.engines)
-        index = {k: i for i, k in enumerate(keys)}
+        flat, index = [], {}  # engine tables; K -> their indices
+        for k in sorted(self.engines):
+            group = self.engines[k] if isinstance(self.engines[k], (list, tuple)) else [self.engines[k]]
+            index[k] = list(range(len(flat), len(flat) + len(group)))
+            flat += group
         for r in requests:
             if r.k not in index:
                 raise ValueError(f"no resident ROM of size {r.k}")
-        tables = (ShawEngine * len(keys))(*[self.engines[k].table() for k in keys])
+        tables = (ShawEngine * len(flat))(*[e.table() for e in flat])
         arr = (ShawRequest * max(1, len(requests)))()
         alive = []  # arrays and callbacks the C loop points into
         outs = []
         for i, r in enumerate(requests):
-            e = self.engines[r.k]
+            ei = index[r.k][r.rid % len(index[r.k])]
+            e = flat[ei]
             b = np.ascontiguousarray(r.b, dtype=np.float64)
             T = r.horizon // r.every if r.every else 0
             seis = np.zeros((r.m, T, e.R)) if (T and e.R and (keep is None or keep(r))) else None
             outs.append(seis)
             q = arr[i]
-            q.id, q.engine, q.m, q.horizon, q.every, q.arrival_s = r.rid, index[r.k], r.m, r.horizon, r.every, r.arrival
+            q.id, q.engine, q.m, q.horizon, q.every, q.arrival_s = r.rid, ei, r.m, r.horizon, r.every, r.arrival
             q.b = b.ctypes.data_as(C.POINTER(C.c_double))
             if r.ftab is not None:
                 f = np.ascontiguousarray(np.asarray(r.ftab)[:r.horizon], dtype=np.float64)
@@ -167,7 +172,7 @@ class Server:
             opts.clock, opts.wait_until = cb_clock, cb_wait
             alive += [cb_clock, cb_wait]
         st = ShawServeStats()
-        shaw_amd._check(L.shaw_serve(tables, len(keys), C.byref(opts), arr, len(requests), C.byref(st)))
+        shaw_amd._check(L.shaw_serve(tables, len(flat), C.byref(opts), arr, len(requests), C.byref(st)))
         for i, r in enumerate(requests):
             r.admitted, r.done = arr[i].admitted_s, arr[i].done_s
             r.seismograms = list(outs[i]) if outs[i] is not None else []
