@@ -900,7 +900,7 @@ __global__ void __launch_bounds__(256) k_quiet(const double* __restrict__ f, uin
   __shared__ uint64_t last[256];
   const uint64_t j = blockIdx.x;
   const unsigned t = threadIdx.x;
-  const double* fj = f + j * nsteps;
+  const double* fj = f + j * nsteps;  // (slots null: the result goes to quiet[j])
   double mx = 0.0;
   for (uint64_t e = t; e < nsteps; e += 256) mx = fmax(mx, fabs(fj[e]));
   peak[t] = mx;
@@ -919,7 +919,7 @@ __global__ void __launch_bounds__(256) k_quiet(const double* __restrict__ f, uin
     if (t < w) last[t] = last[t] > last[t + w] ? last[t] : last[t + w];
     __syncthreads();
   }
-  if (t == 0) quiet[slots[j]] = last[0] < 0xffffffffull ? (uint32_t)last[0] : 0xffffffffu;
+  if (t == 0) quiet[slots ? slots[j] : j] = last[0] < 0xffffffffull ? (uint32_t)last[0] : 0xffffffffu;
 }
 
 __global__ void k_admit_scatter(const uint64_t* __restrict__ slots, uint64_t n, const double* __restrict__ bStage,
@@ -1176,7 +1176,7 @@ struct shaw_rom {
   // quiet step
   std::vector<double> hArV, hArS;
   shaw::DevBuf<double> gvv, gvs, gsv, gss;
-  shaw::DevBuf<uint32_t> quiet;
+  shaw::DevBuf<uint32_t> quiet, quietStage;
   uint32_t macro = 0, macroBuilt = 0;
   shaw::DevBuf<uint64_t> stepCol, horizon, every;
   std::vector<uint64_t> hStep, hHorizon, hEvery;
@@ -1554,8 +1554,25 @@ void admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const dou
       SHAW_HIP(hipMemcpy(dPar, par.data(), par.size() * sizeof(double), hipMemcpyHostToDevice));
       forcingTableFromParams(n, dPar, h->dt, nsteps, h->stage.p + n * h->kvp, nsteps, h->stream);
     }
+    std::vector<uint64_t> slotOf(free);  // column j -> slot
+    if (h->macro) {
+      // columns whose forcing ends at about the same step share tiles, so
+      // that their tiles turn quiet together (a tile waits for its last
+      // column): the free slots in tile order go to the columns by quiet step
+      if (h->quietStage.n < n) h->quietStage.alloc(n);
+      hipLaunchKernelGGL(k_quiet, dim3((unsigned)n), dim3(256), 0, h->stream, h->stage.p + n * h->kvp, nsteps,
+                         nullptr, h->quietStage.p);
+      SHAW_HIP(hipGetLastError());
+      std::vector<uint32_t> q(n);
+      SHAW_HIP(hipMemcpyAsync(q.data(), h->quietStage.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+      SHAW_HIP(hipStreamSynchronize(h->stream));
+      std::vector<uint64_t> order(n);
+      for (uint64_t j = 0; j < n; ++j) order[j] = j;
+      std::stable_sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) { return q[x] < q[y]; });
+      for (uint64_t i = 0; i < n; ++i) slotOf[order[i]] = free[i];
+    }
     if (h->stageSlots.n < n) h->stageSlots.alloc(n);
-    h->stageSlots.upload(free.data(), n);
+    h->stageSlots.upload(slotOf.data(), n);
     hipLaunchKernelGGL(k_admit_scatter, dim3(64, (unsigned)n), dim3(256), 0, h->stream, h->stageSlots.p, n, h->stage.p,
                        h->stage.p + n * h->kvp, h->kvp, h->ksp, nsteps, h->maxSteps, h->b.p, h->ftab.p, h->qv.p, h->qs.p);
     SHAW_HIP(hipGetLastError());
@@ -1563,7 +1580,7 @@ void admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const dou
                        h->stageSlots.p, h->quiet.p);
     SHAW_HIP(hipGetLastError());
     for (uint64_t j = 0; j < n; ++j) {
-      const uint64_t s = free[j];
+      const uint64_t s = slotOf[j];
       h->hStep[s] = 0;
       h->hHorizon[s] = horizons[j];
       h->hEvery[s] = h->R ? every : 0;

@@ -102,16 +102,36 @@ def test_macro_steps_skip_unaligned_tiles():
         rom.close()
         return res, slots
 
-    (one, slots), (mac, slots2) = go(0), go(10)
-    assert slots == slots2
+    (one, _), (mac, slots) = go(0), go(10)
     shared = slots[0] // 16  # the tile of s1, shared with 8 columns of s2
     assert sum(1 for s in slots if s // 16 == shared) == 16
     for j, s in enumerate(slots):
-        if s // 16 == shared:  # no macro steps there: identical
+        if s // 16 == shared:  # no macro steps there: identical to the one-step run
             for x, y in zip(one[j], mac[j]):
                 np.testing.assert_array_equal(x, y)
     # the tile of s2 alone took macro steps
     assert any(np.abs(one[j][0] - mac[j][0]).max() > 0 for j, s in enumerate(slots) if s // 16 != shared)
+
+
+def test_macro_admission_groups_columns_by_quiet_step():
+    """With macro steps on, admission gives the free slots (in tile order)
+    to the columns in the order their forcing ends, so each tile holds
+    columns that turn quiet together."""
+    say("[gpu] ROM macro admission order")
+    K, n = 128, 400
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((K, K)) / np.sqrt(K)
+    M = 48
+    ends = rng.permutation(np.arange(20, 20 + 5 * M, 5))  # distinct last forcing steps
+    ftab = np.zeros((n, M))
+    for j in range(M):
+        ftab[: ends[j], j] = 1.0 + rng.random(ends[j])
+    rom = shaw_amd.Rom(A, -A.T.copy(), 0.5, shaw_amd.ROM_BF16X4, None, max_cols=M)
+    rom.set_macro_steps(10)
+    slots = rom.admit(rng.standard_normal((K, M)), ftab, [n] * M, 0)
+    rom.close()
+    assert sorted(slots) == list(range(M))
+    assert [int(ends[j]) for j in np.argsort(slots)] == sorted(int(e) for e in ends)
 
 
 def test_macro_steps_only_bf16():
