@@ -3,8 +3,8 @@
 A ctypes front end over ``oracle/liboracle.so``, the C++ restatement of the
 reference's FP64 FOM/ROM (``oracle/shaw_oracle.cpp``; each function there
 cites the reference file:line it follows).  Only the tests, the smoke check
-and the CPU-baseline leg of the benchmark use it, and only as the checker:
-the product path never imports this package.
+and the CPU-baseline leg of the benchmark use it, and only to check results
+against: the product path never imports this package.
 """
 from __future__ import annotations
 

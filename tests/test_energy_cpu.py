@@ -28,8 +28,8 @@ def csr(ora, dof, shape):
 
 
 def numpy_energy_weights(app: shaw_amd.App):
-    """A numpy restatement of the energy weights (the checker of the C++
-    shaw_app_energy_weights): (Mv [nVp], Ms [nSp]) of the FOM operators, from
+    """A numpy restatement of the energy weights, which the C++
+    shaw_app_energy_weights is compared with: (Mv [nVp], Ms [nSp]) of the FOM operators, from
     the ratios Ms[q] / Mv[p] = -J_vp[p, q] / J_sp[q, p] of the unmerged
     couplings, chained along row 1 and up every column (exact products of the
     coefficients). Points whose couplings are all merged (rows 0 and nr-1,

@@ -1,6 +1,6 @@
 """Pins the CPU oracle against the reference's own golds and against closed forms.
 
-The oracle (oracle/shaw_oracle.cpp) is the checker every GPU parity test uses,
+The oracle (oracle/shaw_oracle.cpp) is what every GPU parity test compares with,
 so it is validated here first: CSR Jacobians, graphs, coordinates, labels,
 mesh info, source/receiver gids (C++ unit-test literals), every present FOM
 state/seismogram gold (rank-1 and rank-2), the Ricker closed form, and ROM
