@@ -258,6 +258,8 @@ def serving_section(app, w, basis, args) -> dict:
                                max_samples=2000)
             if args.rom_macro:
                 rom.set_macro_steps(args.rom_macro)
+                if args.serve_time_budget:
+                    rom.set_time_budget(args.serve_time_budget)
             group.append(serving.GpuEngine(rom, 4096))
         engines[k] = group if len(group) > 1 else group[0]
     bcol = basis.mv[g] * rho_inv[g] * basis.dev.rows("vp", [g])[0]
@@ -319,7 +321,9 @@ def serving_section(app, w, basis, args) -> dict:
     rep.update({"workload": f"config4 on the energy-consistent basis: Poisson 200 req/s for {args.serve_seconds:g} s, "
                             "M~U{1..256}, K~U{128,256,512}, horizon~U[5000,20000] steps, source depth 640 km, "
                             "periods~U[20,70] s (forcing evaluated on the GPU at admission), BF16x4 engines"
-                            + (f" with macro steps of {args.rom_macro}" if args.rom_macro else "") + ", "
+                            + (f" with macro steps of {args.rom_macro}" if args.rom_macro else "")
+                            + (f" and a time budget per chunk (a macro step counted as {args.serve_time_budget} "
+                               "one-steps)" if args.rom_macro and args.serve_time_budget else "") + ", "
                             "180 receivers every 10 steps, C++ loop "
                             "(least work first, 2 s wait bound); then 10 s at 80 % of the measured drain rate",
                 "setup_s": setup})
@@ -498,6 +502,9 @@ def main() -> int:
                     help="--rom-kernel: forcing ends at seeded steps up to this (0: forcing throughout)")
     ap.add_argument("--rom-macro", type=int, default=10,
                     help="macro steps of the BF16 ROM runs and serving engines (0: one step at a time only)")
+    ap.add_argument("--serve-time-budget", type=int, default=2,
+                    help="serving engines with macro steps: a chunk is the time of chunk one-steps, a macro step "
+                         "counting as this many (0: chunks of steps)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

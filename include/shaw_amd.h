@@ -390,7 +390,20 @@ int shaw_rom_admit_signals(shaw_rom_t h, uint64_t n, const double* b, const int*
  * column's steps depend on the other columns of its tile. m = 0 (the
  * default) turns them off.                                                */
 int shaw_rom_set_macro_steps(shaw_rom_t h, uint32_t m);
-/* Advance every active column by up to nsteps (columns stop at their horizon). */
+/* Time budget for engines with macro steps (no counterpart in the
+ * reference; for serving, where a launch lasts as long as its slowest tile):
+ * with w in 1..64, shaw_rom_advance(h, n) gives every tile the time of n
+ * one-steps, a macro step counting as w of them, instead of n steps; a tile
+ * that takes macro steps then advances up to about n m / w steps (columns
+ * stop at their horizon). How far each column got is known once the
+ * engine's work is done: shaw_rom_steps, shaw_rom_state and the sample
+ * counts of retirement read it back. w = 0 (the default) turns it off.      */
+int shaw_rom_set_time_budget(shaw_rom_t h, uint32_t w);
+/* Step count of each of n admitted slots, after the engine's enqueued work
+ * (waits for it).                                                          */
+int shaw_rom_steps(shaw_rom_t h, uint64_t n, const uint64_t* slots, uint64_t* steps);
+/* Advance every active column by up to nsteps (columns stop at their horizon;
+ * under a time budget, see shaw_rom_set_time_budget).                      */
 int shaw_rom_advance(shaw_rom_t h, uint64_t nsteps);
 int shaw_rom_sync(shaw_rom_t h);
 /* Copy a column's seismogram LayoutLeft [r x T] and final reduced states
@@ -440,9 +453,12 @@ typedef struct {
   int (*admit_signals)(void* ctx, uint64_t n, const double* b, const int* kinds, const double* delays,
                        const double* periods, uint64_t nsteps, const uint64_t* horizons, uint64_t every,
                        uint64_t* slots);
+  /* step count of each slot once the enqueued work is done (shaw_rom_steps);
+   * NULL: every active column advances by exactly the steps asked for      */
+  int (*steps)(void* ctx, uint64_t n, const uint64_t* slots, uint64_t* steps);
 } shaw_engine;
 /* The table of a GPU engine: shaw_rom_admit/advance/sync/retire_batch/free_slots/
- * query/admit_signals. */
+ * query/admit_signals/steps. */
 int shaw_engine_from_rom(shaw_rom_t rom, shaw_engine* out);
 
 typedef struct {

@@ -138,6 +138,8 @@ def lib():
         "shaw_rom_retire_batch": (i32, [vp, u64, vp, u64, vp, vp]),
         "shaw_rom_free_slots": (i32, [vp, P(u64)]),
         "shaw_rom_set_macro_steps": (i32, [vp, C.c_uint32]),
+        "shaw_rom_set_time_budget": (i32, [vp, C.c_uint32]),
+        "shaw_rom_steps": (i32, [vp, u64, vp, vp]),
         "shaw_rom_state": (i32, [vp, u64, vp, vp, P(u64)]),
         "shaw_debug_mfma": (i32, [i32, vp, vp, vp]),
         "shaw_pod_basis": (i32, [vp, u64, u64, u64, vp, vp]),
@@ -697,6 +699,19 @@ class Rom:
         """BF16 engines: quiet, aligned tiles advance m steps per product
         (shaw_rom_set_macro_steps); 0 turns it off."""
         _check(lib().shaw_rom_set_macro_steps(self._h, m))
+
+    def set_time_budget(self, w: int):
+        """Engines with macro steps: advance(n) gives every tile the time of
+        n one-steps, a macro step counting as w (shaw_rom_set_time_budget);
+        0 turns it off."""
+        _check(lib().shaw_rom_set_time_budget(self._h, w))
+
+    def steps(self, slots):
+        """Step count of each slot once the enqueued work is done."""
+        sl = np.ascontiguousarray(slots, dtype=np.uint64)
+        out = np.zeros(max(1, sl.size), dtype=np.uint64)
+        _check(lib().shaw_rom_steps(self._h, sl.size, sl.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p)))
+        return [int(x) for x in out[:sl.size]]
 
     def advance(self, nsteps: int):
         _check(lib().shaw_rom_advance(self._h, nsteps))

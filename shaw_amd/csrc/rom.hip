@@ -195,6 +195,10 @@ struct RomArgs {
   const void *gvv, *gvs, *gsv, *gss;
   const uint32_t* quiet;  // [slot] first step from which the column's forcing is negligible
   uint32_t macro;
+  // time budget (macro steps on): 0, nsteps steps for every tile; w > 0, the
+  // time of nsteps one-steps for every tile, a macro step costing w of them,
+  // so tiles taking macro steps advance further (stepCol holds how far)
+  uint32_t budget;
 };
 
 // A tile's macro plan for a budget of B steps: k one-steps first (until every
@@ -738,15 +742,39 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_steps_bfm(RomArgs a) {
   const uint32_t hz = (uint32_t)a.horizon[slot0 + col];
   const uint32_t ev = (uint32_t)a.every[slot0 + col];
   const double* fcol = a.ftab + (uint64_t)(slot0 + col) * a.maxSteps;
-  uint32_t steps = hz > s0 ? hz - s0 : 0;
+  uint32_t span = hz > s0 ? hz - s0 : 0;  // steps to the tile's last horizon
 #pragma unroll
   for (int off = 32; off >= 1; off /= 2) {
-    const uint32_t o = (uint32_t)__shfl_xor((int)steps, off);
-    steps = o > steps ? o : steps;
+    const uint32_t o = (uint32_t)__shfl_xor((int)span, off);
+    span = o > span ? o : span;
   }
-  if ((uint64_t)steps > a.nsteps) steps = (uint32_t)a.nsteps;
+  uint32_t steps = (uint64_t)span > a.nsteps ? (uint32_t)a.nsteps : span;
   uint64_t mk = 0, mk2 = 0;  // macro steps: mk one-steps, mk2 macro steps, then one-steps to the end
-  if (a.macro) macroPlan(a.macro, s0, hz, ev, a.quiet[slot0 + col], a.nsteps, mk, mk2);
+  if (a.macro && a.budget) {
+    // the time of T one-steps: mk one-steps, mk2 macro steps of a.budget
+    // one-steps' time each, one-steps for the rest (horizons below 2^31)
+    const uint64_t T = a.nsteps < (1ull << 31) ? a.nsteps : (1ull << 31);
+    macroPlan(a.macro, s0, hz, ev, a.quiet[slot0 + col], T * a.macro, mk, mk2);
+    const uint64_t planned = mk2;  // what the horizons allow
+    if (mk2) {
+      const uint64_t cap = mk < T ? (T - mk) / a.budget : 0;
+      mk2 = mk2 < cap ? mk2 : cap;
+    }
+    if (mk2) {
+      // the time left goes to one-steps; unless a column nears its horizon,
+      // whole multiples of m only, so the tile ends aligned for the next launch
+      uint64_t rest = T - mk - a.budget * mk2;
+      if (mk2 < planned) rest -= rest % a.macro;
+      const uint64_t end = mk + (uint64_t)a.macro * mk2 + rest;
+      steps = end < (uint64_t)span ? (uint32_t)end : span;
+    }
+  } else if (a.macro) {
+    macroPlan(a.macro, s0, hz, ev, a.quiet[slot0 + col], a.nsteps, mk, mk2);
+  }
+  // the plan is the same in every lane: keep it in scalar registers
+  steps = (uint32_t)__builtin_amdgcn_readfirstlane((int)steps);
+  mk = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)mk);
+  mk2 = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)mk2);
   const uint32_t mSteps = a.macro;
   Item ring[D][TPP];
   Acc acc[TPP];
@@ -881,8 +909,10 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_steps_bfm(RomArgs a) {
       if (okS[t]) a.qs[colS + rtS[t] * 16 + rowBf16(lane, r)] = qs[t][r];
     }
   if (tid < kTileCols) {
+    // the tile ran `steps` steps: a.nsteps, or fewer when every column
+    // reached its horizon (the same counts), or more under a time budget
     const uint64_t h = a.horizon[slot0 + tid];
-    const uint64_t st = a.stepCol[slot0 + tid], s = st + a.nsteps;
+    const uint64_t st = a.stepCol[slot0 + tid], s = st + steps;
     a.stepCol[slot0 + tid] = s < h ? s : (st < h ? h : st);
   }
 }
@@ -1178,6 +1208,11 @@ struct shaw_rom {
   shaw::DevBuf<double> gvv, gvs, gsv, gss;
   shaw::DevBuf<uint32_t> quiet, quietStage;
   uint32_t macro = 0, macroBuilt = 0;
+  // time budget (shaw_rom_set_time_budget): 0 off, else a macro step's cost
+  // in one-steps; launches then advance tiles by amounts the device records
+  // in stepCol, read back into hStep once the stream is idle (settle)
+  uint32_t timeBudget = 0;
+  bool stepsOnDevice = false;
   shaw::DevBuf<uint64_t> stepCol, horizon, every;
   std::vector<uint64_t> hStep, hHorizon, hEvery;
   std::vector<uint8_t> used;
@@ -1194,6 +1229,13 @@ struct shaw_rom {
       pinnedBytes = bytes;
     }
     return static_cast<double*>(pinned);
+  }
+  void settle() {
+    SHAW_HIP(hipStreamSynchronize(stream));
+    if (stepsOnDevice) {
+      SHAW_HIP(hipMemcpy(hStep.data(), stepCol.p, maxCols * sizeof(uint64_t), hipMemcpyDeviceToHost));
+      stepsOnDevice = false;
+    }
   }
   void uploadSlotTables() {
     stepCol.upload(hStep.data(), maxCols);
@@ -1226,12 +1268,17 @@ struct shaw_rom {
     a.gvv = gvv.p; a.gvs = gvs.p; a.gsv = gsv.p; a.gss = gss.p;
     a.quiet = quiet.p;
     a.macro = precision == SHAW_ROM_FP64 ? 0 : macro;
+    a.budget = a.macro ? timeBudget : 0;
     const dim3 grid((unsigned)(maxCols / kTileCols));
     const size_t lds = ldsBytes();
     if (precision == SHAW_ROM_FP64) launchRomSteps<0>(tpw, grid, lds, stream, a);
     else if (precision == SHAW_ROM_BF16X3) launchRomSteps<2>(tpw, grid, lds, stream, a);
     else launchRomSteps<3>(tpw, grid, lds, stream, a);
     SHAW_HIP(hipGetLastError());
+    if (a.budget) {
+      stepsOnDevice = true;
+      return;
+    }
     for (uint64_t c = 0; c < maxCols; ++c) {
       const uint64_t s = hStep[c] + nsteps;
       hStep[c] = s < hHorizon[c] ? s : std::max(hStep[c], hHorizon[c]);
@@ -1523,7 +1570,7 @@ void admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const dou
     if (n == 0) return;
     // a launch still in flight reads and writes back every slot of its
     // workgroups: admit only on an idle engine
-    SHAW_HIP(hipStreamSynchronize(h->stream));
+    h->settle();
     const uint64_t need = std::max<uint64_t>(h->maxSteps, nsteps);
     if (need > h->maxSteps) {  // grow the per-slot forcing store, keeping admitted columns
       DevBuf<double> nb;
@@ -1618,7 +1665,7 @@ int shaw_rom_set_macro_steps(shaw_rom_t h, uint32_t m) {
     SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle");
     SHAW_REQUIRE(m == 0 || h->precision != SHAW_ROM_FP64, SHAW_ERR_ARG, "macro steps run in the BF16 kernels");
     SHAW_REQUIRE(m == 0 || (m >= 2 && m <= 64), SHAW_ERR_ARG, "macro steps: m must be 0 or 2..64");
-    SHAW_HIP(hipStreamSynchronize(h->stream));
+    h->settle();
     if (m && m != h->macroBuilt) {
       std::vector<double> g[4];
       macroOperator(h->hArV.data(), h->hArS.data(), h->kv, h->ks, h->dt, m, g[0], g[1], g[2], g[3]);
@@ -1637,6 +1684,26 @@ int shaw_rom_set_macro_steps(shaw_rom_t h, uint32_t m) {
   });
 }
 
+int shaw_rom_set_time_budget(shaw_rom_t h, uint32_t w) {
+  using namespace shaw;
+  return guarded([&] {
+    SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle");
+    SHAW_REQUIRE(w <= 64, SHAW_ERR_ARG, "time budget: a macro step's cost must be 0..64 one-steps");
+    h->settle();
+    h->timeBudget = w;
+  });
+}
+
+int shaw_rom_steps(shaw_rom_t h, uint64_t n, const uint64_t* slots, uint64_t* steps) {
+  return guarded([&] {
+    SHAW_REQUIRE(h && (n == 0 || (slots && steps)), SHAW_ERR_ARG, "null argument");
+    for (uint64_t j = 0; j < n; ++j)
+      SHAW_REQUIRE(slots[j] < h->maxCols && h->used[slots[j]], SHAW_ERR_ARG, "bad slot");
+    h->settle();
+    for (uint64_t j = 0; j < n; ++j) steps[j] = h->hStep[slots[j]];
+  });
+}
+
 int shaw_rom_advance(shaw_rom_t h, uint64_t nsteps) {
   return guarded([&] {
     SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle");
@@ -1647,14 +1714,14 @@ int shaw_rom_advance(shaw_rom_t h, uint64_t nsteps) {
 int shaw_rom_sync(shaw_rom_t h) {
   return guarded([&] {
     SHAW_REQUIRE(h, SHAW_ERR_ARG, "null handle");
-    SHAW_HIP(hipStreamSynchronize(h->stream));
+    h->settle();
   });
 }
 
 int shaw_rom_state(shaw_rom_t h, uint64_t slot, double* qv, double* qs, uint64_t* step) {
   return guarded([&] {
     SHAW_REQUIRE(h && slot < h->maxCols && h->used[slot], SHAW_ERR_ARG, "bad slot");
-    SHAW_HIP(hipStreamSynchronize(h->stream));
+    h->settle();
     if (qv) SHAW_HIP(hipMemcpy(qv, h->qv.p + slot * h->kvp, h->kv * 8, hipMemcpyDeviceToHost));
     if (qs) SHAW_HIP(hipMemcpy(qs, h->qs.p + slot * h->ksp, h->ks * 8, hipMemcpyDeviceToHost));
     if (step) *step = h->hStep[slot];
@@ -1667,7 +1734,7 @@ int shaw_rom_retire_batch(shaw_rom_t h, uint64_t n, const uint64_t* slots, uint6
     SHAW_REQUIRE(h && (n == 0 || slots), SHAW_ERR_ARG, "null argument");
     for (uint64_t j = 0; j < n; ++j)
       SHAW_REQUIRE(slots[j] < h->maxCols && h->used[slots[j]], SHAW_ERR_ARG, "bad slot");
-    SHAW_HIP(hipStreamSynchronize(h->stream));
+    h->settle();
     if (seis && h->R && max_samples) {
       // chunks of slots through a pinned buffer of at most 256 MB: async
       // copies of a chunk, one sync, then a threaded host copy
@@ -1702,7 +1769,7 @@ int shaw_rom_retire_batch(shaw_rom_t h, uint64_t n, const uint64_t* slots, uint6
 int shaw_rom_retire(shaw_rom_t h, uint64_t slot, double* seis, double* qv, double* qs, uint64_t* samples) {
   return guarded([&] {
     SHAW_REQUIRE(h && slot < h->maxCols && h->used[slot], SHAW_ERR_ARG, "bad slot");
-    SHAW_HIP(hipStreamSynchronize(h->stream));
+    h->settle();
     const uint64_t T = h->hEvery[slot] ? h->hStep[slot] / h->hEvery[slot] : 0;
     if (seis && T && h->R)
       SHAW_HIP(hipMemcpy(seis, h->seis.p + slot * h->R * h->maxSamples, h->R * T * 8, hipMemcpyDeviceToHost));
@@ -1741,6 +1808,7 @@ int shaw_rom_query(shaw_rom_t h, int* idle) {
       return;
     }
     SHAW_HIP(e);
+    h->settle();  // idle: reads back the steps of a time-budget launch
     *idle = 1;
   });
 }

@@ -16,14 +16,17 @@
 //      none passes it while it does not fit, so no request waits without
 //      bound under sustained load,
 //   3. advances every idle engine with active columns by one chunk of steps
-//      (each engine on its own stream), so the engines' work overlaps on the
-//      device, and
+//      (each engine on its own stream; under a time budget, by the time of
+//      that many one-steps, see shaw_rom_set_time_budget), so the engines'
+//      work overlaps on the device, and
 //   4. when an engine's chunk is done (a non-blocking query), counts its
-//      columns' steps and retires its requests that reached their horizon
-//      (latency = completion - arrival); the engine is then admitted into
-//      and advanced again without waiting for the others.
-// A column's arithmetic does not depend on the others in the batch, so a
-// request's result is the same alone and in the mix.
+//      columns' steps (the engine's steps call, when it has one) and
+//      retires its requests that reached their horizon (latency =
+//      completion - arrival); the engine is then admitted into and advanced
+//      again without waiting for the others.
+// With one step at a time a column's arithmetic does not depend on the
+// others in the batch, so a request's result is the same alone and in the
+// mix (with macro steps, to the BF16 rounding).
 // The reference has no serving loop; its ROM entry point runs one problem
 // per process (src/kokkos/main.cc:66-73, rom_problem_rank_one.hpp:123-159).
 #include <algorithm>
@@ -154,8 +157,11 @@ void serve(const shaw_engine* eng, int ne, const shaw_serve_options& o, shaw_req
       admitOne(best);
     }
   };
-  // an engine whose chunk finished: count the steps its columns took, retire
-  // the requests that reached their horizon
+  // an engine whose chunk finished: count the steps its columns took (the
+  // steps asked for, or under a time budget what the engine reports: a
+  // request is as far as its slowest column), retire the requests that
+  // reached their horizon
+  std::vector<uint64_t> stepsOf;
   auto harvest = [&](int k) {
     std::vector<Live> keep;
     for (Live& L : live) {
@@ -164,7 +170,14 @@ void serve(const shaw_engine* eng, int ne, const shaw_serve_options& o, shaw_req
         keep.push_back(std::move(L));
         continue;
       }
-      const uint64_t taken = std::min(launched[k], L.left);
+      uint64_t taken = std::min(launched[k], L.left);
+      if (eng[k].steps) {
+        stepsOf.resize(r.m);
+        check(eng[k].steps(eng[k].ctx, r.m, L.slots.data(), stepsOf.data()), "steps");
+        const uint64_t done = std::min(*std::min_element(stepsOf.begin(), stepsOf.end()), r.horizon);
+        const uint64_t before = r.horizon - L.left;
+        taken = done > before ? done - before : 0;
+      }
       L.left -= taken;
       forcingSteps += taken * r.m;
       if (L.left > 0) {
@@ -263,6 +276,9 @@ int romRetire(void* c, uint64_t n, const uint64_t* slots, uint64_t maxSamples, d
 }
 int romFree(void* c, uint64_t* n) { return shaw_rom_free_slots(static_cast<shaw_rom_t>(c), n); }
 int romQuery(void* c, int* idle) { return shaw_rom_query(static_cast<shaw_rom_t>(c), idle); }
+int romSteps(void* c, uint64_t n, const uint64_t* slots, uint64_t* steps) {
+  return shaw_rom_steps(static_cast<shaw_rom_t>(c), n, slots, steps);
+}
 }  // namespace
 
 extern "C" {
@@ -283,6 +299,7 @@ int shaw_engine_from_rom(shaw_rom_t rom, shaw_engine* out) {
     e.free_slots = romFree;
     e.query = romQuery;
     e.admit_signals = romAdmitSignals;
+    e.steps = romSteps;
     *out = e;
   });
 }

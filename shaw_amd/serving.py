@@ -31,6 +31,7 @@ _ADMIT_SIG = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_double),
                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_uint64), C.c_uint64,
                          C.POINTER(C.c_uint64))
 _QUERY = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_int))
+_STEPS = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
 _CLOCK = C.CFUNCTYPE(C.c_double, C.c_void_p)
 _WAIT = C.CFUNCTYPE(None, C.c_void_p, C.c_double)
 
@@ -38,7 +39,8 @@ _WAIT = C.CFUNCTYPE(None, C.c_void_p, C.c_double)
 class ShawEngine(C.Structure):
     _fields_ = [("ctx", C.c_void_p), ("kv", C.c_uint64), ("ks", C.c_uint64), ("r", C.c_uint64),
                 ("capacity", C.c_uint64), ("admit", _ADMIT), ("advance", _ADVANCE), ("sync", _SYNC),
-                ("retire_batch", _RETIRE), ("free_slots", _FREE), ("query", _QUERY), ("admit_signals", _ADMIT_SIG)]
+                ("retire_batch", _RETIRE), ("free_slots", _FREE), ("query", _QUERY), ("admit_signals", _ADMIT_SIG),
+                ("steps", _STEPS)]
 
 
 class ShawRequest(C.Structure):
@@ -219,10 +221,14 @@ class NumpyEngine:
     """CPU engine with the engine contract (FP64 reduced leapfrog in numpy,
     the step of rom_run_rank_one.hpp:112-151 per column), for testing the
     serving loop without a GPU. `tick(steps)` is called on every advance so a
-    simulated clock can charge time for the work."""
+    simulated clock can charge time for the work. With stretch > 1 it acts
+    like a GPU engine under a time budget: advance(n) takes a column whose
+    forcing is over n * stretch steps (one-steps, so its results do not
+    change), and the table reports the step counts (the `steps` call)."""
 
-    def __init__(self, ar_v, ar_s, dt, C_, capacity, tick=None):
+    def __init__(self, ar_v, ar_s, dt, C_, capacity, tick=None, stretch=1):
         self.ar_v, self.ar_s, self.dt, self.C = ar_v, ar_s, dt, C_
+        self.stretch = stretch
         self.kv, self.ks = ar_v.shape
         self.R = 0 if C_ is None else C_.shape[0]
         self.capacity = capacity
@@ -248,7 +254,8 @@ class NumpyEngine:
         if self.tick:
             self.tick(n)
         for c in self.cols.values():
-            for _ in range(n):
+            quiet = self.stretch > 1 and not np.any(c["f"][c["step"]:c["h"]])
+            for _ in range(n * self.stretch if quiet else n):
                 if c["step"] >= c["h"]:
                     break
                 fdt = c["f"][c["step"]] * self.dt
@@ -318,12 +325,19 @@ class NumpyEngine:
             idle[0] = 1  # advance() is synchronous
             return 0
 
+        def steps(ctx, n, slots, out):
+            for j in range(n):
+                out[j] = self.cols[int(slots[j])]["step"]
+            return 0
+
         t = ShawEngine()
         t.kv, t.ks, t.r, t.capacity = self.kv, self.ks, self.R, self.capacity
         t.admit, t.advance, t.sync = _ADMIT(guard(admit)), _ADVANCE(guard(advance)), _SYNC(lambda ctx: 0)
         t.retire_batch, t.free_slots, t.query = _RETIRE(guard(retire)), _FREE(guard(free)), _QUERY(guard(query))
         t.admit_signals = _ADMIT_SIG(guard(admit_signals))
-        self._keep = (t.admit, t.advance, t.sync, t.retire_batch, t.free_slots, t.query, t.admit_signals)
+        if self.stretch > 1:
+            t.steps = _STEPS(guard(steps))
+        self._keep = (t.admit, t.advance, t.sync, t.retire_batch, t.free_slots, t.query, t.admit_signals, t.steps)
         return t
 
 

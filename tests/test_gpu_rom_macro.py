@@ -55,6 +55,52 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("precision", [shaw_amd.ROM_BF16X3, shaw_amd.ROM_BF16X4])
+@pytest.mark.parametrize("K", [128, 512])
+def test_time_budget_vs_fp64(K, precision):
+    """Under a time budget (shaw_rom_set_time_budget, a macro step counting
+    as two one-steps) the first launch of 200 gives a tile k0 one-steps, k0
+    the first multiple of 10 at which its columns' forcing has ended, and
+    (200 - k0) / 2 macro steps, then every launch 100 macro steps (1000
+    steps); the step counters say so, every column ends at its horizon in 4
+    launches instead of 15, and the results stay within the macro-step
+    bound of the FP64 ROM."""
+    say(f"[gpu] ROM time budget K={K} precision={precision}")
+    M, n, every = 48, 3000, 10
+    A, B, C, ftab, hz = problem(K, M, n, 7 * K + precision)
+    ref_seis, ref_q, ref_steps = run(A, B, C, ftab, hz, shaw_amd.ROM_FP64, 0, every, [n])
+    rom = shaw_amd.Rom(A, -A.T.copy(), 0.5, precision, C, max_cols=M, max_samples=n // every)
+    rom.set_macro_steps(10)
+    rom.set_time_budget(2)
+    slots = rom.admit(B, ftab, hz, every)
+    seen = []
+    for _ in range(15):
+        rom.advance(200)
+        seen.append(rom.steps(slots))
+        if all(s == h for s, h in zip(seen[-1], hz)):
+            break
+    say(f"[gpu] steps after each launch (column 0): {[s[0] for s in seen]}")
+    nf = 40 + 7 * (np.arange(M) % 5)  # where each column's forcing ends (problem())
+    last = {}
+    for j, sl in enumerate(slots):
+        last[sl // 16] = max(last.get(sl // 16, 0), int(nf[j]))
+    k0 = [-(-last[sl // 16] // 10) * 10 for sl in slots]
+    assert seen[0] == [k + 10 * ((200 - k) // 2) for k in k0]
+    assert [b - a for a, b in zip(seen[0], seen[1])] == [1000] * M
+    assert len(seen) == 4 and seen[-1] == hz
+    assert [rom.state(s)[2] for s in slots] == hz
+    out = [rom.retire(s, n // every) for s in slots]
+    rom.close()
+    seis = [o[0] for o in out]
+    q = np.array([np.concatenate([o[1], o[2]]) for o in out])
+    for a, b in zip(seis, ref_seis):
+        assert a.shape == b.shape
+    err = max(rel(a, b) for a, b in zip(seis, ref_seis))
+    say(f"[gpu] K={K} precision={precision}: time budget vs FP64 seismograms {err:.2e}, states {rel(q, ref_q):.2e}")
+    bound = 2e-3 if precision == shaw_amd.ROM_BF16X3 else 5e-4
+    assert err <= bound and rel(q, ref_q) <= bound
+
+
+@pytest.mark.parametrize("precision", [shaw_amd.ROM_BF16X3, shaw_amd.ROM_BF16X4])
 @pytest.mark.parametrize("K", [128, 256, 384, 512])
 def test_macro_steps_vs_fp64(K, precision):
     say(f"[gpu] ROM macro K={K} precision={precision}")

@@ -81,14 +81,18 @@ def test_gpu_server_macro_steps_vs_fp64():
     30 steps, sampling every 10 steps, chunks of 20 so that admissions stay
     aligned): every request's seismograms stay within the BF16 bound of the
     FP64 engines' (which are bit-identical alone and in a mix), and the macro
-    steps did run (the results differ from one-step BF16x4)."""
+    steps did run (the results differ from one-step BF16x4). The same under a
+    time budget (quiet tiles run ahead; the loop reads the engines' step
+    counts), with the same seismogram lengths."""
     say("[gpu] server with macro steps")
 
-    def engines(prec, macro):
+    def engines(prec, macro, budget=0):
         e = {32: serving.GpuEngine(skew_rom(32, 10, prec), 64), 80: serving.GpuEngine(skew_rom(80, 11, prec), 64)}
         for eng in e.values():
             if macro:
                 eng.rom.set_macro_steps(macro)
+            if budget:
+                eng.rom.set_time_budget(budget)
         return e
 
     specs = [(0, 32, 5, 120), (1, 80, 17, 60), (2, 32, 40, 200), (3, 80, 3, 95), (4, 32, 20, 40), (5, 80, 33, 180),
@@ -102,21 +106,26 @@ def test_gpu_server_macro_steps_vs_fp64():
             out.append(r)
         return out
 
-    res = {}
-    for name, prec, macro in (("fp64", shaw_amd.ROM_FP64, 0), ("one", shaw_amd.ROM_BF16X4, 0),
-                              ("macro", shaw_amd.ROM_BF16X4, 10)):
-        e = engines(prec, macro)
+    res, launches = {}, {}
+    for name, prec, macro, budget in (("fp64", shaw_amd.ROM_FP64, 0, 0), ("one", shaw_amd.ROM_BF16X4, 0, 0),
+                                      ("macro", shaw_amd.ROM_BF16X4, 10, 0),
+                                      ("budget", shaw_amd.ROM_BF16X4, 10, 2)):
+        e = engines(prec, macro, budget)
         reqs = requests()
         rep = serving.Server(e, hbm_budget=1e12, chunk=20).run(reqs)
         assert rep["requests"] == len(specs)
-        res[name] = [r.seismograms for r in reqs]
+        assert rep["forcing_steps"] == sum(r.horizon * r.m for r in reqs)
+        res[name], launches[name] = [r.seismograms for r in reqs], rep["launches"]
         for eng in e.values():
             eng.rom.close()
-    worst = 0.0
-    for ref, mac in zip(res["fp64"], res["macro"]):
-        for x, y in zip(mac, ref):
-            assert x.shape == y.shape
-            worst = max(worst, float(np.linalg.norm(x - y) / np.linalg.norm(y)))
-    say(f"[gpu] macro-step serving vs FP64: worst column {worst:.2e}")
-    assert worst <= 2e-3
-    assert any(np.abs(x - y).max() > 0 for a, b in zip(res["one"], res["macro"]) for x, y in zip(a, b))
+    for name in ("macro", "budget"):
+        worst = 0.0
+        for ref, mac in zip(res["fp64"], res[name]):
+            for x, y in zip(mac, ref):
+                assert x.shape == y.shape
+                worst = max(worst, float(np.linalg.norm(x - y) / np.linalg.norm(y)))
+        say(f"[gpu] {name} serving vs FP64: worst column {worst:.2e}, {launches[name]} launches")
+        assert worst <= 2e-3
+        assert any(np.abs(x - y).max() > 0 for a, b in zip(res["one"], res[name]) for x, y in zip(a, b))
+    # quiet tiles ran ahead: fewer chunks than one step at a time
+    assert launches["budget"] < launches["macro"]

@@ -238,3 +238,41 @@ def test_signal_requests_equal_table_requests():
     with pytest.raises(shaw_amd.ShawError, match="forcing tables only"):
         serve({8: TablesOnly()}, [serving.Request(2, 8, b, None, h, 5, 0.0, signals=(kinds, delays, periods))],
               serving.SimClock(), hbm_budget=1e12)
+
+
+def quiet_req(rid, k, m, h, arrival, seed, forced, every=5):
+    """A request whose forcing is zero after its first `forced` steps."""
+    r = req(rid, k, m, h, arrival, seed, every)
+    r.ftab[forced:] = 0.0
+    return r
+
+
+def test_time_budget_engine_reports_its_steps():
+    """An engine whose columns advance by different amounts per chunk (as
+    GPU engines under a time budget do, quiet columns further) reports its
+    step counts; the loop retires a request only when its slowest column is
+    at the horizon, counts every step once, and the results are the engine's
+    own one-step results."""
+    clk = serving.SimClock()
+    arv, ars = stable_ops(8, 0)
+    C = np.random.default_rng(1).standard_normal((3, 8))
+    e = serving.NumpyEngine(arv, ars, 0.5, C, 32, tick=clk.work, stretch=4)
+    reqs = [quiet_req(0, 8, 3, 90, 0.0, 40, forced=10), quiet_req(1, 8, 2, 60, 0.0, 41, forced=60),
+            quiet_req(2, 8, 4, 75, 0.001, 42, forced=5), quiet_req(3, 8, 1, 30, 0.002, 43, forced=0)]
+    # one column of request 0 keeps its forcing to the end: that request
+    # advances at the one-step pace of its slowest column
+    reqs[0].ftab[:, 1] = np.random.default_rng(7).standard_normal(90)
+    rep = serve({8: e}, reqs, clk, hbm_budget=1e12, chunk=10)
+    assert rep["forcing_steps"] == sum(r.horizon * r.m for r in reqs)
+    assert ("advance", 10) in e.calls
+    for r in reqs:
+        assert all(s.shape == (r.horizon // r.every, 3) for s in r.seismograms)
+        d = make_engine(8, 8)
+        slots = d.admit(r.b, r.ftab, [r.horizon] * r.m, r.every)
+        d.advance(r.horizon)
+        for j, s in enumerate(slots):
+            np.testing.assert_array_equal(r.seismograms[j], d.retire(s))
+    # request 2 (75 steps, quiet after 5) takes 3 chunks and finishes before
+    # request 1 (60 steps, forced throughout, 6 chunks); one step at a time
+    # it would take 8
+    assert reqs[2].done < reqs[1].done
