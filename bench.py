@@ -236,9 +236,9 @@ def serving_section(app, w, basis, args) -> dict:
     depth (periods ~ U[20, 70] s), a ROM size K ~ U{128, 256, 512} (nested
     modes of one POD) and a horizon ~ U[5000, 20000] steps; one resident
     BF16x4 engine per K; admission by HBM footprint, least work first with a
-    wait bound; p50/p99 latency, the drain rate and the longest queue. A
-    second run at the arrival rate that offers 80 % of the measured drain
-    rate gives the latency of a load the engines sustain. Sampled requests'
+    wait bound; p50/p99 latency, the throughput and the longest queue. A
+    second run at 300 requests/s keeps the queue growing, so its throughput
+    is the engines' drain rate. Sampled requests'
     seismograms are checked against the FP64 ROM of the same K and the FP64
     FOM; the other requests' seismograms are discarded on retirement."""
     import shaw_amd
@@ -309,12 +309,14 @@ def serving_section(app, w, basis, args) -> dict:
                                 "err_vs_fp64_rom": a, "err_vs_fp64_fom": b_}
                                for r, a, b_ in zip(kept, e_rom, e_fom)]
     del reqs, kept
-    rate2 = 0.8 * rep["throughput_forcing_steps_per_s"] / mean_work
+    # the drain capacity: arrivals at 1.5x the rate, so that the queue grows
+    # and the engines never wait for work
+    rate2 = 1.5 * 200.0
     _, _, rep2 = one_run(rate2, args.serve_seconds, 5, 10 ** 9)
     log(f"serving {rate2:.1f} req/s: {rep2}")
-    rep["sustainable"] = {k: rep2[k] for k in ("arrival_rate_per_s", "offered_forcing_steps_per_s", "requests",
-                                                "p50_latency_s", "p99_latency_s", "max_latency_s", "max_wait_s",
-                                                "throughput_forcing_steps_per_s", "wall_s", "launches", "max_queue")}
+    rep["overload"] = {k: rep2[k] for k in ("arrival_rate_per_s", "offered_forcing_steps_per_s", "requests",
+                                             "p50_latency_s", "p99_latency_s", "max_latency_s", "max_wait_s",
+                                             "throughput_forcing_steps_per_s", "wall_s", "launches", "max_queue")}
     for e in engines.values():
         for eng in (e if isinstance(e, list) else [e]):
             eng.rom.close()
@@ -325,7 +327,7 @@ def serving_section(app, w, basis, args) -> dict:
                             + (f" and a time budget per chunk (a macro step counted as {args.serve_time_budget} "
                                "one-steps)" if args.rom_macro and args.serve_time_budget else "") + ", "
                             "180 receivers every 10 steps, C++ loop "
-                            "(least work first, 2 s wait bound); then 10 s at 80 % of the measured drain rate",
+                            "(least work first, 2 s wait bound); then 10 s at 300 req/s for the drain rate",
                 "setup_s": setup})
     return rep
 
