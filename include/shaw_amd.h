@@ -488,7 +488,8 @@ typedef struct {
   uint64_t chunk_steps;  /* steps per engine launch between admissions */
   int policy;            /* SHAW_SERVE_* */
   double max_wait_s;     /* LEAST_WORK: a request waiting this long is admitted before any
-                            other and is not passed (0: no bound) */
+                            other and no request for its engine passes it (none at all when
+                            it waits for HBM budget; 0: no bound) */
   double wall_limit_s;   /* fail with SHAW_ERR_STATE past this (0: none) */
   /* optional clock [s] and idle wait (tests): NULL = steady clock and sleep */
   double (*clock)(void* clock_ctx);

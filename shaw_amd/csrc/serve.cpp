@@ -13,8 +13,10 @@
 //      that does not fit. LEAST_WORK admits, among the requests that fit,
 //      the one with the least work (horizon x columns) first; a request
 //      that has waited max_wait seconds is admitted before any other, and
-//      none passes it while it does not fit, so no request waits without
-//      bound under sustained load,
+//      no request for its engine passes it while it does not fit (none at
+//      all when the HBM budget is what it waits for), so no request waits
+//      without bound under sustained load, and the other engines keep
+//      admitting,
 //   3. advances every idle engine with active columns by one chunk of steps
 //      (each engine on its own stream; under a time budget, by the time of
 //      that many one-steps, see shaw_rom_set_time_budget), so the engines'
@@ -140,17 +142,38 @@ void serve(const shaw_engine* eng, int ne, const shaw_serve_options& o, shaw_req
     resident += L.bytes;
     live.push_back(std::move(L));
   };
+  std::vector<char> held(ne);
   auto admit = [&]() {
     while (!queue.empty()) {
-      if (o.policy == SHAW_SERVE_FIFO || (o.max_wait_s > 0 && rel() - req[queue.front()].arrival_s >= o.max_wait_s)) {
-        // in arrival order; the head (or a request past its wait bound) is not passed
+      if (o.policy == SHAW_SERVE_FIFO) {
+        // in arrival order; the head is not passed
         if (!fits(queue.front())) return;
         admitOne(queue.begin());
         continue;
       }
+      // least work first, but requests past their wait bound go first, in
+      // arrival order; one that does not fit holds its engine (no request
+      // passes it there), or every engine when the HBM budget is short
+      std::fill(held.begin(), held.end(), 0);
+      bool admitted = false;
+      if (o.max_wait_s > 0) {
+        for (auto it = queue.begin(); it != queue.end(); ++it) {  // the queue is in arrival order
+          const uint64_t i = *it;
+          if (rel() - req[i].arrival_s < o.max_wait_s) break;
+          if (held[req[i].engine]) continue;
+          if (fits(i)) {
+            admitOne(it);
+            admitted = true;
+            break;
+          }
+          if (resident + footprint[i] > o.hbm_budget) return;
+          held[req[i].engine] = 1;
+        }
+      }
+      if (admitted) continue;
       auto best = queue.end();
       for (auto it = queue.begin(); it != queue.end(); ++it) {
-        if (!fits(*it)) continue;
+        if (held[req[*it].engine] || !fits(*it)) continue;
         if (best == queue.end() || req[*it].horizon * req[*it].m < req[*best].horizon * req[*best].m) best = it;
       }
       if (best == queue.end()) return;

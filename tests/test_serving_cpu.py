@@ -186,6 +186,32 @@ def test_least_work_wait_is_bounded_under_sustained_overload():
     assert rep["max_wait_s"] == pytest.approx(max(r.admitted - r.arrival for r in small + [big]))
 
 
+def test_overdue_request_holds_only_its_engine():
+    """A request past its wait bound that waits for its engine's slots
+    keeps the requests of that engine from passing it, but not those of
+    another engine: they are admitted as they arrive (within about one
+    chunk of either engine), although the overdue one waits 20 chunks for
+    its engine's resident requests to finish."""
+    clk = serving.SimClock(per_call=1e-6, per_step=1e-4)
+    a, b = make_engine(4, 8, r=1, clock=clk), make_engine(6, 8, seed=5, r=1, clock=clk)
+    small = [req(i + 1, 4, 2, 200, 2e-4 * i, 100 + i, every=10) for i in range(40)]
+    big = req(0, 4, 8, 10, 1e-3, 7, every=10)
+    other = [req(1000 + i, 6, 1, 10, 5e-4 * i + 1e-4, 500 + i, every=10) for i in range(200)]
+    W = 0.005
+    rep = serve({4: a, 6: b}, [big] + small + other, clk, hbm_budget=1e12, chunk=10, policy="least_work",
+                max_wait=W)
+    assert rep["requests"] == 1 + len(small) + len(other)
+    overdue = big.arrival + W
+    assert big.admitted - overdue > 0.02  # its engine's resident requests take 200 steps
+    waiting = [r for r in other if overdue < r.arrival < big.admitted]
+    assert len(waiting) > 20
+    # the other engine's requests are not held back by it
+    assert max(r.admitted - r.arrival for r in waiting) < 3e-3
+    # no later request of its own engine passes it while it waits
+    assert not [r for r in small if r.arrival > big.arrival and overdue <= r.admitted < big.admitted]
+    assert any(r.arrival > big.arrival and r.admitted > big.admitted for r in small)
+
+
 def test_unknown_policy_is_refused():
     with pytest.raises(ValueError, match="admission policy"):
         serving.Server({8: make_engine(8, 6)}, hbm_budget=1e12, policy="lifo")
