@@ -284,7 +284,7 @@ def serving_section(app, w, basis, args) -> dict:
         kept = [r for r in reqs if r.rid % keep_every == 0][:6]
         for r in reqs:
             r.keep = r in kept
-        srv = serving.Server(engines, hbm_budget=200e9, chunk=1000, policy="least_work", max_wait=2.0)
+        srv = serving.Server(engines, hbm_budget=200e9, chunk=args.serve_chunk, policy="least_work", max_wait=2.0)
         rep = srv.run(reqs, wall_limit=900.0, keep=lambda r: r.keep)
         rep["offered_forcing_steps_per_s"] = rate * mean_work
         rep["arrival_rate_per_s"] = rate
@@ -504,6 +504,7 @@ def main() -> int:
                     help="--rom-kernel: forcing ends at seeded steps up to this (0: forcing throughout)")
     ap.add_argument("--rom-macro", type=int, default=10,
                     help="macro steps of the BF16 ROM runs and serving engines (0: one step at a time only)")
+    ap.add_argument("--serve-chunk", type=int, default=1000, help="serving: steps (or one-step times) per chunk")
     ap.add_argument("--serve-time-budget", type=int, default=2,
                     help="serving engines with macro steps: a chunk is the time of chunk one-steps, a macro step "
                          "counting as this many (0: chunks of steps)")

@@ -5,7 +5,9 @@ at the bounds the one-step BF16 runs meet, on every row-tile layout (K = 128,
 256, 384, 512), with horizons that are not multiples of m, sampling, a tile
 whose columns sit at different residues mod m (no macro steps), and chunked
 launches; and the step counters and seismogram lengths equal the one-step
-run's exactly."""
+run's exactly. Under a time budget (shaw_rom_set_time_budget) quiet tiles
+run ahead by the stated rule, with the same bounds; without macro steps a
+budget changes nothing."""
 import sys
 
 import numpy as np
@@ -191,4 +193,25 @@ def test_macro_steps_only_bf16():
     rom = shaw_amd.Rom(A, -A.T.copy(), 0.5, shaw_amd.ROM_BF16X3, None, max_cols=16)
     with pytest.raises(shaw_amd.ShawError):
         rom.set_macro_steps(1)
+    with pytest.raises(shaw_amd.ShawError):
+        rom.set_time_budget(65)
     rom.close()
+
+
+def test_time_budget_without_macro_steps_keeps_step_counts():
+    """A time budget on an engine without macro steps (FP64, or BF16 with
+    m = 0) changes nothing: every launch advances exactly its steps."""
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((32, 32)) / 6.0
+    B = rng.standard_normal((32, 20))
+    f = np.zeros((500, 20))
+    f[:5] = rng.standard_normal((5, 20))
+    for prec in (shaw_amd.ROM_FP64, shaw_amd.ROM_BF16X4):
+        rom = shaw_amd.Rom(A, -A.T.copy(), 0.5, prec, None, max_cols=32)
+        rom.set_time_budget(2)
+        slots = rom.admit(B, f, [500] * 20, 0)
+        rom.advance(120)
+        assert rom.steps(slots) == [120] * 20
+        rom.advance(1000)
+        assert rom.steps(slots) == [500] * 20
+        rom.close()
