@@ -184,9 +184,18 @@ void serve(const shaw_engine* eng, int ne, const shaw_serve_options& o, shaw_req
   // steps asked for, or under a time budget what the engine reports: a
   // request is as far as its slowest column), retire the requests that
   // reached their horizon
-  std::vector<uint64_t> stepsOf;
+  std::vector<uint64_t> slotsOf, stepsOf;
   auto harvest = [&](int k) {
+    // the step counts of all the engine's live columns, in one call
+    if (eng[k].steps) {
+      slotsOf.clear();
+      for (const Live& L : live)
+        if (req[L.idx].engine == k) slotsOf.insert(slotsOf.end(), L.slots.begin(), L.slots.end());
+      stepsOf.resize(slotsOf.size());
+      if (!slotsOf.empty()) check(eng[k].steps(eng[k].ctx, slotsOf.size(), slotsOf.data(), stepsOf.data()), "steps");
+    }
     std::vector<Live> keep;
+    size_t at = 0;  // this request's columns in stepsOf
     for (Live& L : live) {
       shaw_request& r = req[L.idx];
       if (r.engine != k) {
@@ -195,9 +204,8 @@ void serve(const shaw_engine* eng, int ne, const shaw_serve_options& o, shaw_req
       }
       uint64_t taken = std::min(launched[k], L.left);
       if (eng[k].steps) {
-        stepsOf.resize(r.m);
-        check(eng[k].steps(eng[k].ctx, r.m, L.slots.data(), stepsOf.data()), "steps");
-        const uint64_t done = std::min(*std::min_element(stepsOf.begin(), stepsOf.end()), r.horizon);
+        const uint64_t done = std::min(*std::min_element(stepsOf.begin() + at, stepsOf.begin() + at + r.m), r.horizon);
+        at += r.m;
         const uint64_t before = r.horizon - L.left;
         taken = done > before ? done - before : 0;
       }
