@@ -125,13 +125,15 @@ __global__ void k_assemble_sp(DevGrid g, int kind, const double* __restrict__ mp
   }
 }
 
-// out[s * m + k] = signal_k(s * dt) (forcingTable's time: the product s dt);
-// signal k: kind (as a double), delay, period at par[3 k .. 3 k + 2]
+// out[s * m + k] (ldOut 0) or out[k * ldOut + s] = signal_k(s * dt)
+// (forcingTable's time: the product s dt); signal k: kind (as a double),
+// delay, period at par[3 k .. 3 k + 2]. Consecutive threads take consecutive
+// entries of out in either layout, so the stores coalesce.
 __global__ void k_forcing_table(uint64_t m, const double* __restrict__ par, double dt, uint64_t nsteps,
                                 double* __restrict__ out, uint64_t ldOut) {
   const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m * nsteps) return;
-  const uint64_t s = e / m, k = e % m;
+  const uint64_t s = ldOut ? e % nsteps : e / m, k = ldOut ? e / nsteps : e % m;
   const double period = par[3 * k + 2];
   const double freq = 1.0 / period;
   const double t = (double)s * dt;

@@ -953,12 +953,14 @@ __global__ void __launch_bounds__(256) k_quiet(const double* __restrict__ f, uin
 }
 
 __global__ void k_admit_scatter(const uint64_t* __restrict__ slots, uint64_t n, const double* __restrict__ bStage,
-                                const double* __restrict__ fStage, uint32_t kvp, uint32_t ksp, uint64_t nsteps,
-                                uint64_t maxSteps, double* __restrict__ b, double* __restrict__ ftab,
-                                double* __restrict__ qv, double* __restrict__ qs) {
+                                const double* __restrict__ fStage, const uint32_t* __restrict__ quietStage,
+                                uint32_t kvp, uint32_t ksp, uint64_t nsteps, uint64_t maxSteps,
+                                double* __restrict__ b, double* __restrict__ ftab, double* __restrict__ qv,
+                                double* __restrict__ qs, uint32_t* __restrict__ quiet) {
   const uint64_t j = blockIdx.y;
   if (j >= n) return;
   const uint64_t s = slots[j];
+  if (blockIdx.x == 0 && threadIdx.x == 0) quiet[s] = quietStage[j];
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nsteps; e += (uint64_t)gridDim.x * blockDim.x)
     ftab[s * maxSteps + e] = fStage[j * nsteps + e];
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < kvp; e += (uint64_t)gridDim.x * blockDim.x) {
@@ -1220,6 +1222,23 @@ struct shaw_rom {
   shaw::DevBuf<uint64_t> stageSlots;
   void* pinned = nullptr;              // host staging of retired seismograms
   size_t pinnedBytes = 0;
+  // pinned host staging of admissions (forcing columns, signal parameters,
+  // slots) and of the slot tables, copied on the engine's stream without
+  // waiting: nothing rewrites them before the next call on the engine,
+  // which waits for the stream first (settle)
+  void* admitPinned = nullptr;
+  size_t admitPinnedBytes = 0;
+  uint64_t* slotPinned = nullptr;      // [3][maxCols]: steps, horizons, sampling periods
+  void* admitStage(size_t bytes) {
+    if (bytes > admitPinnedBytes) {
+      if (admitPinned) (void)hipHostFree(admitPinned);
+      admitPinned = nullptr;
+      admitPinnedBytes = 0;
+      SHAW_HIP(hipHostMalloc(&admitPinned, bytes));
+      admitPinnedBytes = bytes;
+    }
+    return admitPinned;
+  }
   double* pinnedStage(size_t bytes) {
     if (bytes > pinnedBytes) {
       if (pinned) (void)hipHostFree(pinned);
@@ -1237,14 +1256,22 @@ struct shaw_rom {
       stepsOnDevice = false;
     }
   }
+  // stream-ordered upload of the slot tables (callers have settled the stream)
   void uploadSlotTables() {
-    stepCol.upload(hStep.data(), maxCols);
-    horizon.upload(hHorizon.data(), maxCols);
-    every.upload(hEvery.data(), maxCols);
+    if (!slotPinned) SHAW_HIP(hipHostMalloc((void**)&slotPinned, 3 * maxCols * sizeof(uint64_t)));
+    const size_t bytes = maxCols * sizeof(uint64_t);
+    std::memcpy(slotPinned, hStep.data(), bytes);
+    std::memcpy(slotPinned + maxCols, hHorizon.data(), bytes);
+    std::memcpy(slotPinned + 2 * maxCols, hEvery.data(), bytes);
+    SHAW_HIP(hipMemcpyAsync(stepCol.p, slotPinned, bytes, hipMemcpyHostToDevice, stream));
+    SHAW_HIP(hipMemcpyAsync(horizon.p, slotPinned + maxCols, bytes, hipMemcpyHostToDevice, stream));
+    SHAW_HIP(hipMemcpyAsync(every.p, slotPinned + 2 * maxCols, bytes, hipMemcpyHostToDevice, stream));
   }
   ~shaw_rom() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (pinned) (void)hipHostFree(pinned);
+    if (admitPinned) (void)hipHostFree(admitPinned);
+    if (slotPinned) (void)hipHostFree(slotPinned);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -1537,6 +1564,16 @@ int shaw_rom_create(const shaw_rom_desc* d, shaw_rom_t* out) {
 
 namespace shaw {
 namespace {
+// when a signal (signalAt) falls for good to 2^-64 of its peak, in seconds,
+// for ordering columns by when their forcing ends: a Ricker wavelet at
+// |t - delay| = 2.23 periods (pi^2 f^2 tau^2 = 49), gaussDer at 6.87
+// (f^2 tau^2 = 47.1); a sinusoid stays on
+double signalEnd(int kind, double delay, double period) {
+  if (kind == 0) return delay + 2.23 * period;
+  if (kind == 2) return delay + 6.87 * period;
+  return std::numeric_limits<double>::infinity();
+}
+
 // admission with the forcing series from a host table ftab [nsteps][n], or
 // (ftab null) evaluated on the device from signal parameters
 void admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const double* ftab, const int* kinds,
@@ -1582,49 +1619,69 @@ void admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const dou
       std::swap(nb.n, h->ftab.n);
       h->maxSteps = need;
     }
-    // stage [n][Kvp] forcing columns and [n][nsteps] forcing series, one copy, one scatter
-    // (with signals the parameters [n][3] follow the series region, and only
-    // b and the parameters cross PCIe)
-    std::vector<double> st(n * (h->kvp + (ftab ? nsteps : 0)), 0.0);
-    for (uint64_t j = 0; j < n; ++j) {
-      for (uint64_t k = 0; k < h->kv; ++k) st[j * h->kvp + k] = b[k * n + j];
-      if (!ftab) continue;
-      double* f = st.data() + n * h->kvp + j * nsteps;
-      for (uint64_t t = 0; t < nsteps; ++t) f[t] = ftab[t * n + j];
-    }
+    // stage [n][Kvp] forcing columns and [n][nsteps] forcing series, one scatter.
+    // With signals only b, the parameters [n][3] (after the series region)
+    // and the slots cross PCIe, from the pinned admission buffer on the
+    // engine's stream, and the series is evaluated on the device.
     const uint64_t stageNeed = n * (h->kvp + nsteps) + (ftab ? 0 : 3 * n);
     if (h->stage.n < stageNeed) h->stage.alloc(stageNeed);
-    h->stage.upload(st.data(), st.size());
-    if (!ftab) {  // the series [n][nsteps] of the staging buffer, evaluated on the device
+    uint64_t* hSlots = nullptr;  // pinned slots of the signal path
+    if (ftab) {
+      std::vector<double> st(n * (h->kvp + nsteps), 0.0);
+      for (uint64_t j = 0; j < n; ++j) {
+        for (uint64_t k = 0; k < h->kv; ++k) st[j * h->kvp + k] = b[k * n + j];
+        double* f = st.data() + n * h->kvp + j * nsteps;
+        for (uint64_t t = 0; t < nsteps; ++t) f[t] = ftab[t * n + j];
+      }
+      h->stage.upload(st.data(), st.size());
+    } else {
       const std::vector<double> par = signalParams(n, kinds, delays, periods);
+      double* hb = static_cast<double*>(h->admitStage((n * h->kvp + 3 * n) * sizeof(double) + n * sizeof(uint64_t)));
+      std::fill(hb, hb + n * h->kvp, 0.0);
+      for (uint64_t j = 0; j < n; ++j)
+        for (uint64_t k = 0; k < h->kv; ++k) hb[j * h->kvp + k] = b[k * n + j];
+      std::copy(par.begin(), par.end(), hb + n * h->kvp);
+      hSlots = reinterpret_cast<uint64_t*>(hb + n * h->kvp + 3 * n);
       double* dPar = h->stage.p + n * (h->kvp + nsteps);
-      SHAW_HIP(hipMemcpy(dPar, par.data(), par.size() * sizeof(double), hipMemcpyHostToDevice));
+      SHAW_HIP(hipMemcpyAsync(h->stage.p, hb, n * h->kvp * sizeof(double), hipMemcpyHostToDevice, h->stream));
+      SHAW_HIP(hipMemcpyAsync(dPar, hb + n * h->kvp, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
       forcingTableFromParams(n, dPar, h->dt, nsteps, h->stage.p + n * h->kvp, nsteps, h->stream);
     }
+    // each column's first quiet step, staged; the scatter puts it at the slot
+    if (h->quietStage.n < n) h->quietStage.alloc(n);
+    hipLaunchKernelGGL(k_quiet, dim3((unsigned)n), dim3(256), 0, h->stream, h->stage.p + n * h->kvp, nsteps, nullptr,
+                       h->quietStage.p);
+    SHAW_HIP(hipGetLastError());
     std::vector<uint64_t> slotOf(free);  // column j -> slot
     if (h->macro) {
       // columns whose forcing ends at about the same step share tiles, so
       // that their tiles turn quiet together (a tile waits for its last
-      // column): the free slots in tile order go to the columns by quiet step
-      if (h->quietStage.n < n) h->quietStage.alloc(n);
-      hipLaunchKernelGGL(k_quiet, dim3((unsigned)n), dim3(256), 0, h->stream, h->stage.p + n * h->kvp, nsteps,
-                         nullptr, h->quietStage.p);
-      SHAW_HIP(hipGetLastError());
-      std::vector<uint32_t> q(n);
-      SHAW_HIP(hipMemcpyAsync(q.data(), h->quietStage.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
-      SHAW_HIP(hipStreamSynchronize(h->stream));
+      // column): the free slots in tile order go to the columns by quiet
+      // step; for signals by when the signal ends (no wait for the device)
+      std::vector<double> q(n);
+      if (ftab) {
+        std::vector<uint32_t> qd(n);
+        SHAW_HIP(hipMemcpyAsync(qd.data(), h->quietStage.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+        SHAW_HIP(hipStreamSynchronize(h->stream));
+        for (uint64_t j = 0; j < n; ++j) q[j] = qd[j];
+      } else {
+        for (uint64_t j = 0; j < n; ++j) q[j] = signalEnd(kinds[j], delays[j], periods[j]);
+      }
       std::vector<uint64_t> order(n);
       for (uint64_t j = 0; j < n; ++j) order[j] = j;
       std::stable_sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) { return q[x] < q[y]; });
       for (uint64_t i = 0; i < n; ++i) slotOf[order[i]] = free[i];
     }
     if (h->stageSlots.n < n) h->stageSlots.alloc(n);
-    h->stageSlots.upload(slotOf.data(), n);
+    if (hSlots) {
+      std::copy(slotOf.begin(), slotOf.end(), hSlots);
+      SHAW_HIP(hipMemcpyAsync(h->stageSlots.p, hSlots, n * sizeof(uint64_t), hipMemcpyHostToDevice, h->stream));
+    } else {
+      h->stageSlots.upload(slotOf.data(), n);
+    }
     hipLaunchKernelGGL(k_admit_scatter, dim3(64, (unsigned)n), dim3(256), 0, h->stream, h->stageSlots.p, n, h->stage.p,
-                       h->stage.p + n * h->kvp, h->kvp, h->ksp, nsteps, h->maxSteps, h->b.p, h->ftab.p, h->qv.p, h->qs.p);
-    SHAW_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_quiet, dim3((unsigned)n), dim3(256), 0, h->stream, h->stage.p + n * h->kvp, nsteps,
-                       h->stageSlots.p, h->quiet.p);
+                       h->stage.p + n * h->kvp, h->quietStage.p, h->kvp, h->ksp, nsteps, h->maxSteps, h->b.p,
+                       h->ftab.p, h->qv.p, h->qs.p, h->quiet.p);
     SHAW_HIP(hipGetLastError());
     for (uint64_t j = 0; j < n; ++j) {
       const uint64_t s = slotOf[j];
@@ -1634,7 +1691,6 @@ void admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const dou
       h->used[s] = 1;
       slots[j] = s;
     }
-    SHAW_HIP(hipStreamSynchronize(h->stream));
     h->uploadSlotTables();
 }
 }  // namespace
