@@ -917,20 +917,18 @@ __global__ void __launch_bounds__(WAVES * 64) k_rom_steps_bfm(RomArgs a) {
   }
 }
 
-// admission: column j of the staged inputs goes to slot slots[j]; its states
-// start at zero
 // admission: the first step from which the staged forcing series of
 // column j ([n][nsteps]) stays at or below 2^-64 of its largest magnitude (0
-// when it is all zero) goes to quiet[slots[j]]. A forcing value that small
-// adds under 2^-64 of the column's response, far below the BF16 kernels'
+// when it is all zero) goes to quiet[j]. A forcing value that small adds
+// under 2^-64 of the column's response, far below the BF16 kernels'
 // rounding, so a quiet column may take macro steps without its forcing.
 __global__ void __launch_bounds__(256) k_quiet(const double* __restrict__ f, uint64_t nsteps,
-                                               const uint64_t* __restrict__ slots, uint32_t* __restrict__ quiet) {
+                                               uint32_t* __restrict__ quiet) {
   __shared__ double peak[256];
   __shared__ uint64_t last[256];
   const uint64_t j = blockIdx.x;
   const unsigned t = threadIdx.x;
-  const double* fj = f + j * nsteps;  // (slots null: the result goes to quiet[j])
+  const double* fj = f + j * nsteps;
   double mx = 0.0;
   for (uint64_t e = t; e < nsteps; e += 256) mx = fmax(mx, fabs(fj[e]));
   peak[t] = mx;
@@ -949,9 +947,11 @@ __global__ void __launch_bounds__(256) k_quiet(const double* __restrict__ f, uin
     if (t < w) last[t] = last[t] > last[t + w] ? last[t] : last[t + w];
     __syncthreads();
   }
-  if (t == 0) quiet[slots ? slots[j] : j] = last[0] < 0xffffffffull ? (uint32_t)last[0] : 0xffffffffu;
+  if (t == 0) quiet[j] = last[0] < 0xffffffffull ? (uint32_t)last[0] : 0xffffffffu;
 }
 
+// admission: column j of the staged inputs (forcing column, series, quiet
+// step) goes to slot slots[j]; its states start at zero
 __global__ void k_admit_scatter(const uint64_t* __restrict__ slots, uint64_t n, const double* __restrict__ bStage,
                                 const double* __restrict__ fStage, const uint32_t* __restrict__ quietStage,
                                 uint32_t kvp, uint32_t ksp, uint64_t nsteps, uint64_t maxSteps,
@@ -1649,7 +1649,7 @@ void admit(shaw_rom_t h, uint64_t n, const double* b, uint64_t nsteps, const dou
     }
     // each column's first quiet step, staged; the scatter puts it at the slot
     if (h->quietStage.n < n) h->quietStage.alloc(n);
-    hipLaunchKernelGGL(k_quiet, dim3((unsigned)n), dim3(256), 0, h->stream, h->stage.p + n * h->kvp, nsteps, nullptr,
+    hipLaunchKernelGGL(k_quiet, dim3((unsigned)n), dim3(256), 0, h->stream, h->stage.p + n * h->kvp, nsteps,
                        h->quietStage.p);
     SHAW_HIP(hipGetLastError());
     std::vector<uint64_t> slotOf(free);  // column j -> slot
