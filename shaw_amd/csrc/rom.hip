@@ -1252,7 +1252,12 @@ struct shaw_rom {
   void settle() {
     SHAW_HIP(hipStreamSynchronize(stream));
     if (stepsOnDevice) {
-      SHAW_HIP(hipMemcpy(hStep.data(), stepCol.p, maxCols * sizeof(uint64_t), hipMemcpyDeviceToHost));
+      // through the pinned slot-table buffer (its uploads are done: the
+      // stream is idle), on the engine's stream
+      if (!slotPinned) SHAW_HIP(hipHostMalloc((void**)&slotPinned, 3 * maxCols * sizeof(uint64_t)));
+      SHAW_HIP(hipMemcpyAsync(slotPinned, stepCol.p, maxCols * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+      SHAW_HIP(hipStreamSynchronize(stream));
+      std::memcpy(hStep.data(), slotPinned, maxCols * sizeof(uint64_t));
       stepsOnDevice = false;
     }
   }
