@@ -182,6 +182,36 @@ def test_macro_admission_groups_columns_by_quiet_step():
     assert [int(ends[j]) for j in np.argsort(slots)] == sorted(int(e) for e in ends)
 
 
+def test_macro_signal_admission_orders_by_signal_end():
+    """Signal requests are ordered on the host by when their signals end
+    (no wait for the device); the order agrees with the quiet steps of the
+    forcing series themselves (the first step from which |f| stays at or
+    below 2^-64 of its peak), here from the host forcing table: Ricker
+    wavelets and gaussDer pulses of mixed delays and periods, and a
+    sinusoid (never quiet) last."""
+    say("[gpu] ROM macro signal admission order")
+    K, n, dt = 128, 6000, 0.05
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((K, K)) / np.sqrt(K)
+    M = 40
+    kinds = ["ricker"] * 30 + ["gaussDer"] * 9 + ["sinusoid"]
+    delays = list(rng.uniform(20.0, 120.0, M))
+    periods = list(rng.uniform(5.0, 30.0, M))
+    f = shaw_amd.forcing_table(kinds, delays, periods, dt, n)
+    peak = np.abs(f).max(axis=0)
+    above = np.abs(f) > peak * 2.0 ** -64
+    quiet = np.array([n - int(np.argmax(above[::-1, j])) if above[-1, j] == 0 else n + 1 for j in range(M)])
+    rom = shaw_amd.Rom(A, -A.T.copy(), dt, shaw_amd.ROM_BF16X4, None, max_cols=M)
+    rom.set_macro_steps(10)
+    slots = rom.admit_signals(rng.standard_normal((K, M)), kinds, delays, periods, n, None, 0)
+    rom.close()
+    assert sorted(slots) == list(range(M))
+    by_slot = [int(quiet[j]) for j in np.argsort(slots)]
+    say(f"[gpu] quiet steps in slot order: {by_slot}")
+    assert by_slot == sorted(by_slot)
+    assert by_slot[-1] == n + 1  # the sinusoid
+
+
 def test_macro_steps_only_bf16():
     rng = np.random.default_rng(0)
     A = rng.standard_normal((32, 32)) / 6.0
