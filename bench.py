@@ -504,7 +504,7 @@ def main() -> int:
                     help="--rom-kernel: forcing ends at seeded steps up to this (0: forcing throughout)")
     ap.add_argument("--rom-macro", type=int, default=10,
                     help="macro steps of the BF16 ROM runs and serving engines (0: one step at a time only)")
-    ap.add_argument("--serve-chunk", type=int, default=1000, help="serving: steps (or one-step times) per chunk")
+    ap.add_argument("--serve-chunk", type=int, default=1500, help="serving: steps (or one-step times) per chunk")
     ap.add_argument("--serve-time-budget", type=int, default=2,
                     help="serving engines with macro steps: a chunk is the time of chunk one-steps, a macro step "
                          "counting as this many (0: chunks of steps)")
